@@ -1,0 +1,184 @@
+/*
+ * snusolver.h -- C-ABI of the MI355X-native SnuSolver factor-and-solve path.
+ *
+ * One process, one GPU, one handle per matrix.  Plain pointers and sizes only.
+ * Every entry point returns an int status (SNU_OK == 0) and never exits the
+ * process; snu_last_error() returns the message of the last failure on the
+ * calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   snu_initialize          <- void initialize()            src/snusolver.h:5
+ *                              (src/solver.cu:35-49)
+ *   snumat_construct        <- SnuMat::SnuMat(csr_matrix, double *b, ...)
+ *                              src/SnuMat.cu:67-112 (ordering, structure,
+ *                              allocation; b is supplied at solve time here)
+ *   snumat_factorize +      <- SnuMat::solve(double *x)
+ *   snumat_solve               src/SnuMat_factorize.cu:213-357 (factorize with
+ *                              fused forward solve, then backward solve);
+ *                              split here so factors stay resident
+ *   snu_csr_t               <- csr_matrix  src/matrix.h:9-13 (same layout)
+ *   snu_solve_csr           <- void solve(csr_matrix, double *b, double *x)
+ *                              src/snusolver.h:6 (src/solver.cu:51-89)
+ * The C++ shims with the reference's exact initialize()/solve() signatures
+ * are declared in include/snusolver_compat.hpp.
+ */
+#ifndef SNUSOLVER_AMD_H
+#define SNUSOLVER_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SNU_OK 0
+#define SNU_ERR_BAD_INPUT -1     /* malformed matrix, options or ordering */
+#define SNU_ERR_STRUCTURE -2     /* ordering violates the separator property */
+#define SNU_ERR_ZERO_DIAG -3     /* structurally zero diagonal in a leaf */
+#define SNU_ERR_DUPLICATE -4     /* duplicate (row, col) entries */
+#define SNU_ERR_NO_DEVICE -5     /* no HIP device / HIP init failed */
+#define SNU_ERR_HIP -6           /* a HIP runtime call failed */
+#define SNU_ERR_OOM -7           /* device allocation or HBM budget exceeded */
+#define SNU_ERR_NOT_FACTORED -8  /* solve before factorize */
+#define SNU_ERR_NUMERIC -9       /* Inf/NaN in the factors (zero pivot) */
+#define SNU_ERR_NOT_CONVERGED -10
+
+/* ordering kinds */
+#define SNU_ORDER_ND 0    /* project nested dissection of A + A^T */
+#define SNU_ORDER_GRID 1  /* geometric nested dissection of an nx*ny*nz grid */
+#define SNU_ORDER_GIVEN 2 /* caller's order[n] / sizes[2np-1] (ParMETIS layout) */
+
+/* precision kinds */
+#define SNU_FP64 0
+
+/* Same field layout as the reference's csr_matrix (src/matrix.h:9-13):
+ * int32 n, m, nnz; rowptr[n+1]; colidx[nnz] (0-based, column-sorted rows);
+ * double data[nnz]. */
+typedef struct {
+  int n, m, nnz;
+  int *rowptr, *colidx;
+  double *data;
+} snu_csr_t;
+
+typedef struct {
+  int np;          /* leaves of the separator tree, power of two >= 1 */
+  int ordering;    /* SNU_ORDER_* */
+  int grid[3];     /* nx, ny, nz for SNU_ORDER_GRID (nz = 1 for 2D) */
+  const int *order;/* SNU_ORDER_GIVEN: order[i] = new index of vertex i */
+  const int *sizes;/* SNU_ORDER_GIVEN: 2np-1 block sizes, ParMETIS layout */
+  double eps;      /* leaf pivot clamp, reference src/param.h:3 (1e-15) */
+  int precision;   /* SNU_FP64 */
+  int device;      /* HIP device ordinal */
+  uint64_t seed;   /* ordering seed (the ND is deterministic for a seed) */
+} snu_opts_t;
+
+typedef struct snumat snumat_t;
+
+/* Fills opts with the defaults: np = 4, ND ordering, eps = 1e-15, FP64. */
+void snu_default_opts(snu_opts_t *opts);
+
+/* Selects the device and checks that it is usable (<- initialize()). */
+int snu_initialize(int device);
+
+/* Ordering, symbolic structure, device layout and upload of A's values.
+ * Host-only when device < 0 (structure analysis only; no GPU needed). */
+int snumat_construct(const snu_csr_t *A, const snu_opts_t *opts,
+                     snumat_t **out);
+
+/* Numeric factorization on the GPU from the resident copy of A's values. */
+int snumat_factorize(snumat_t *h);
+
+/* Replace A's values (same pattern) before the next factorize. */
+int snumat_update_values(snumat_t *h, const double *values);
+
+/* Solve A x = b for nrhs right-hand sides, column-major with leading
+ * dimension ld >= n.  Host pointers. */
+int snumat_solve(snumat_t *h, const double *b, double *x, int nrhs, int ld);
+
+/* Same with device pointers (resident in HBM, no PCIe in the path). */
+int snumat_solve_device(snumat_t *h, const double *d_b, double *d_x, int nrhs,
+                        int ld);
+
+int snumat_destroy(snumat_t *h);
+
+/* Reference-style one shot: construct + factorize + solve + destroy
+ * (<- void solve(csr_matrix, double *b, double *x)). */
+int snu_solve_csr(const snu_csr_t *A, const double *b, double *x,
+                  const snu_opts_t *opts);
+
+const char *snu_last_error(void);
+
+/* ---------------- introspection (tests, bench) ---------------- */
+/* info[0..15]: n, nnz, np, num_block, max_level, total leaf LU nnz,
+ * dense entries (int64 split lo/hi at [6],[7]), max core_n, max leaf_size,
+ * max boundary rows, max boundary cols, device(0/1), factored(0/1), 0... */
+int snumat_info(const snumat_t *h, int64_t *info16);
+/* block_start/block_size of e = 1..2np-1, written at [e-1]. */
+int snumat_blocks(const snumat_t *h, int *block_start, int *block_size);
+/* order[n] and sizes[2np-1] actually used. */
+int snumat_ordering(const snumat_t *h, int *order, int *sizes);
+/* PAP^T rowptr/colidx and perm_map (original nnz -> merged COO position). */
+int snumat_permuted(const snumat_t *h, int *pa_rowptr, int *pa_colidx,
+                    int *perm_map);
+/* leaf of rank r (heap block 2np-1-r): sizes[4] = leaf_size, core_n,
+ * core_nnz, lu_nnz. */
+int snumat_leaf_sizes(const snumat_t *h, int rank, int *sizes4);
+int snumat_leaf_struct(const snumat_t *h, int rank, int *lu_rowptr,
+                       int *lu_colidx, int *lu_diag, int *lu_bias,
+                       int *core_map);
+/* leaf LU values after factorize (copied back from the device). */
+int snumat_leaf_values(const snumat_t *h, int rank, double *lu_data);
+/* dense diagonal block of separator e (1 <= e < np), s_e x s_e col-major. */
+int snumat_sep_diag(const snumat_t *h, int e, double *out);
+/* debug: run the factorization up to a stage (1 = leaves and the boundary
+ * Schur assembly only, 2 = everything) and read a dense separator block
+ * (i, j) of one root path, s_i x s_j col-major. */
+int snumat_factorize_stage(snumat_t *h, int stage);
+int snumat_sep_block(const snumat_t *h, int i, int j, double *out);
+/* phase timings of the last factorize / solve in ms (GPU events):
+ * t[0] scatter, t[1] leaf rows, t[2] leaf separator rows + Schur,
+ * t[3] Schur assembly, t[4] dense levels, t[5] factorize total,
+ * t[6] solve total, t[7] dense GEMM kernel time (sum), t[8] dense GEMM
+ * launches, t[9] dense GEMM algorithmic flops (1e9), t[10..15] reserved. */
+int snumat_timings(const snumat_t *h, double *t16);
+/* algorithmic counts from the symbolic structure: c[0] F_dense (flops),
+ * c[1] F_leaf (flops), c[2] leaf factor bytes, c[3] solve bytes per rhs. */
+int snumat_counts(const snumat_t *h, double *c4);
+
+/* ---------------- host utilities ---------------- */
+/* Synthetic grid operators (seeded; see the design notes for the recipe).
+ * stencil: 5, 9 (2D), 7, 27 (3D).  recipe 0: convection-diffusion with
+ * off-diagonals -1 + 0.25 (u - 0.5) and the given diagonal; recipe 1:
+ * anisotropic 1:10:100 axis weights, +-10% perturbation, diagonal 1.01 x the
+ * off-diagonal row sum. */
+int snu_gen_grid_size(int nx, int ny, int nz, int stencil, int *n, int *nnz);
+int snu_gen_grid(int nx, int ny, int nz, int stencil, int recipe, double diag,
+                 uint64_t seed, int *rowptr, int *colidx, double *val);
+/* x_true ~ U[-1, 1) (seed), column-major n x nrhs; b = A x_true. */
+int snu_gen_rhs(const snu_csr_t *A, int nrhs, uint64_t seed, double *x_true,
+                double *b);
+/* Orderings in the ParMETIS contract (order[n], sizes[2np-1]). */
+int snu_order_grid(int nx, int ny, int nz, int np, int *order, int *sizes);
+int snu_order_nd(int n, const int *rowptr, const int *colidx, int np,
+                 uint64_t seed, int *order, int *sizes);
+/* Matrix Market: read header (n, m, nnz after symmetric expansion), then
+ * data into caller arrays (CSR, rows column-sorted). */
+int snu_mtx_size(const char *path, int *n, int *m, int *nnz, int *symmetric);
+int snu_mtx_read(const char *path, int *rowptr, int *colidx, double *val);
+int snu_mtx_write_vector(const char *path, const double *x, int n);
+/* relative backward error ||Ax-b||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
+double snu_backward_error(const snu_csr_t *A, const double *x,
+                          const double *b);
+
+/* ---------------- kernel-level test hooks (host pointers) ----------------
+ * Not part of the reference interface: they expose single kernels of this
+ * library to the parity tests. */
+/* C -= A B (assign = 0) or C = A B (assign = 1); col-major. */
+int snu_test_gemm(int M, int N, int K, const double *A, int lda,
+                  const double *B, int ldb, double *C, int ldc, int assign);
+/* in-place no-pivot LU of an nb x nb block (nb <= 64); Linv/Uinv: 64 x 64. */
+int snu_test_diag_lu(int nb, double *D, int ld, double *Linv, double *Uinv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

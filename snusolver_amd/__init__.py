@@ -1,0 +1,331 @@
+"""MI355X-native SnuSolver factor-and-solve path (Python host mirror).
+
+Thin ctypes layer over the C-ABI in ``include/snusolver.h`` (libsnusolver.so,
+built in-tree).  It mirrors the reference's operator interface for this path:
+
+* ``SnuMat(A, ...)``  -- construct: ordering, symbolic structure, device
+  layout (reference ``SnuMat::SnuMat``, src/SnuMat.cu:67-112)
+* ``SnuMat.factorize()`` / ``SnuMat.solve(b)`` -- the reference's fused
+  ``SnuMat::solve(double *x)`` (src/SnuMat_factorize.cu:213-357), split so the
+  factors stay resident in HBM
+* ``solve(A, b)`` -- the reference's ``void solve(csr_matrix, double*,
+  double*)`` (src/solver.cu:51-89)
+
+There is no CPU fallback: every numeric call runs the HIP kernels and raises
+``SnuError`` when the library or the GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+__all__ = [
+    "SnuError", "CSR", "SnuMat", "lib", "gen_grid", "gen_rhs", "order_grid",
+    "order_nd", "read_mtx", "backward_error", "solve", "config_matrix",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsnusolver.so")
+
+ORDER_ND, ORDER_GRID, ORDER_GIVEN = 0, 1, 2
+_STATUS = {
+    -1: "BAD_INPUT", -2: "STRUCTURE", -3: "ZERO_DIAG", -4: "DUPLICATE",
+    -5: "NO_DEVICE", -6: "HIP", -7: "OOM", -8: "NOT_FACTORED", -9: "NUMERIC",
+    -10: "NOT_CONVERGED",
+}
+
+
+class SnuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Csr(C.Structure):
+    _fields_ = [("n", C.c_int), ("m", C.c_int), ("nnz", C.c_int),
+                ("rowptr", C.POINTER(C.c_int)), ("colidx", C.POINTER(C.c_int)),
+                ("data", C.POINTER(C.c_double))]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("np", C.c_int), ("ordering", C.c_int), ("grid", C.c_int * 3),
+                ("order", C.POINTER(C.c_int)), ("sizes", C.POINTER(C.c_int)),
+                ("eps", C.c_double), ("precision", C.c_int), ("device", C.c_int),
+                ("seed", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libsnusolver.so (fails loudly if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SnuError(-5, f"{LIB_PATH} is missing; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.snu_last_error.restype = C.c_char_p
+        L.snu_backward_error.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise SnuError(rc, lib().snu_last_error().decode())
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclass
+class CSR:
+    """CSR matrix with the reference's csr_matrix fields (src/matrix.h:9-13)."""
+    n: int
+    rowptr: np.ndarray
+    colidx: np.ndarray
+    data: np.ndarray
+
+    def __post_init__(self):
+        self.rowptr = np.ascontiguousarray(self.rowptr, dtype=np.int32)
+        self.colidx = np.ascontiguousarray(self.colidx, dtype=np.int32)
+        self.data = np.ascontiguousarray(self.data, dtype=np.float64)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rowptr[-1])
+
+    def c(self) -> _Csr:
+        return _Csr(self.n, self.n, self.nnz, _ip(self.rowptr), _ip(self.colidx), _dp(self.data))
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        rows = np.repeat(np.arange(self.n), np.diff(self.rowptr))
+        x = np.asarray(x, dtype=np.float64)
+        if x.ndim == 1:
+            return np.bincount(rows, weights=self.data * x[self.colidx], minlength=self.n)
+        return np.stack([self.matvec(x[:, j]) for j in range(x.shape[1])], axis=1)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.data, self.colidx, self.rowptr), shape=(self.n, self.n))
+
+
+def gen_grid(nx: int, ny: int, nz: int, stencil: int, recipe: int = 0,
+             diag: float = 4.1, seed: int = 0) -> CSR:
+    """Seeded grid operator; see snu_gen.cpp for the recipe."""
+    n, nnz = C.c_int(), C.c_int()
+    _check(lib().snu_gen_grid_size(nx, ny, nz, stencil, C.byref(n), C.byref(nnz)))
+    rp = np.zeros(n.value + 1, np.int32)
+    ci = np.zeros(nnz.value, np.int32)
+    v = np.zeros(nnz.value, np.float64)
+    _check(lib().snu_gen_grid(nx, ny, nz, stencil, recipe, C.c_double(diag), C.c_uint64(seed),
+                              _ip(rp), _ip(ci), _dp(v)))
+    return CSR(n.value, rp, ci, v)
+
+
+def gen_rhs(A: CSR, nrhs: int = 1, seed: int = 1):
+    """x_true ~ U[-1,1) (seeded), b = A x_true; both n x nrhs (Fortran order)."""
+    xt = np.zeros((nrhs, A.n), np.float64)
+    b = np.zeros((nrhs, A.n), np.float64)
+    a = A.c()
+    _check(lib().snu_gen_rhs(C.byref(a), nrhs, C.c_uint64(seed), _dp(xt), _dp(b)))
+    return xt.T, b.T
+
+
+def order_grid(nx: int, ny: int, nz: int, np_: int):
+    order = np.zeros(nx * ny * nz, np.int32)
+    sizes = np.zeros(2 * np_ - 1, np.int32)
+    _check(lib().snu_order_grid(nx, ny, nz, np_, _ip(order), _ip(sizes)))
+    return order, sizes
+
+
+def order_nd(A: CSR, np_: int, seed: int = 0):
+    order = np.zeros(A.n, np.int32)
+    sizes = np.zeros(2 * np_ - 1, np.int32)
+    _check(lib().snu_order_nd(A.n, _ip(A.rowptr), _ip(A.colidx), np_, C.c_uint64(seed),
+                              _ip(order), _ip(sizes)))
+    return order, sizes
+
+
+def read_mtx(path: str) -> CSR:
+    n, m, nnz, sym = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    p = path.encode()
+    _check(lib().snu_mtx_size(p, C.byref(n), C.byref(m), C.byref(nnz), C.byref(sym)))
+    if n.value != m.value:
+        raise SnuError(-1, "matrix is not square")
+    rp = np.zeros(n.value + 1, np.int32)
+    ci = np.zeros(nnz.value, np.int32)
+    v = np.zeros(nnz.value, np.float64)
+    _check(lib().snu_mtx_read(p, _ip(rp), _ip(ci), _dp(v)))
+    return CSR(n.value, rp, ci, v)
+
+
+def backward_error(A: CSR, x: np.ndarray, b: np.ndarray) -> float:
+    """||Ax - b||_inf / (||A||_inf ||x||_inf + ||b||_inf), per column max."""
+    x = np.asarray(x, np.float64)
+    b = np.asarray(b, np.float64)
+    if x.ndim == 2:
+        return max(backward_error(A, x[:, j], b[:, j]) for j in range(x.shape[1]))
+    r = A.matvec(x) - b
+    anorm = np.max(np.bincount(np.repeat(np.arange(A.n), np.diff(A.rowptr)),
+                               weights=np.abs(A.data), minlength=A.n))
+    den = anorm * np.max(np.abs(x)) + np.max(np.abs(b))
+    return float(np.max(np.abs(r)) / den) if den > 0 else float(np.max(np.abs(r)))
+
+
+class SnuMat:
+    """construct -> factorize -> solve on one GPU (device=-1: host analysis only)."""
+
+    def __init__(self, A: CSR, np_: int = 4, ordering: str = "nd", grid=None,
+                 order=None, sizes=None, device: int = 0, eps: float = 1e-15, seed: int = 0):
+        self.A = A
+        o = _Opts()
+        lib().snu_default_opts(C.byref(o))
+        o.np = np_
+        o.eps = eps
+        o.device = device
+        o.seed = seed
+        self._keep = []
+        if order is not None:
+            o.ordering = ORDER_GIVEN
+            order = np.ascontiguousarray(order, np.int32)
+            sizes = np.ascontiguousarray(sizes, np.int32)
+            self._keep += [order, sizes]
+            o.order, o.sizes = _ip(order), _ip(sizes)
+        elif ordering == "grid" or grid is not None:
+            o.ordering = ORDER_GRID
+            g = list(grid) + [1] * (3 - len(grid))
+            o.grid[0], o.grid[1], o.grid[2] = g
+        else:
+            o.ordering = ORDER_ND
+        self.np = np_
+        self._h = C.c_void_p()
+        a = A.c()
+        _check(lib().snumat_construct(C.byref(a), C.byref(o), C.byref(self._h)))
+
+    # -- lifecycle --
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().snumat_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- numerics --
+    def factorize(self):
+        _check(lib().snumat_factorize(self._h))
+        return self
+
+    def update_values(self, values: np.ndarray):
+        v = np.ascontiguousarray(values, np.float64)
+        _check(lib().snumat_update_values(self._h, _dp(v)))
+
+    def solve(self, b: np.ndarray) -> np.ndarray:
+        b = np.asarray(b, np.float64)
+        one = b.ndim == 1
+        B = np.asfortranarray(b.reshape(self.A.n, -1))
+        X = np.zeros_like(B, order="F")
+        _check(lib().snumat_solve(self._h, _dp(B), _dp(X), B.shape[1], self.A.n))
+        return X[:, 0].copy() if one else X
+
+    def solve_device_ptr(self, d_b: int, d_x: int, nrhs: int):
+        _check(lib().snumat_solve_device(self._h, C.c_void_p(d_b), C.c_void_p(d_x), nrhs, self.A.n))
+
+    # -- introspection --
+    def info(self) -> dict:
+        a = (C.c_int64 * 16)()
+        _check(lib().snumat_info(self._h, a))
+        keys = ["n", "nnz", "np", "num_block", "max_level", "leaf_lu_nnz", "dense_entries",
+                "max_core_n", "max_leaf_size", "max_boundary_rows", "max_boundary_cols",
+                "on_device", "factored", "device_bytes", "waves", "boundary_targets"]
+        return {k: int(a[i]) for i, k in enumerate(keys)}
+
+    def blocks(self):
+        nb = 2 * self.np - 1
+        bs, sz = np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+        _check(lib().snumat_blocks(self._h, _ip(bs), _ip(sz)))
+        return bs, sz
+
+    def ordering(self):
+        order, sizes = np.zeros(self.A.n, np.int32), np.zeros(2 * self.np - 1, np.int32)
+        _check(lib().snumat_ordering(self._h, _ip(order), _ip(sizes)))
+        return order, sizes
+
+    def permuted(self):
+        rp = np.zeros(self.A.n + 1, np.int32)
+        ci = np.zeros(self.A.nnz, np.int32)
+        pm = np.zeros(self.A.nnz, np.int32)
+        _check(lib().snumat_permuted(self._h, _ip(rp), _ip(ci), _ip(pm)))
+        return rp, ci, pm
+
+    def leaf_sizes(self, rank: int):
+        s = np.zeros(4, np.int32)
+        _check(lib().snumat_leaf_sizes(self._h, rank, _ip(s)))
+        return [int(v) for v in s]
+
+    def leaf_struct(self, rank: int):
+        ls, cn, cnz, lnz = self.leaf_sizes(rank)
+        rp, ci = np.zeros(cn + 1, np.int32), np.zeros(lnz, np.int32)
+        dg, bias = np.zeros(cn, np.int32), np.zeros(ls, np.int32)
+        cm = np.zeros(cnz, np.int32)
+        _check(lib().snumat_leaf_struct(self._h, rank, _ip(rp), _ip(ci), _ip(dg), _ip(bias), _ip(cm)))
+        return rp, ci, dg, bias, cm
+
+    def leaf_values(self, rank: int) -> np.ndarray:
+        lnz = self.leaf_sizes(rank)[3]
+        v = np.zeros(lnz, np.float64)
+        _check(lib().snumat_leaf_values(self._h, rank, _dp(v)))
+        return v
+
+    def sep_diag(self, e: int) -> np.ndarray:
+        s = int(self.blocks()[1][e - 1])
+        out = np.zeros(s * s, np.float64)
+        _check(lib().snumat_sep_diag(self._h, e, _dp(out)))
+        return out.reshape(s, s, order="F")
+
+    def timings(self) -> dict:
+        t = (C.c_double * 16)()
+        _check(lib().snumat_timings(self._h, t))
+        keys = ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms",
+                "factor_ms", "solve_ms", "gemm_ms", "gemm_launches", "gemm_gflop"]
+        return {k: float(t[i]) for i, k in enumerate(keys)}
+
+    def counts(self) -> dict:
+        c = (C.c_double * 4)()
+        _check(lib().snumat_counts(self._h, c))
+        return {"f_dense": c[0], "f_leaf": c[1], "b_leaf": c[2], "b_solve": c[3]}
+
+
+def solve(A: CSR, b: np.ndarray, np_: int = 4, **kw) -> np.ndarray:
+    """One-shot construct + factorize + solve (reference ``solve``)."""
+    with SnuMat(A, np_, **kw) as M:
+        return M.factorize().solve(b)
+
+
+def config_matrix(name: str):
+    """Matrices of the named workloads: returns (A, grid dims, np)."""
+    if name == "c1":      # 20x20 2D 5-point, diag 4.1, seed 0; single process, np = 4
+        return gen_grid(20, 20, 1, 5, 0, 4.1, 0), (20, 20, 1), 4
+    if name == "c2":      # 64^3 7-point, diag 6.1, seed 0; leaves of ~2k rows
+        return gen_grid(64, 64, 64, 7, 0, 6.1, 0), (64, 64, 64), 128
+    if name == "c3":      # 96^3 27-point anisotropic 1:10:100, seed 0
+        return gen_grid(96, 96, 96, 27, 1, 0.0, 0), (96, 96, 96), 512
+    raise ValueError(name)

@@ -1,0 +1,458 @@
+// Structure analysis of the construct phase (host, integer work).
+//
+// Produces, bit-exactly, the reference's integer structures for one process
+// that owns every leaf:
+//   block layout        calculate_block           SnuMat_construct.cu:21-38
+//   PAP^T + perm_map    permutate                 SnuMat_construct.cu:40-80
+//   merged COO layout   construct/setGrid/addElement  SnuMat_construct.cu:81-159
+//   per-leaf core CSR   core_preprocess           kernel_sparse.cu:526-573
+//   symbolic LU         core_symbfact             kernel_sparse.cu:368-524
+//   core_map, LU_bias   core_preprocess           kernel_sparse.cu:576-597
+// and the MI355X layout: one copy of every separator block (no per-rank
+// replication), 64-bit offsets, the leaf value arena, the map from each
+// entry of A to its slot, and per-leaf level sets for the GPU kernels.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+#include "snu_internal.h"
+
+namespace snu {
+
+int64_t Plan::sep_index(int bi, int lr, int bj, int lc) const {
+  if (bi == bj) return offR[bi] + lr + (int64_t)lc * block_size[bi];
+  if (bi > bj)  // row block deeper: U panel of bi
+    return offR[bi] + lr + (int64_t)(block_size[bi] + anc_off(bi, bj) + lc) * block_size[bi];
+  return offL[bj] + (anc_off(bj, bi) + lr) + (int64_t)lc * anc_sum[bj];
+}
+
+namespace {
+
+int ilog2(int v) {
+  int l = 0;
+  while (v > 1) v >>= 1, l++;
+  return l;
+}
+
+void dfs_block_order(std::vector<int> &v, int cur, int parts) {
+  if (parts > 1) {
+    dfs_block_order(v, cur * 2 + 1, parts / 2);
+    dfs_block_order(v, cur * 2, parts / 2);
+  }
+  v.push_back(cur);
+}
+
+// core_symbfact (kernel_sparse.cu:368-524): filled pattern of every core row;
+// separator rows close over leaf columns only.
+void symbfact(LeafPlan &L) {
+  int core_n = L.core_n, ls = L.leaf_size;
+  L.lu_rowptr.assign(core_n + 1, 0);
+  L.lu_diag.assign(core_n, 0);
+  L.lu_colidx.clear();
+  L.lu_colidx.reserve((size_t)L.core_nnz * 4 + 16);
+  std::vector<char> vst(core_n + 1, 0);
+  std::vector<int> Lj, Uj;
+  Lj.reserve(core_n);
+  Uj.reserve(core_n);
+  for (int j = 0; j < core_n; j++) {
+    Lj.clear();
+    Uj.clear();
+    for (int i = L.core_rowptr[j]; i < L.core_rowptr[j + 1]; i++) {
+      int h = L.core_colidx[i];
+      if (h < j) Lj.push_back(h);
+      else Uj.push_back(h);
+      vst[h] = 1;
+    }
+    for (size_t idx = 0; idx < Lj.size(); idx++) {
+      int i = Lj[idx];
+      for (int u = L.lu_diag[i]; u < L.lu_rowptr[i + 1]; u++) {
+        int h = L.lu_colidx[u];
+        if (j >= ls && h >= ls) break;
+        if (vst[h]) continue;
+        vst[h] = 1;
+        if (h < j) Lj.push_back(h);
+        else Uj.push_back(h);
+      }
+    }
+    std::sort(Lj.begin(), Lj.end());
+    std::sort(Uj.begin(), Uj.end());
+    for (int h : Lj) L.lu_colidx.push_back(h), vst[h] = 0;
+    L.lu_diag[j] = (int)L.lu_colidx.size();
+    for (int h : Uj) L.lu_colidx.push_back(h), vst[h] = 0;
+    L.lu_rowptr[j + 1] = (int)L.lu_colidx.size();
+  }
+  L.lu_nnz = (int)L.lu_colidx.size();
+}
+
+int build_leaf(const Plan &P, LeafPlan &L) {
+  const int leaf = L.leaf;
+  L.npath = 0;
+  for (int i = leaf; i >= 1; i /= 2) L.npath++;
+  L.path.resize(L.npath);
+  L.cstart.resize(L.npath + 1);
+  int sum = 0;
+  for (int k = 0; k < L.npath; k++) {
+    L.path[k] = leaf >> k;
+    L.cstart[k] = sum;
+    sum += P.block_size[leaf >> k];
+  }
+  L.cstart[L.npath] = sum;
+  L.core_n = sum;
+  L.leaf_size = P.block_size[leaf];
+  L.coo_off = P.mstart[leaf];
+  L.core_nnz = P.msize[leaf];
+  const int core_n = L.core_n, ls = L.leaf_size;
+  // core blocks in merge order: (leaf,leaf), (leaf,p1), (p1,leaf), ...
+  struct Cb { int bi, bj, ki, kj; };
+  std::vector<Cb> cb;
+  cb.push_back({leaf, leaf, 0, 0});
+  for (int k = 1; k < L.npath; k++) {
+    cb.push_back({leaf, leaf >> k, 0, k});
+    cb.push_back({leaf >> k, leaf, k, 0});
+  }
+  // core CSR: rows keep the merge order, which is column-ascending
+  L.core_rowptr.assign(core_n + 1, 0);
+  L.core_colidx.assign(L.core_nnz, 0);
+  for (const Cb &e : cb) {
+    int g = P.grid_id(e.bi, e.bj);
+    for (int p = 0; p < P.gnnz[g]; p++) {
+      int q = P.goff[g] + p;
+      L.core_rowptr[P.coo_r[q] - P.block_start[e.bi] + L.cstart[e.ki] + 1]++;
+    }
+  }
+  for (int i = 0; i < core_n; i++) L.core_rowptr[i + 1] += L.core_rowptr[i];
+  {
+    std::vector<int> fillp(L.core_rowptr.begin(), L.core_rowptr.end() - 1);
+    for (const Cb &e : cb) {
+      int g = P.grid_id(e.bi, e.bj);
+      for (int p = 0; p < P.gnnz[g]; p++) {
+        int q = P.goff[g] + p;
+        int r = P.coo_r[q] - P.block_start[e.bi] + L.cstart[e.ki];
+        int c = P.coo_c[q] - P.block_start[e.bj] + L.cstart[e.kj];
+        L.core_colidx[fillp[r]++] = c;
+      }
+    }
+  }
+  symbfact(L);
+  for (int i = 0; i < ls; i++)
+    if (L.lu_diag[i] >= L.lu_rowptr[i + 1] || L.lu_colidx[L.lu_diag[i]] != i)
+      return set_error(SNU_ERR_ZERO_DIAG, "structurally zero diagonal in leaf " + std::to_string(leaf));
+  // core_map: merged-COO entry of the leaf -> LU slot
+  L.core_map.assign(L.core_nnz, 0);
+  {
+    std::vector<int> cur(L.lu_rowptr.begin(), L.lu_rowptr.end() - 1);
+    for (const Cb &e : cb) {
+      int g = P.grid_id(e.bi, e.bj);
+      for (int p = 0; p < P.gnnz[g]; p++) {
+        int q = P.goff[g] + p;
+        int r = P.coo_r[q] - P.block_start[e.bi] + L.cstart[e.ki];
+        int c = P.coo_c[q] - P.block_start[e.bj] + L.cstart[e.kj];
+        while (L.lu_colidx[cur[r]] != c) cur[r]++;
+        L.core_map[q - L.coo_off] = cur[r];
+      }
+    }
+  }
+  L.lu_bias.assign(ls, 0);
+  for (int i = 0; i < ls; i++) {
+    int ptr = L.lu_rowptr[i];
+    while (ptr < L.lu_rowptr[i + 1] && L.lu_colidx[ptr] < ls) ptr++;
+    L.lu_bias[i] = ptr;
+  }
+  // boundary rows / cols and the workspace column of every LU entry
+  int nsep = core_n - ls;
+  std::vector<int> binv(nsep, -1);
+  for (int r = 0; r < ls; r++)
+    for (int p = L.lu_bias[r]; p < L.lu_rowptr[r + 1]; p++) binv[L.lu_colidx[p] - ls] = 0;
+  L.bcol.clear();
+  for (int c = 0; c < nsep; c++)
+    if (binv[c] == 0) binv[c] = (int)L.bcol.size(), L.bcol.push_back(c + ls);
+  L.nbc = (int)L.bcol.size();
+  L.brow.clear();
+  for (int r = ls; r < core_n; r++)
+    if (L.lu_rowptr[r + 1] > L.lu_rowptr[r]) L.brow.push_back(r);
+  L.nbr = (int)L.brow.size();
+  L.wcol.resize(L.lu_nnz);
+  for (int p = 0; p < L.lu_nnz; p++) {
+    int c = L.lu_colidx[p];
+    L.wcol[p] = c < ls ? c : ls + binv[c - ls];
+  }
+  // level sets of the leaf rows: factor/forward (L deps), backward (U deps)
+  std::vector<int> lev(ls, 0);
+  int nlev = 0;
+  for (int j = 0; j < ls; j++) {
+    int m = 0;
+    for (int p = L.lu_rowptr[j]; p < L.lu_diag[j]; p++) m = std::max(m, lev[L.lu_colidx[p]] + 1);
+    lev[j] = m;
+    nlev = std::max(nlev, m + 1);
+  }
+  L.lev_ptr.assign(nlev + 1, 0);
+  for (int j = 0; j < ls; j++) L.lev_ptr[lev[j] + 1]++;
+  for (int l = 0; l < nlev; l++) L.lev_ptr[l + 1] += L.lev_ptr[l];
+  L.lev_rows.resize(ls);
+  {
+    std::vector<int> f(L.lev_ptr.begin(), L.lev_ptr.end() - 1);
+    for (int j = 0; j < ls; j++) L.lev_rows[f[lev[j]]++] = j;
+  }
+  std::vector<int> ulev(ls, 0);
+  int nulev = 0;
+  for (int j = ls - 1; j >= 0; j--) {
+    int m = 0;
+    for (int p = L.lu_diag[j] + 1; p < L.lu_bias[j]; p++) m = std::max(m, ulev[L.lu_colidx[p]] + 1);
+    ulev[j] = m;
+    nulev = std::max(nulev, m + 1);
+  }
+  L.ulev_ptr.assign(nulev + 1, 0);
+  for (int j = 0; j < ls; j++) L.ulev_ptr[ulev[j] + 1]++;
+  for (int l = 0; l < nulev; l++) L.ulev_ptr[l + 1] += L.ulev_ptr[l];
+  L.ulev_rows.resize(ls);
+  {
+    std::vector<int> f(L.ulev_ptr.begin(), L.ulev_ptr.end() - 1);
+    for (int j = ls - 1; j >= 0; j--) L.ulev_rows[f[ulev[j]]++] = j;
+  }
+  // structural flops: every pivot of a row costs 1 division and 2 flops per
+  // U entry of the pivot row it applies
+  double fl = 0;
+  for (int i = 0; i < core_n; i++) {
+    int lend = i < ls ? L.lu_diag[i] : L.lu_rowptr[i + 1];
+    for (int p = L.lu_rowptr[i]; p < lend; p++) {
+      int k = L.lu_colidx[p];
+      fl += 1.0 + 2.0 * (L.lu_rowptr[k + 1] - L.lu_diag[k] - 1);
+    }
+  }
+  L.flops = fl;
+  return SNU_OK;
+}
+
+template <class F>
+void parallel_for(int n, F f) {
+  int nt = (int)std::thread::hardware_concurrency();
+  if (nt < 1) nt = 1;
+  if (nt > 16) nt = 16;
+  if (nt > n) nt = n;
+  if (nt <= 1) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::atomic<int> next(0);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([&]() {
+      for (;;) {
+        int i = next++;
+        if (i >= n) break;
+        f(i);
+      }
+    });
+  for (auto &t : th) t.join();
+}
+
+}  // namespace
+
+int analyze(const snu_csr_t *A, const snu_opts_t *opts, Plan &P) {
+  if (!A || !opts) return set_error(SNU_ERR_BAD_INPUT, "null matrix or options");
+  const int n = A->n, nnz = A->nnz, np = opts->np;
+  if (n < 1 || A->m != n) return set_error(SNU_ERR_BAD_INPUT, "matrix must be square with n >= 1");
+  if (nnz < 0 || !A->rowptr || (nnz && (!A->colidx || !A->data)))
+    return set_error(SNU_ERR_BAD_INPUT, "null CSR arrays");
+  if (np < 1 || (np & (np - 1)) || np > (1 << 20)) return set_error(SNU_ERR_BAD_INPUT, "np must be a power of two");
+  if (A->rowptr[0] != 0 || A->rowptr[n] != nnz) return set_error(SNU_ERR_BAD_INPUT, "rowptr inconsistent with nnz");
+  for (int i = 0; i < n; i++) {
+    if (A->rowptr[i + 1] < A->rowptr[i]) return set_error(SNU_ERR_BAD_INPUT, "rowptr not monotone");
+    for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; q++) {
+      if (A->colidx[q] < 0 || A->colidx[q] >= n) return set_error(SNU_ERR_BAD_INPUT, "column index out of range");
+      if (q > A->rowptr[i] && A->colidx[q] <= A->colidx[q - 1])
+        return set_error(A->colidx[q] == A->colidx[q - 1] ? SNU_ERR_DUPLICATE : SNU_ERR_BAD_INPUT,
+                         A->colidx[q] == A->colidx[q - 1] ? "duplicate entry in a row" : "rows must be column-sorted");
+    }
+  }
+  P.n = n; P.nnz = nnz; P.np = np;
+  P.num_block = 2 * np - 1;
+  P.max_level = ilog2(np);
+  P.nk = P.max_level + 1;
+  const int nb = P.num_block;
+  // ---- ordering (the ParMETIS contract) ----
+  P.order.assign(n, 0);
+  P.sizes.assign(nb, 0);
+  int rc = SNU_OK;
+  if (opts->ordering == SNU_ORDER_GIVEN) {
+    if (!opts->order || !opts->sizes) return set_error(SNU_ERR_BAD_INPUT, "given ordering without arrays");
+    std::copy(opts->order, opts->order + n, P.order.begin());
+    std::copy(opts->sizes, opts->sizes + nb, P.sizes.begin());
+  } else if (opts->ordering == SNU_ORDER_GRID) {
+    long long cells = (long long)opts->grid[0] * opts->grid[1] * opts->grid[2];
+    if (cells != n) return set_error(SNU_ERR_BAD_INPUT, "grid dims do not match n");
+    rc = order_grid(opts->grid[0], opts->grid[1], opts->grid[2], np, P.order.data(), P.sizes.data());
+  } else if (opts->ordering == SNU_ORDER_ND) {
+    rc = order_nd(n, A->rowptr, A->colidx, np, opts->seed, P.order.data(), P.sizes.data());
+  } else {
+    return set_error(SNU_ERR_BAD_INPUT, "unknown ordering kind");
+  }
+  if (rc) return rc;
+  {
+    std::vector<char> seen(n, 0);
+    long long s = 0;
+    for (int i = 0; i < n; i++) {
+      int o = P.order[i];
+      if (o < 0 || o >= n || seen[o]) return set_error(SNU_ERR_BAD_INPUT, "order is not a permutation");
+      seen[o] = 1;
+    }
+    for (int e = 0; e < nb; e++) {
+      if (P.sizes[e] < 0) return set_error(SNU_ERR_BAD_INPUT, "negative block size");
+      s += P.sizes[e];
+    }
+    if (s != n) return set_error(SNU_ERR_BAD_INPUT, "block sizes do not sum to n");
+  }
+  // ---- calculate_block ----
+  P.block_start.assign(nb + 1, 0);
+  P.block_size.assign(nb + 1, 0);
+  P.level.assign(nb + 1, 0);
+  {
+    std::vector<int> bo;
+    dfs_block_order(bo, 1, np);
+    int cur = 0;
+    for (int e : bo) {
+      P.block_start[e] = cur;
+      P.block_size[e] = P.sizes[np + np - 1 - e];
+      cur += P.block_size[e];
+    }
+    for (int e = 1; e <= nb; e++) P.level[e] = ilog2(e);
+  }
+  // ---- permutate: PAP^T rows sorted by new column; perm_map ----
+  P.pa_rowptr.assign(n + 1, 0);
+  P.pa_colidx.assign(nnz, 0);
+  P.perm_map.assign(nnz, 0);
+  for (int i = 0; i < n; i++) P.pa_rowptr[P.order[i] + 1] = A->rowptr[i + 1] - A->rowptr[i];
+  for (int i = 0; i < n; i++) P.pa_rowptr[i + 1] += P.pa_rowptr[i];
+  {
+    std::vector<std::pair<int, int>> v;
+    for (int i = 0; i < n; i++) {
+      v.clear();
+      for (int j = A->rowptr[i]; j < A->rowptr[i + 1]; j++) v.push_back({P.order[A->colidx[j]], j});
+      std::sort(v.begin(), v.end());
+      int base = P.pa_rowptr[P.order[i]];
+      for (size_t jj = 0; jj < v.size(); jj++) {
+        P.pa_colidx[base + jj] = v[jj].first;
+        P.perm_map[v[jj].second] = base + (int)jj;
+      }
+    }
+  }
+  // ---- construct: merged COO layout ----
+  std::vector<int> block(n);
+  for (int b = 1; b <= nb; b++)
+    for (int i = P.block_start[b]; i < P.block_start[b] + P.block_size[b]; i++) block[i] = b;
+  for (int r = 0; r < n; r++)
+    for (int q = P.pa_rowptr[r]; q < P.pa_rowptr[r + 1]; q++) {
+      int x = block[r], y = block[P.pa_colidx[q]];
+      if (x < y) std::swap(x, y);
+      while (x > y) x /= 2;
+      if (x != y) return set_error(SNU_ERR_STRUCTURE, "ordering violates the separator property");
+    }
+  const int ng = (nb + 1) * P.nk * 2;
+  P.gnnz.assign(ng, 0);
+  P.goff.assign(ng, 0);
+  P.mstart.assign(nb + 1, 0);
+  P.msize.assign(nb + 1, 0);
+  for (int r = 0; r < n; r++)
+    for (int q = P.pa_rowptr[r]; q < P.pa_rowptr[r + 1]; q++)
+      P.gnnz[P.grid_id(block[r], block[P.pa_colidx[q]])]++;
+  {
+    std::vector<int> gend(ng, 0);
+    int bias = 0;
+    auto set = [&](int i, int j) {
+      int g = P.grid_id(i, j);
+      P.goff[g] = bias;
+      bias += P.gnnz[g];
+      gend[g] = bias;
+    };
+    for (int i = nb; i >= 1; i--) {
+      P.mstart[i] = bias;
+      set(i, i);
+      for (int ii = i / 2; ii; ii /= 2) set(i, ii), set(ii, i);
+      P.msize[i] = bias - P.mstart[i];
+    }
+    P.coo_r.assign(nnz, 0);
+    P.coo_c.assign(nnz, 0);
+    std::vector<int> cmap(nnz);
+    for (int r = n - 1; r >= 0; r--)
+      for (int idx = P.pa_rowptr[r + 1] - 1; idx >= P.pa_rowptr[r]; idx--) {
+        int c = P.pa_colidx[idx];
+        int pos = --gend[P.grid_id(block[r], block[c])];
+        P.coo_r[pos] = r;
+        P.coo_c[pos] = c;
+        cmap[idx] = pos;
+      }
+    for (int i = 0; i < nnz; i++) P.perm_map[i] = cmap[P.perm_map[i]];
+  }
+  // ---- per-leaf structures (leaf-parallel) ----
+  P.leaves.assign(np, LeafPlan());
+  std::vector<int> rcs(np, SNU_OK);
+  std::vector<std::string> msgs(np);
+  parallel_for(np, [&](int r) {
+    LeafPlan &L = P.leaves[r];
+    L.rank = r;
+    L.leaf = np + np - 1 - r;  // SnuMat_distribute_structure.cu:142
+    rcs[r] = build_leaf(P, L);
+  });
+  for (int r = 0; r < np; r++)
+    if (rcs[r]) return set_error(rcs[r], "structurally zero diagonal in leaf of rank " + std::to_string(r));
+  // ---- dense single-copy layout ----
+  P.anc_sum.assign(nb + 1, 0);
+  P.offR.assign(nb + 1, 0);
+  P.offL.assign(nb + 1, 0);
+  int64_t off = 0;
+  for (int p = 1; p < np; p++) {
+    int a = 0;
+    for (int q = p / 2; q; q /= 2) a += P.block_size[q];
+    P.anc_sum[p] = a;
+    int64_t s = P.block_size[p];
+    P.offR[p] = off;
+    off += s * (s + a);
+    P.offL[p] = off;
+    off += (int64_t)a * s;
+  }
+  P.dense_total = off;
+  P.leaf_off.assign(np + 1, 0);
+  for (int r = 0; r < np; r++) P.leaf_off[r + 1] = P.leaf_off[r] + P.leaves[r].lu_nnz;
+  P.leaf_total = P.leaf_off[np];
+  // ---- destination of every entry of A ----
+  {
+    std::vector<int64_t> dest_q(nnz, -1);
+    for (int i = nb; i >= 1; i--) {
+      for (int q = P.mstart[i]; q < P.mstart[i] + P.msize[i]; q++) {
+        if (i >= np) {
+          const LeafPlan &L = P.leaves[np + np - 1 - i];
+          dest_q[q] = P.leaf_off[L.rank] + L.core_map[q - L.coo_off];
+        } else {
+          int bi = block[P.coo_r[q]], bj = block[P.coo_c[q]];
+          dest_q[q] = P.leaf_total + P.sep_index(bi, P.coo_r[q] - P.block_start[bi], bj,
+                                                 P.coo_c[q] - P.block_start[bj]);
+        }
+      }
+    }
+    P.a_dest.assign(nnz, 0);
+    for (int k = 0; k < nnz; k++) P.a_dest[k] = dest_q[P.perm_map[k]];
+  }
+  // ---- algorithmic counts ----
+  double fd = 0, bsolve = 0, fl = 0, bleaf = 0;
+  for (int p = 1; p < np; p++) {
+    double s = P.block_size[p], a = P.anc_sum[p];
+    fd += 2.0 / 3.0 * s * s * s + 2.0 * s * s * a + 2.0 * s * a * a;
+    bsolve += 8.0 * (s * s + 2.0 * s * a);
+  }
+  for (const LeafPlan &L : P.leaves) {
+    fl += L.flops;
+    bleaf += 8.0 * L.core_nnz + 16.0 * L.lu_nnz + 4.0 * (L.lu_nnz + L.core_n) +
+             16.0 * (double)L.nbr * L.nbc;
+    bsolve += 12.0 * L.lu_nnz;
+  }
+  bsolve += 16.0 * n;
+  P.f_dense = fd;
+  P.f_leaf = fl;
+  P.b_leaf = bleaf;
+  P.b_solve = bsolve;
+  return SNU_OK;
+}
+
+}  // namespace snu

@@ -1,0 +1,77 @@
+// Internal host-side types of the MI355X SnuSolver path.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/snusolver.h"
+
+namespace snu {
+
+int set_error(int code, const std::string &msg);
+uint64_t splitmix64(uint64_t &s);
+double u01(uint64_t &s);
+
+// Per-leaf symbolic structure (one leaf = one rank of the reference).
+// "core" = leaf rows followed by the ancestor separators bottom-up
+// (kernel_sparse.cu:526-531).
+struct LeafPlan {
+  int rank = 0, leaf = 0, leaf_size = 0, core_n = 0, npath = 0;
+  std::vector<int> path;    // path[k] = leaf >> k
+  std::vector<int> cstart;  // core-local start of path[k], npath + 1 entries
+  int core_nnz = 0, coo_off = 0;
+  std::vector<int> core_rowptr, core_colidx, core_map;
+  int lu_nnz = 0;
+  std::vector<int> lu_rowptr, lu_colidx, lu_diag, lu_bias;
+  // GPU-side views
+  std::vector<int> wcol;    // workspace column of each LU entry
+  int nbr = 0, nbc = 0;     // boundary rows (non-empty L_SL) / cols (U_LS)
+  std::vector<int> brow;    // core rows of the boundary rows
+  std::vector<int> bcol;    // core cols of the boundary cols
+  std::vector<int> lev_ptr, lev_rows;    // factor / forward level sets
+  std::vector<int> ulev_ptr, ulev_rows;  // backward level sets
+  double flops = 0;         // structural leaf flops (LU + Schur)
+};
+
+struct Plan {
+  int n = 0, nnz = 0, np = 0, num_block = 0, max_level = 0, nk = 0;
+  std::vector<int> order, sizes;
+  std::vector<int> block_start, block_size, level;  // index 0..num_block
+  std::vector<int> pa_rowptr, pa_colidx, perm_map;
+  std::vector<int> coo_r, coo_c;
+  std::vector<int> gnnz, goff;    // merged COO layout per grid id
+  std::vector<int> mstart, msize; // merge region of each block
+  std::vector<LeafPlan> leaves;   // by rank
+  // dense single-copy separator storage:
+  //   R_p = [D_p | Up_p], s_p x (s_p + A_p), ld s_p (ancestors bottom-up)
+  //   Lp_p, A_p x s_p, ld A_p
+  std::vector<int> anc_sum;          // A_p
+  std::vector<int64_t> offR, offL;   // per separator, in the dense arena
+  int64_t dense_total = 0;
+  std::vector<int64_t> leaf_off;     // per rank, in the leaf value arena
+  int64_t leaf_total = 0;
+  std::vector<int64_t> a_dest;       // per original nnz: value arena index
+                                     // (leaf arena, then dense arena)
+  int grid_id(int i, int j) const {
+    int x = i > j ? i : j, y = i < j ? i : j, k = 0;
+    for (int jj = x; jj > y; jj /= 2) k++;
+    return (x * nk + k) * 2 + (i >= j ? 0 : 1);
+  }
+  // offset of ancestor q inside p's ancestor list (bottom-up)
+  int anc_off(int p, int q) const {
+    int s = 0;
+    for (int a = p / 2; a > q; a /= 2) s += block_size[a];
+    return s;
+  }
+  // value-arena index of separator entry (bi, lr) x (bj, lc)
+  int64_t sep_index(int bi, int lr, int bj, int lc) const;
+  double f_dense = 0, f_leaf = 0, b_leaf = 0, b_solve = 0;
+};
+
+int analyze(const snu_csr_t *A, const snu_opts_t *opts, Plan &P);
+
+int order_grid(int nx, int ny, int nz, int np, int *order, int *sizes);
+int order_nd(int n, const int *rowptr, const int *colidx, int np,
+             uint64_t seed, int *order, int *sizes);
+
+}  // namespace snu
