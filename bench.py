@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Benchmark of the factor-and-solve path on one MI355X.
+
+Workload (default): 64^3 7-point convection-diffusion operator, n = 262,144,
+nnz = 1,810,432, diagonal 6.1, off-diagonals -1 + 0.25 (u - 0.5) (seed 0),
+b = A x_true with x_true ~ U[-1,1) (seed 1), FP64, 128 nested-dissection
+leaves of ~1.6-2k rows (geometric ordering).  One step = one numeric
+factorization from A's values resident in HBM plus one solve with b resident
+in HBM (the reference's "Factsolve", src/solver.cu:68-76).  Construct
+(ordering, symbolic analysis, upload) is outside the timed region.
+
+Prints one JSON line.  Besides the contract fields it reports the factor and
+solve times, solves/s with resident factors (1 and 16 right-hand sides), the
+backward error and the relative difference to the CPU oracle, the roofline
+of the dominant kernel (the grouped FP64 MFMA GEMM, measured with HIP events
+on its stream) and the CPU baseline (the oracle's MKL host path, timed on
+this host's cores on the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "factor time & solve/s at 1e-12 backward error, FP64 vs FP32+IR; p99 latency"
+PEAK_FP64_TFLOPS = 78.6
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=["c2", "c3"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {max(args.gpus, world)}); "
+              "the separator tree is not sharded across GPUs, so there is no multi-GPU figure.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import snusolver_amd as S
+
+    A, grid, np_ = S.config_matrix(args.config)
+    n = A.n
+    _, b1 = S.gen_rhs(A, 1, 1)
+    _, b16 = S.gen_rhs(A, 16, 1)
+    t0 = time.perf_counter()
+    M = S.SnuMat(A, np_, grid=grid, device=0)
+    construct_s = time.perf_counter() - t0
+    counts = M.counts()
+
+    d_b = S.DeviceBuffer(8 * n * 16)
+    d_x = S.DeviceBuffer(8 * n * 16)
+    d_b.upload(np.asfortranarray(b1))
+
+    def step():
+        M.factorize()
+        M.solve_device_ptr(d_b.ptr.value, d_x.ptr.value, 1)
+
+    for _ in range(args.warmup):
+        step()
+    S.device_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    S.device_sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    tim = M.timings()
+    x = np.zeros(n)
+    d_x.download(x)
+    berr = S.backward_error(A, x, b1[:, 0])
+
+    # solves per second with resident factors
+    def solve_rate(nrhs, reps):
+        if nrhs > 1:
+            d_b.upload(np.asfortranarray(b16[:, :nrhs]))
+        M.solve_device_ptr(d_b.ptr.value, d_x.ptr.value, nrhs)
+        S.device_sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            M.solve_device_ptr(d_b.ptr.value, d_x.ptr.value, nrhs)
+        S.device_sync()
+        return reps / (time.perf_counter() - t)
+
+    solves1 = solve_rate(1, 10)
+    solves16 = solve_rate(16, 5)
+    X16 = np.zeros((n, 16), order="F")
+    d_x.download(X16)
+    berr16 = S.backward_error(A, X16, b16)
+
+    # dominant kernel: grouped FP64 MFMA GEMM (C -= sum A B), timed per launch
+    M.set_profiling(True)
+    M.factorize()
+    M.set_profiling(False)
+    pt = M.timings()
+    gemm_ms, gemm_launches, gemm_gflop = pt["gemm_sub_ms"], pt["gemm_sub_launches"], pt["gemm_sub_gflop"]
+    achieved = gemm_gflop / gemm_ms if gemm_ms > 0 else 0.0  # GFLOP/ms = TFLOP/s
+    roofline = {
+        "bound": "mfma",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+        "traffic": None,
+        "kernel": "snu::k_gemm<false> (grouped FP64 MFMA 16x16x4, C -= sum A B)",
+        "launches": int(gemm_launches),
+        "avg_launch_ms": round(gemm_ms / max(gemm_launches, 1), 4),
+        "gflop_per_launch": round(gemm_gflop / max(gemm_launches, 1), 4),
+        "assign_variant": {"ms": round(pt["gemm_assign_ms"], 3),
+                           "launches": int(pt["gemm_assign_launches"]),
+                           "gflop": round(pt["gemm_assign_gflop"], 3)},
+    }
+    M.close()
+    d_b.free()
+    d_x.free()
+
+    factor_flops = counts["f_dense"] + counts["f_leaf"]
+    line = {
+        "metric": METRIC,
+        "value": round(1e3 / ms_per_step, 4),
+        "unit": "factorize+solve/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "64^3 7-point convection-diffusion, FP64, 128 ND leaves, 1 rhs, factorize+solve"
+            if args.config == "c2" else "96^3 27-point anisotropic, FP64, 512 ND leaves, 1 rhs",
+            "n": n, "nnz": A.nnz, "leaves": np_, "nrhs": 1, "ordering": "geometric ND",
+        },
+        "factor_ms": round(tim["factor_ms"], 3),
+        "solve_ms": round(tim["solve_ms"], 3),
+        "phases_ms": {k: round(tim[k], 3) for k in
+                      ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms"]},
+        "factor_gflops": round(factor_flops / (tim["factor_ms"] * 1e6), 2),
+        "factor_flops_algorithmic": {"dense": counts["f_dense"], "leaf": counts["f_leaf"]},
+        "solves_per_s_nrhs1": round(solves1, 2),
+        "solves_per_s_nrhs16": round(solves16, 2),
+        "solve_bytes_per_rhs": counts["b_solve"],
+        "backward_error": berr,
+        "backward_error_nrhs16": berr16,
+        "construct_s": round(construct_s, 3),
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"], xo = cpu_baseline(A, grid, np_, b1[:, 0], args.cpu_threads)
+        line["rel_diff_vs_oracle"] = float(np.max(np.abs(x - xo)) / np.max(np.abs(xo)))
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(A, grid, np_, b, threads):
+    """The oracle's MKL host path (reference offlvl = -1 semantics), one full
+    factor+solve of the same workload, on this host's cores."""
+    cores = threads or min(16, os.cpu_count() or 1)
+    os.environ.setdefault("MKL_NUM_THREADS", str(cores))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import snusolver_amd as S
+    from oracle import Oracle, dense_backend
+
+    order, sizes = S.order_grid(*grid, np_)
+    O = Oracle(A, np_, order, sizes)
+    t0 = time.perf_counter()
+    x = O.factor_solve(b, nthreads=cores)
+    dt = time.perf_counter() - t0
+    O.close()
+    return ({
+        "value": round(1.0 / dt, 5),
+        "unit": "factorize+solve/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "one full factorize+solve of the same 64^3 workload (oracle C restatement, "
+                  f"leaves on {cores} threads, dense levels through "
+                  f"{'MKL' if dense_backend() else 'fallback loops'} with {cores} threads), "
+                  f"{dt:.1f} s",
+        "seconds": round(dt, 3),
+    }, x)
+
+
+if __name__ == "__main__":
+    main()

@@ -24,6 +24,7 @@
  */
 #ifndef SNUSOLVER_AMD_H
 #define SNUSOLVER_AMD_H
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -137,8 +138,11 @@ int snumat_sep_block(const snumat_t *h, int i, int j, double *out);
 /* phase timings of the last factorize / solve in ms (GPU events):
  * t[0] scatter, t[1] leaf rows, t[2] leaf separator rows + Schur,
  * t[3] Schur assembly, t[4] dense levels, t[5] factorize total,
- * t[6] solve total, t[7] dense GEMM kernel time (sum), t[8] dense GEMM
- * launches, t[9] dense GEMM algorithmic flops (1e9), t[10..15] reserved. */
+ * t[6] solve total; dense GEMM launches of the factorization:
+ * t[7] C -= AB kernel time (sum over launches, needs snumat_set_profiling),
+ * t[8] its launches, t[9] its algorithmic GFLOP (2 M N K per problem);
+ * t[10..12] the same for the C = AB (triangular solve by block inverse)
+ * launches; t[13..15] reserved. */
 int snumat_timings(const snumat_t *h, double *t16);
 /* algorithmic counts from the symbolic structure: c[0] F_dense (flops),
  * c[1] F_leaf (flops), c[2] leaf factor bytes, c[3] solve bytes per rhs. */
@@ -168,6 +172,17 @@ int snu_mtx_write_vector(const char *path, const double *x, int n);
 /* relative backward error ||Ax-b||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
 double snu_backward_error(const snu_csr_t *A, const double *x,
                           const double *b);
+
+/* ---------------- device memory plumbing ----------------
+ * For callers that keep b / x resident in HBM (kind 1: host->device,
+ * 2: device->host, 3: device->device). */
+int snu_dev_alloc(size_t bytes, void **p);
+int snu_dev_free(void *p);
+int snu_dev_copy(void *dst, const void *src, size_t bytes, int kind);
+int snu_dev_sync(void);
+/* Time every dense GEMM launch of the next factorizations with HIP events on
+ * the handle's stream (timings t[7]); adds event records between launches. */
+int snumat_set_profiling(snumat_t *h, int on);
 
 /* ---------------- kernel-level test hooks (host pointers) ----------------
  * Not part of the reference interface: they expose single kernels of this

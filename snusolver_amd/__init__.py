@@ -168,6 +168,33 @@ def read_mtx(path: str) -> CSR:
     return CSR(n.value, rp, ci, v)
 
 
+class DeviceBuffer:
+    """Device memory for right-hand sides / solutions kept resident in HBM."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = C.c_void_p()
+        self.nbytes = nbytes
+        _check(lib().snu_dev_alloc(C.c_size_t(nbytes), C.byref(self.ptr)))
+
+    def upload(self, arr: np.ndarray):
+        a = np.asfortranarray(arr)  # column-major, as the C-ABI expects
+        assert a.nbytes <= self.nbytes
+        _check(lib().snu_dev_copy(self.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1))
+
+    def download(self, arr: np.ndarray):
+        assert arr.flags.f_contiguous, "download into a column-major array"
+        _check(lib().snu_dev_copy(arr.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(arr.nbytes), 2))
+
+    def free(self):
+        if self.ptr.value:
+            lib().snu_dev_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def device_sync():
+    _check(lib().snu_dev_sync())
+
+
 def backward_error(A: CSR, x: np.ndarray, b: np.ndarray) -> float:
     """||Ax - b||_inf / (||A||_inf ||x||_inf + ||b||_inf), per column max."""
     x = np.asarray(x, np.float64)
@@ -249,6 +276,9 @@ class SnuMat:
     def solve_device_ptr(self, d_b: int, d_x: int, nrhs: int):
         _check(lib().snumat_solve_device(self._h, C.c_void_p(d_b), C.c_void_p(d_x), nrhs, self.A.n))
 
+    def set_profiling(self, on: bool):
+        _check(lib().snumat_set_profiling(self._h, 1 if on else 0))
+
     # -- introspection --
     def info(self) -> dict:
         a = (C.c_int64 * 16)()
@@ -305,7 +335,8 @@ class SnuMat:
         t = (C.c_double * 16)()
         _check(lib().snumat_timings(self._h, t))
         keys = ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms",
-                "factor_ms", "solve_ms", "gemm_ms", "gemm_launches", "gemm_gflop"]
+                "factor_ms", "solve_ms", "gemm_sub_ms", "gemm_sub_launches", "gemm_sub_gflop",
+                "gemm_assign_ms", "gemm_assign_launches", "gemm_assign_gflop"]
         return {k: float(t[i]) for i, k in enumerate(keys)}
 
     def counts(self) -> dict:

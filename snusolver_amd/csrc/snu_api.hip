@@ -115,7 +115,7 @@ struct snumat {
   SolvePlan sp;
   // timing
   bool profile = false;
-  std::vector<hipEvent_t> ev;
+  std::vector<hipEvent_t> ev, pev;
   double t[16] = {0};
 };
 
@@ -153,6 +153,8 @@ void free_device(snumat *h) {
   h->allocs.clear();
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   h->ev.clear();
+  for (hipEvent_t e : h->pev) (void)hipEventDestroy(e);
+  h->pev.clear();
   if (h->stream) (void)hipStreamDestroy(h->stream);
   h->stream = nullptr;
 }
@@ -635,8 +637,26 @@ static int factorize_stage(snumat_t *h, int stage) {
                         h->d_arena + P.leaf_total, dense_dev(h), st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(h->ev[4], st));
-  if (stage >= 2)
-    for (const Launch &L : h->fact) run_launch(h, L, h->d_gp, h->d_gs, h->d_dp, nullptr, 1);
+  int ngemm = 0;
+  if (stage >= 2) {
+    if (h->profile && h->pev.empty()) {
+      for (const Launch &L : h->fact)
+        if (L.kind == 1 || L.kind == 2) ngemm++;
+      h->pev.resize(2 * ngemm);
+      for (auto &e : h->pev) HIPCHK(hipEventCreate(&e));
+    }
+    int g = 0;
+    for (const Launch &L : h->fact) {
+      bool gemm = L.kind == 1 || L.kind == 2;
+      if (h->profile && gemm) HIPCHK(hipEventRecord(h->pev[2 * g], st));
+      run_launch(h, L, h->d_gp, h->d_gs, h->d_dp, nullptr, 1);
+      if (h->profile && gemm) {
+        HIPCHK(hipEventRecord(h->pev[2 * g + 1], st));
+        g++;
+      }
+    }
+    ngemm = g;
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(h->ev[5], st));
   HIPCHK(hipEventSynchronize(h->ev[5]));
@@ -646,12 +666,20 @@ static int factorize_stage(snumat_t *h, int stage) {
   h->t[3] = ev_ms(h->ev[3], h->ev[4]);
   h->t[4] = ev_ms(h->ev[4], h->ev[5]);
   h->t[5] = ev_ms(h->ev[0], h->ev[5]);
-  double gfl = 0;
-  int nl = 0;
-  for (const Launch &L : h->fact)
-    if (L.kind == 1 || L.kind == 2) gfl += L.flops, nl++;
-  h->t[8] = nl;
-  h->t[9] = gfl * 1e-9;
+  // t[7..9]: SUB GEMM launches (time when profiling, count, GFLOP);
+  // t[10..12]: the same for the ASSIGN (trsm-by-inverse) launches
+  for (int q = 7; q <= 12; q++) h->t[q] = 0;
+  {
+    int g = 0;
+    for (const Launch &L : h->fact) {
+      if (L.kind != 1 && L.kind != 2) continue;
+      int base = L.kind == 2 ? 7 : 10;
+      if (h->profile && stage >= 2) h->t[base] += ev_ms(h->pev[2 * g], h->pev[2 * g + 1]);
+      h->t[base + 1] += 1;
+      h->t[base + 2] += L.flops * 1e-9;
+      g++;
+    }
+  }
   h->factored = stage >= 2;
   return SNU_OK;
 }
@@ -919,5 +947,31 @@ extern "C" int snu_test_diag_lu(int nb, double *D, int ld, double *Linv,
   (void)hipFree(dD);
   (void)hipFree(dI);
   (void)hipFree(dp);
+  return SNU_OK;
+}
+
+// ---------------- device memory plumbing (bench, tests) ----------------
+extern "C" int snu_dev_alloc(size_t bytes, void **p) {
+  if (!p) return set_error(SNU_ERR_BAD_INPUT, "null");
+  HIPCHK(hipMalloc(p, bytes ? bytes : 1));
+  return SNU_OK;
+}
+extern "C" int snu_dev_free(void *p) {
+  HIPCHK(hipFree(p));
+  return SNU_OK;
+}
+extern "C" int snu_dev_copy(void *dst, const void *src, size_t bytes, int kind) {
+  hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
+                                                                  : hipMemcpyDeviceToDevice;
+  HIPCHK(hipMemcpy(dst, src, bytes, k));
+  return SNU_OK;
+}
+extern "C" int snu_dev_sync(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return SNU_OK;
+}
+extern "C" int snumat_set_profiling(snumat_t *h, int on) {
+  if (!h) return set_error(SNU_ERR_BAD_INPUT, "null");
+  h->profile = on != 0;
   return SNU_OK;
 }
