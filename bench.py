@@ -15,6 +15,14 @@ backward error and the relative difference to the CPU oracle, the roofline
 of the dominant kernel (the grouped FP64 MFMA GEMM, measured with HIP events
 on its stream) and the CPU baseline (the oracle's MKL host path, timed on
 this host's cores on the same workload).
+
+Two further legs (skippable) report the other forms of the path:
+  "precision": 96^3 27-point anisotropic operator (n = 884,736, 512 leaves),
+     FP32 factors + FP64 refinement to <= 1e-12 backward error against the
+     plain FP64 path (factor and solve times, refinement steps, errors);
+  "batch": 512 2D 9-point matrices (sides log-uniform in [16, 192]) factorized
+     and solved as one batch, FP64 and FP32 + refinement, against the same
+     matrices run one by one.
 """
 import argparse
 import json
@@ -37,6 +45,8 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c2", "c3"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-precision", action="store_true", help="skip the 96^3 FP32+IR leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-matrix batch leg")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -112,7 +122,7 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
         "traffic": None,
-        "kernel": "snu::k_gemm<false> (grouped FP64 MFMA 16x16x4, C -= sum A B)",
+        "kernel": "snu::k_gemm<double, 0, 2, 2> (grouped FP64 MFMA 4x4x4, C -= sum A B)",
         "launches": int(gemm_launches),
         "avg_launch_ms": round(gemm_ms / max(gemm_launches, 1), 4),
         "gflop_per_launch": round(gemm_gflop / max(gemm_launches, 1), 4),
@@ -160,7 +170,95 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"], xo = cpu_baseline(A, grid, np_, b1[:, 0], args.cpu_threads)
         line["rel_diff_vs_oracle"] = float(np.max(np.abs(x - xo)) / np.max(np.abs(xo)))
+    for flag, key, leg in ((args.no_precision, "precision", precision_leg), (args.no_batch, "batch", batch_leg)):
+        if flag:
+            continue
+        try:
+            line[key] = leg(S, np)
+        except Exception as e:  # report the failure of a side leg, keep the main line
+            line[key] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(line), flush=True)
+
+
+def precision_leg(S, np, reps=3):
+    """96^3 27-point anisotropic operator: plain FP64 vs FP32 factors with
+    FP64 refinement (factor and solve times are the best of reps)."""
+    A, grid, np_ = S.config_matrix("c3")
+    _, b = S.gen_rhs(A, 1, 1)
+    b = b[:, 0]
+    out = {"workload": "96^3 27-point anisotropic 1:10:100, 512 ND leaves, 1 rhs",
+           "n": A.n, "nnz": A.nnz}
+    xs = {}
+    for prec in ("fp64", "fp32"):
+        print(f"precision leg: {prec}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        M = S.SnuMat(A, np_, grid=grid, device=0, precision=prec)
+        construct = time.perf_counter() - t0
+        d_b = S.DeviceBuffer(8 * A.n)
+        d_x = S.DeviceBuffer(8 * A.n)
+        d_b.upload(np.asfortranarray(b.reshape(-1, 1)))
+        fms, sms, steps, ibe = [], [], 0, 0.0
+        for _ in range(reps):
+            M.factorize()
+            fms.append(M.timings()["factor_ms"])
+            M.solve_device_ptr(d_b.ptr.value, d_x.ptr.value, 1)
+            t = M.timings()
+            sms.append(t["solve_ms"])
+            steps, ibe = t["ir_steps"], t["ir_berr"]
+        x = np.zeros(A.n)
+        d_x.download(x)
+        xs[prec] = x
+        out[prec] = {"factor_ms": round(min(fms), 3), "solve_ms": round(min(sms), 3),
+                     "factorize_solve_per_s": round(1e3 / (min(fms) + min(sms)), 4),
+                     "backward_error": S.backward_error(A, x, b), "construct_s": round(construct, 2),
+                     "device_gb": round(M.info()["device_bytes"] / 1e9, 2)}
+        if prec == "fp32":
+            out[prec]["ir_steps"] = int(steps)
+            out[prec]["ir_backward_error"] = ibe
+        M.close()
+        d_b.free()
+        d_x.free()
+    out["rel_diff_fp32ir_vs_fp64"] = float(np.max(np.abs(xs["fp32"] - xs["fp64"])) / np.max(np.abs(xs["fp64"])))
+    out["speedup_fp32ir_vs_fp64"] = round(out["fp32"]["factorize_solve_per_s"] / out["fp64"]["factorize_solve_per_s"], 3)
+    return out
+
+
+def batch_leg(S, np, count=512):
+    """512 2D 9-point matrices factorized + solved as one batch vs one by one."""
+    mem = S.config_batch(count)
+    bs = [S.gen_rhs(A, 1, 3)[1][:, 0] for A, _, _ in mem]
+    out = {"workload": f"{count} 2D 9-point convection-diffusion, sides log-uniform [16,192], 1 rhs each",
+           "total_n": int(sum(A.n for A, _, _ in mem))}
+    for prec in ("fp64", "fp32"):
+        print(f"batch leg: {prec}", file=sys.stderr, flush=True)
+        mats = [S.SnuMat(A, np_, grid=(side, side, 1), device=0, precision=prec) for A, side, np_ in mem]
+        with S.Batch(mats) as B:
+            B.factorize()
+            B.solve(bs)
+            S.device_sync()
+            t0 = time.perf_counter()
+            reps = 3
+            for _ in range(reps):
+                B.factorize()
+                xs = B.solve(bs)
+            S.device_sync()
+            batched = (time.perf_counter() - t0) / reps
+            bt = B.timings()
+            fl, sl = B.launches()
+        worst = max(S.backward_error(A, x, b) for (A, _, _), x, b in zip(mem, xs, bs))
+        t0 = time.perf_counter()
+        for M, b in zip(mats, bs):
+            M.factorize()
+            M.solve(b)
+        single = time.perf_counter() - t0
+        for M in mats:
+            M.close()
+        out[prec] = {"batch_factorize_solve_s": round(batched, 4), "one_by_one_s": round(single, 4),
+                     "matrices_per_s": round(count / batched, 2), "speedup_vs_one_by_one": round(single / batched, 2),
+                     "batch_factor_ms": round(bt["factor_ms"], 3), "batch_solve_ms": round(bt["solve_ms"], 3),
+                     "dense_factor_launches": fl, "dense_solve_launches": sl,
+                     "worst_backward_error": worst}
+    return out
 
 
 def cpu_baseline(A, grid, np_, b, threads):

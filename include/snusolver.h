@@ -49,7 +49,9 @@ extern "C" {
 #define SNU_ORDER_GIVEN 2 /* caller's order[n] / sizes[2np-1] (ParMETIS layout) */
 
 /* precision kinds */
-#define SNU_FP64 0
+#define SNU_FP64 0 /* FP64 factors and solves (the reference's arithmetic) */
+#define SNU_FP32 1 /* FP32 factors and sweeps, FP64 iterative refinement
+                      (no reference counterpart) */
 
 /* Same field layout as the reference's csr_matrix (src/matrix.h:9-13):
  * int32 n, m, nnz; rowptr[n+1]; colidx[nnz] (0-based, column-sorted rows);
@@ -67,14 +69,17 @@ typedef struct {
   const int *order;/* SNU_ORDER_GIVEN: order[i] = new index of vertex i */
   const int *sizes;/* SNU_ORDER_GIVEN: 2np-1 block sizes, ParMETIS layout */
   double eps;      /* leaf pivot clamp, reference src/param.h:3 (1e-15) */
-  int precision;   /* SNU_FP64 */
+  int precision;   /* SNU_FP64 or SNU_FP32 */
   int device;      /* HIP device ordinal */
   uint64_t seed;   /* ordering seed (the ND is deterministic for a seed) */
+  int ir_max_steps;/* SNU_FP32: refinement steps per solve (default 10) */
+  double ir_tol;   /* SNU_FP32: target backward error per column (1e-15) */
 } snu_opts_t;
 
 typedef struct snumat snumat_t;
 
-/* Fills opts with the defaults: np = 4, ND ordering, eps = 1e-15, FP64. */
+/* Fills opts with the defaults: np = 4, ND ordering, eps = 1e-15, FP64,
+ * ir_max_steps = 10, ir_tol = 1e-15. */
 void snu_default_opts(snu_opts_t *opts);
 
 /* Selects the device and checks that it is usable (<- initialize()). */
@@ -92,7 +97,12 @@ int snumat_factorize(snumat_t *h);
 int snumat_update_values(snumat_t *h, const double *values);
 
 /* Solve A x = b for nrhs right-hand sides, column-major with leading
- * dimension ld >= n.  Host pointers. */
+ * dimension ld >= n.  Host pointers.
+ * SNU_FP32 handles: x0 from the FP32 factors, then x += A_32^-1 (b - A x)
+ * with the residual in FP64 until every column's backward error
+ * |b - Ax|inf / (|A|inf |x|inf + |b|inf) <= ir_tol, or until a step stops
+ * halving it; returns SNU_ERR_NOT_CONVERGED if it is then still above 1e-12
+ * (x holds the last iterate). */
 int snumat_solve(snumat_t *h, const double *b, double *x, int nrhs, int ld);
 
 /* Same with device pointers (resident in HBM, no PCIe in the path). */
@@ -100,6 +110,35 @@ int snumat_solve_device(snumat_t *h, const double *d_b, double *d_x, int nrhs,
                         int ld);
 
 int snumat_destroy(snumat_t *h);
+
+/* ---------------- batches (no reference counterpart) ----------------
+ * A batch runs the factorization and the solves of several constructed
+ * matrices (same device and precision) as one launch per step across all of
+ * them: the leaves of all matrices share the leaf launches, separators of
+ * equal tree height (counted from the leaves) share one grouped launch per
+ * dense step.  Each matrix's arithmetic is that of its standalone
+ * factorization; after snubatch_factorize every member is factored and may
+ * also be solved on its own.  The batch does not own the matrices, which
+ * must outlive it. */
+typedef struct snubatch snubatch_t;
+int snubatch_create(snumat_t *const *mats, int count, snubatch_t **out);
+int snubatch_factorize(snubatch_t *b);
+/* Per matrix m: nrhs[m] columns (0 skips it), b[m] / x[m] column-major with
+ * leading dimension ld[m] >= n_m.  Device pointers; all right-hand sides are
+ * packed into one sweep (one launch per step and tree height). */
+int snubatch_solve_device(snubatch_t *b, const double *const *d_b,
+                          double *const *d_x, const int *nrhs, const int *ld);
+/* Same with host pointers. */
+int snubatch_solve(snubatch_t *b, const double *const *hb, double *const *hx,
+                   const int *nrhs, const int *ld);
+/* t[0..5] factor phases as snumat_timings, t[6] last solve, t[13] the most
+ * refinement steps of any member (SNU_FP32), t[14] the largest final
+ * backward error. */
+int snubatch_timings(const snubatch_t *b, double *t16);
+/* number of dense factor launches and dense solve launches of the batch */
+int snubatch_launches(const snubatch_t *b, int *factor_launches,
+                      int *solve_launches);
+int snubatch_destroy(snubatch_t *b);
 
 /* Reference-style one shot: construct + factorize + solve + destroy
  * (<- void solve(csr_matrix, double *b, double *x)). */
@@ -110,8 +149,9 @@ const char *snu_last_error(void);
 
 /* ---------------- introspection (tests, bench) ---------------- */
 /* info[0..15]: n, nnz, np, num_block, max_level, total leaf LU nnz,
- * dense entries (int64 split lo/hi at [6],[7]), max core_n, max leaf_size,
- * max boundary rows, max boundary cols, device(0/1), factored(0/1), 0... */
+ * dense entries, max core_n, max leaf_size, max boundary rows, max boundary
+ * cols, on device (0/1), factored (0/1), device bytes held, waves per leaf
+ * workgroup, boundary target rows. */
 int snumat_info(const snumat_t *h, int64_t *info16);
 /* block_start/block_size of e = 1..2np-1, written at [e-1]. */
 int snumat_blocks(const snumat_t *h, int *block_start, int *block_size);
@@ -142,14 +182,16 @@ int snumat_sep_block(const snumat_t *h, int i, int j, double *out);
  * t[7] C -= AB kernel time (sum over launches, needs snumat_set_profiling),
  * t[8] its launches, t[9] its algorithmic GFLOP (2 M N K per problem);
  * t[10..12] the same for the C = AB (triangular solve by block inverse)
- * launches; t[13..15] reserved. */
+ * launches; t[13] refinement steps of the last solve (SNU_FP32), t[14] its
+ * final backward error (max over columns), t[15] reserved. */
 int snumat_timings(const snumat_t *h, double *t16);
 /* algorithmic counts from the symbolic structure: c[0] F_dense (flops),
  * c[1] F_leaf (flops), c[2] leaf factor bytes, c[3] solve bytes per rhs. */
 int snumat_counts(const snumat_t *h, double *c4);
 
 /* ---------------- host utilities ---------------- */
-/* Synthetic grid operators (seeded; see the design notes for the recipe).
+/* Synthetic grid operators (seeded splitmix64 stream, one draw per
+ * off-diagonal entry in row-major CSR order, x fastest in the grid).
  * stencil: 5, 9 (2D), 7, 27 (3D).  recipe 0: convection-diffusion with
  * off-diagonals -1 + 0.25 (u - 0.5) and the given diagonal; recipe 1:
  * anisotropic 1:10:100 axis weights, +-10% perturbation, diagonal 1.01 x the
@@ -187,11 +229,20 @@ int snumat_set_profiling(snumat_t *h, int on);
 /* ---------------- kernel-level test hooks (host pointers) ----------------
  * Not part of the reference interface: they expose single kernels of this
  * library to the parity tests. */
-/* C -= A B (assign = 0) or C = A B (assign = 1); col-major. */
+/* assign = 0: C -= A B (64 x 64 tiles), 2: C -= A B (128 x 128 tiles),
+ * 1: C = A B, 3: C = -A B; col-major. */
 int snu_test_gemm(int M, int N, int K, const double *A, int lda,
                   const double *B, int ldb, double *C, int ldc, int assign);
+int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,
+                      const float *B, int ldb, float *C, int ldc, int assign);
+/* average time (ms) of the grouped C -= A B kernel with tile 64 or 128 on
+ * device-resident operands, K split into nseg concatenated segments;
+ * prec SNU_FP64 or SNU_FP32. */
+int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
+                   double *ms_out);
 /* in-place no-pivot LU of an nb x nb block (nb <= 64); Linv/Uinv: 64 x 64. */
 int snu_test_diag_lu(int nb, double *D, int ld, double *Linv, double *Uinv);
+int snu_test_diag_lu_f32(int nb, float *D, int ld, float *Linv, float *Uinv);
 
 #ifdef __cplusplus
 }

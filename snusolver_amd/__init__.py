@@ -24,13 +24,15 @@ import numpy as np
 
 __all__ = [
     "SnuError", "CSR", "SnuMat", "lib", "gen_grid", "gen_rhs", "order_grid",
-    "order_nd", "read_mtx", "backward_error", "solve", "config_matrix",
+    "order_nd", "read_mtx", "backward_error", "solve", "config_matrix", "Batch",
+    "config_batch", "batch_sides", "batch_np",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsnusolver.so")
 
 ORDER_ND, ORDER_GRID, ORDER_GIVEN = 0, 1, 2
+FP64, FP32 = 0, 1
 _STATUS = {
     -1: "BAD_INPUT", -2: "STRUCTURE", -3: "ZERO_DIAG", -4: "DUPLICATE",
     -5: "NO_DEVICE", -6: "HIP", -7: "OOM", -8: "NOT_FACTORED", -9: "NUMERIC",
@@ -54,7 +56,7 @@ class _Opts(C.Structure):
     _fields_ = [("np", C.c_int), ("ordering", C.c_int), ("grid", C.c_int * 3),
                 ("order", C.POINTER(C.c_int)), ("sizes", C.POINTER(C.c_int)),
                 ("eps", C.c_double), ("precision", C.c_int), ("device", C.c_int),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("ir_max_steps", C.c_int), ("ir_tol", C.c_double)]
 
 
 _lib = None
@@ -212,7 +214,10 @@ class SnuMat:
     """construct -> factorize -> solve on one GPU (device=-1: host analysis only)."""
 
     def __init__(self, A: CSR, np_: int = 4, ordering: str = "nd", grid=None,
-                 order=None, sizes=None, device: int = 0, eps: float = 1e-15, seed: int = 0):
+                 order=None, sizes=None, device: int = 0, eps: float = 1e-15, seed: int = 0,
+                 precision: str = "fp64", ir_max_steps: int = 10, ir_tol: float = 1e-15):
+        """precision "fp64": the reference's arithmetic; "fp32": FP32 factors
+        and sweeps with FP64 iterative refinement (ir_max_steps, ir_tol)."""
         self.A = A
         o = _Opts()
         lib().snu_default_opts(C.byref(o))
@@ -220,6 +225,12 @@ class SnuMat:
         o.eps = eps
         o.device = device
         o.seed = seed
+        if precision not in ("fp64", "fp32"):
+            raise ValueError(precision)
+        o.precision = FP32 if precision == "fp32" else FP64
+        o.ir_max_steps = ir_max_steps
+        o.ir_tol = ir_tol
+        self.precision = precision
         self._keep = []
         if order is not None:
             o.ordering = ORDER_GIVEN
@@ -336,7 +347,8 @@ class SnuMat:
         _check(lib().snumat_timings(self._h, t))
         keys = ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms",
                 "factor_ms", "solve_ms", "gemm_sub_ms", "gemm_sub_launches", "gemm_sub_gflop",
-                "gemm_assign_ms", "gemm_assign_launches", "gemm_assign_gflop"]
+                "gemm_assign_ms", "gemm_assign_launches", "gemm_assign_gflop",
+                "ir_steps", "ir_berr"]
         return {k: float(t[i]) for i, k in enumerate(keys)}
 
     def counts(self) -> dict:
@@ -345,10 +357,112 @@ class SnuMat:
         return {"f_dense": c[0], "f_leaf": c[1], "b_leaf": c[2], "b_solve": c[3]}
 
 
+class Batch:
+    """Several constructed matrices factorized and solved together, one
+    launch per step across all of them (leaves of all matrices together,
+    separators of equal tree height together).  Each matrix's arithmetic is
+    that of its standalone factorization and solve."""
+
+    def __init__(self, mats):
+        self.mats = list(mats)
+        arr = (C.c_void_p * len(self.mats))(*[m._h.value for m in self.mats])
+        self._h = C.c_void_p()
+        _check(lib().snubatch_create(arr, len(self.mats), C.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().snubatch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def factorize(self):
+        _check(lib().snubatch_factorize(self._h))
+        return self
+
+    def solve(self, bs):
+        """bs: one array per matrix (n or n x k; None skips the matrix).
+        Returns the solutions in the same shapes."""
+        k = len(self.mats)
+        Bs, Xs, outs = [], [], []
+        nrhs = (C.c_int * k)()
+        ld = (C.c_int * k)()
+        pb = (C.c_void_p * k)()
+        px = (C.c_void_p * k)()
+        for i, (M, b) in enumerate(zip(self.mats, bs)):
+            ld[i] = M.A.n
+            if b is None:
+                nrhs[i] = 0
+                Bs.append(None)
+                Xs.append(None)
+                continue
+            b = np.asarray(b, np.float64)
+            B = np.asfortranarray(b.reshape(M.A.n, -1))
+            X = np.zeros_like(B, order="F")
+            nrhs[i] = B.shape[1]
+            pb[i] = B.ctypes.data
+            px[i] = X.ctypes.data
+            Bs.append(B)
+            Xs.append((X, b.ndim == 1))
+        _check(lib().snubatch_solve(self._h, pb, px, nrhs, ld))
+        for x in Xs:
+            outs.append(None if x is None else (x[0][:, 0].copy() if x[1] else x[0]))
+        return outs
+
+    def timings(self) -> dict:
+        t = (C.c_double * 16)()
+        _check(lib().snubatch_timings(self._h, t))
+        keys = ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms",
+                "factor_ms", "solve_ms"]
+        out = {k: float(t[i]) for i, k in enumerate(keys)}
+        out["ir_steps"] = float(t[13])
+        out["ir_berr"] = float(t[14])
+        return out
+
+    def launches(self):
+        f, s_ = C.c_int(), C.c_int()
+        _check(lib().snubatch_launches(self._h, C.byref(f), C.byref(s_)))
+        return f.value, s_.value
+
+
 def solve(A: CSR, b: np.ndarray, np_: int = 4, **kw) -> np.ndarray:
     """One-shot construct + factorize + solve (reference ``solve``)."""
     with SnuMat(A, np_, **kw) as M:
         return M.factorize().solve(b)
+
+
+def batch_sides(count: int = 512, lo: int = 16, hi: int = 192, seed: int = 2):
+    """Grid sides of the batched workload: log-uniform in [lo, hi] from
+    numpy's default generator (PCG64) with the given seed, rounded."""
+    rng = np.random.default_rng(seed)
+    return [int(round(v)) for v in np.exp(rng.uniform(np.log(lo), np.log(hi), count))]
+
+
+def batch_np(n: int, rows_per_leaf: int = 2048) -> int:
+    """Leaves per matrix: the power of two nearest to n / rows_per_leaf (>= 1)."""
+    r = max(1.0, n / rows_per_leaf)
+    return 1 << int(round(np.log2(r)))
+
+
+def config_batch(count: int = 512, seed: int = 2):
+    """The batched workload: count 2D 9-point convection-diffusion matrices
+    (diagonal 8.1; matrix i draws its coefficients from seed i) with sides
+    from batch_sides; returns a list of (A, side, np)."""
+    out = []
+    for i, side in enumerate(batch_sides(count, seed=seed)):
+        A = gen_grid(side, side, 1, 9, 0, 8.1, i)
+        out.append((A, side, batch_np(A.n)))
+    return out
 
 
 def config_matrix(name: str):

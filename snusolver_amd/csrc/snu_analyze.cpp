@@ -268,6 +268,12 @@ int analyze(const snu_csr_t *A, const snu_opts_t *opts, Plan &P) {
     }
   }
   P.n = n; P.nnz = nnz; P.np = np;
+  if (opts->precision != SNU_FP64 && opts->precision != SNU_FP32)
+    return set_error(SNU_ERR_BAD_INPUT, "unknown precision kind");
+  if (opts->precision == SNU_FP32) {  // pattern of A for the FP64 residuals
+    P.a_rowptr.assign(A->rowptr, A->rowptr + n + 1);
+    P.a_colidx.assign(A->colidx, A->colidx + nnz);
+  }
   P.num_block = 2 * np - 1;
   P.max_level = ilog2(np);
   P.nk = P.max_level + 1;
@@ -433,6 +439,76 @@ int analyze(const snu_csr_t *A, const snu_opts_t *opts, Plan &P) {
     }
     P.a_dest.assign(nnz, 0);
     for (int k = 0; k < nnz; k++) P.a_dest[k] = dest_q[P.perm_map[k]];
+  }
+  // ---- structural nonzeros of the separator panels ----
+  // Sources: A's separator-separator entries, every leaf's boundary Schur
+  // block (its boundary rows x boundary cols), and the Schur update of each
+  // separator into its ancestors (rows nonzero in its L panel x columns
+  // nonzero in its U panel), propagated bottom-up.  The triangular solves of
+  // the panels keep zero rows (L panel) and zero columns (U panel) zero.
+  {
+    std::vector<std::vector<char>> nzL(nb + 1), nzU(nb + 1);
+    for (int p = 1; p < np; p++) nzL[p].assign(P.anc_sum[p], 0), nzU[p].assign(P.anc_sum[p], 0);
+    for (int i = 1; i < np; i++)
+      for (int ii = i / 2; ii; ii /= 2) {
+        int g = P.grid_id(i, ii), off = P.anc_off(i, ii);
+        for (int q = P.goff[g]; q < P.goff[g] + P.gnnz[g]; q++)
+          nzU[i][off + P.coo_c[q] - P.block_start[ii]] = 1;
+        g = P.grid_id(ii, i);
+        for (int q = P.goff[g]; q < P.goff[g] + P.gnnz[g]; q++)
+          nzL[i][off + P.coo_r[q] - P.block_start[ii]] = 1;
+      }
+    for (const LeafPlan &L : P.leaves) {
+      std::vector<std::vector<int>> rows(L.npath), cols(L.npath);
+      auto kof = [&](int c) {
+        int k = 1;
+        while (c >= L.cstart[k + 1]) k++;
+        return k;
+      };
+      for (int r : L.brow) { int k = kof(r); rows[k].push_back(r - L.cstart[k]); }
+      for (int c : L.bcol) { int k = kof(c); cols[k].push_back(c - L.cstart[k]); }
+      for (int k1 = 1; k1 < L.npath; k1++)
+        for (int k2 = 1; k2 < L.npath; k2++) {
+          if (k1 == k2 || rows[k1].empty() || cols[k2].empty()) continue;
+          int j1 = L.path[k1], j2 = L.path[k2];
+          if (j1 > j2) {
+            int off = P.anc_off(j1, j2);
+            for (int c : cols[k2]) nzU[j1][off + c] = 1;
+          } else {
+            int off = P.anc_off(j2, j1);
+            for (int r : rows[k1]) nzL[j2][off + r] = 1;
+          }
+        }
+    }
+    for (int l = P.max_level - 1; l >= 0; l--)
+      for (int d = 1 << l; d < std::min(2 << l, np); d++)
+        for (int j1 = d / 2; j1; j1 /= 2)
+          for (int j2 = d / 2; j2; j2 /= 2) {
+            if (j1 == j2) continue;
+            int o1 = P.anc_off(d, j1), o2 = P.anc_off(d, j2);
+            bool rowsnz = false, colsnz = false;
+            for (int r = 0; r < P.block_size[j1] && !rowsnz; r++) rowsnz = nzL[d][o1 + r];
+            for (int c = 0; c < P.block_size[j2] && !colsnz; c++) colsnz = nzU[d][o2 + c];
+            if (!rowsnz || !colsnz) continue;
+            if (j1 > j2) {
+              int off = P.anc_off(j1, j2);
+              for (int c = 0; c < P.block_size[j2]; c++) nzU[j1][off + c] |= nzU[d][o2 + c];
+            } else {
+              int off = P.anc_off(j2, j1);
+              for (int r = 0; r < P.block_size[j1]; r++) nzL[j2][off + r] |= nzL[d][o1 + r];
+            }
+          }
+    P.cntL.assign(nb + 1, std::vector<int>(1, 0));
+    P.cntU.assign(nb + 1, std::vector<int>(1, 0));
+    for (int p = 1; p < np; p++) {
+      int a = P.anc_sum[p];
+      P.cntL[p].assign(a + 1, 0);
+      P.cntU[p].assign(a + 1, 0);
+      for (int x = 0; x < a; x++) {
+        P.cntL[p][x + 1] = P.cntL[p][x] + nzL[p][x];
+        P.cntU[p][x + 1] = P.cntU[p][x] + nzU[p][x];
+      }
+    }
   }
   // ---- algorithmic counts ----
   double fd = 0, bsolve = 0, fl = 0, bleaf = 0;

@@ -1,0 +1,230 @@
+// Host-side engine of the MI355X SnuSolver path: device state of one matrix
+// (struct snumat), the launch lists of its factorization and solves, and the
+// executors that run a set of matrices (one, or a batch) with one launch per
+// step across all of them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "snu_internal.h"
+#include "snu_kernels.h"
+
+#define HIPCHK(x)                                                          \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess)                                                  \
+      return snu::set_error(SNU_ERR_HIP, std::string("HIP error ") +       \
+                                             hipGetErrorString(e_) + " at " #x); \
+  } while (0)
+
+namespace snu {
+
+// One grouped launch.  key orders the launches of one matrix; launches of
+// several matrices with equal (key, kind) are independent and are merged
+// into one launch.
+struct Launch {
+  int kind;  // 0 diag LU; GEMM (64 tile) 1: C = AB, 2: C -= AB, 6: C = -AB;
+             // 5: GEMM C -= AB (128 tile); GEMV 7: y = Ax, 8: y -= Ax
+  int p0, np, tiles;
+  double flops;
+  long long key;
+};
+inline bool is_gemm(int kind) { return kind == 1 || kind == 2 || kind == 5 || kind == 6; }
+
+// Launch list with its descriptors (device copies once uploaded).
+template <typename T>
+struct DensePlan {
+  std::vector<DiagProbT<T>> dp;
+  std::vector<GemmProbT<T>> gp;
+  std::vector<GemmSegT<T>> gs;
+  std::vector<Launch> launches;
+  DiagProbT<T> *d_dp = nullptr;
+  GemmProbT<T> *d_gp = nullptr;
+  GemmSegT<T> *d_gs = nullptr;
+};
+template <typename T>
+struct GemvPlan {
+  std::vector<GemvProbT<T>> vp;
+  std::vector<GemvSegT<T>> vs;
+  std::vector<Launch> fwd, bwd;
+  GemvProbT<T> *d_vp = nullptr;
+  GemvSegT<T> *d_vs = nullptr;
+};
+
+// Value-typed device state of one matrix (FP64, or FP32 refined in FP64).
+template <typename T>
+struct Values {
+  T *arena = nullptr;  // leaf LU values, then dense blocks
+  T *S = nullptr;      // boundary Schur blocks
+  T *inv = nullptr;    // per separator: full L^-1 then U^-1 (s x s each)
+  T *tmp = nullptr;    // products of the triangular inversion
+  DensePlan<T> fact;
+  // standalone solves
+  T *bp = nullptr, *ys = nullptr, *cbuf = nullptr;
+  int plan_nrhs = -1;
+  GemvPlan<T> sp;
+};
+
+// Host copies of the per-leaf work lists (kept for batching).
+struct WorkLists {
+  std::vector<int> wg_leaf, wg_first;    // leaf separator-row workgroups
+  std::vector<int> fwg_leaf, fwg_first;  // forward boundary-row workgroups
+  std::vector<int> trow_blk, trow_loc, trow_glob, tptr, tsrc_leaf, tsrc_ri;  // targets
+};
+
+// Device copies of the work lists of a launch set (one matrix or a batch).
+struct WorkDev {
+  LeafDev *leaves = nullptr;
+  int nleaves = 0;
+  int *wg_leaf = nullptr, *wg_first = nullptr, nwg = 0;
+  int *fwg_leaf = nullptr, *fwg_first = nullptr, nfwg = 0;
+  int *trow_blk = nullptr, *trow_loc = nullptr, *trow_glob = nullptr, *tptr = nullptr;
+  int *tsrc_leaf = nullptr, *tsrc_ri = nullptr, ntarget = 0;
+  int cb_total = 0;
+  int wsz = 0, waves = 1;
+};
+
+}  // namespace snu
+
+struct snumat {
+  snu::Plan P;
+  snu_opts_t opts;
+  int prec = SNU_FP64;
+  bool on_device = false;
+  bool factored = false;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  std::vector<void *> allocs;
+  size_t dev_bytes = 0;
+  long long arena_len = 0;
+  double *d_aval = nullptr;  // A's values in the caller's CSR order (FP64)
+  long long *d_adest = nullptr;
+  int *d_arow = nullptr, *d_acol = nullptr;  // A's pattern (refinement residuals)
+  int *d_order = nullptr;
+  // leaves and work lists
+  std::vector<snu::LeafDev> h_leaves;  // vec = 0, cb_off local
+  snu::WorkLists wl;
+  snu::WorkDev wd;
+  int max_leaf_w = 0;  // widest leaf workspace (leaf_size + nbc)
+  // dense tables
+  snu::DenseDev *d_dd = nullptr;
+  std::vector<long long> inv_off;
+  long long inv_total = 0, tmp_total = 0;
+  snu::Values<double> v64;
+  snu::Values<float> v32;
+  // standalone solves
+  double *d_bx = nullptr;  // staging for host-pointer solves
+  double *d_r = nullptr, *d_dx = nullptr;  // refinement residual / correction
+  unsigned long long *d_norms = nullptr;     // |A|inf (bits of a double)
+  unsigned long long *d_colnorms = nullptr;  // refinement column norms (3 x nrhs)
+  int *d_active = nullptr;
+  snu::VecDev *d_vec = nullptr;
+  snu::PermDev *d_perm = nullptr;
+  snu::PrepDev *d_prep = nullptr;
+  snu::IrDev *d_ir = nullptr;
+  double anorm = 0;
+  int nrhs_cap = 0;
+  int ir_steps = 0;
+  double ir_berr = 0;
+  // timing
+  bool profile = false;
+  std::vector<hipEvent_t> ev, pev;
+  double t[16] = {0};
+};
+
+namespace snu {
+
+template <typename T>
+Values<T> &vals(snumat *h) {
+  if constexpr (std::is_same<T, double>::value) return h->v64;
+  else return h->v32;
+}
+template <typename T>
+const Values<T> &vals(const snumat *h) {
+  if constexpr (std::is_same<T, double>::value) return h->v64;
+  else return h->v32;
+}
+
+int upload(snumat *h);
+void free_device(snumat *h);
+int device_alloc(snumat *h, void **p, size_t bytes);
+float ev_ms(hipEvent_t a, hipEvent_t b);
+int waves_for(int wsz, int vsize);
+
+template <typename T>
+int upload_list(T **p, const std::vector<T> &v);
+
+// Append problem C (M x N) -= / = sum_seg A_seg B_seg to a grouped GEMM
+// launch with tb x tb tiles; returns 1 if a problem was added.
+template <typename T>
+int add_gemm(std::vector<GemmProbT<T>> &gp, std::vector<GemmSegT<T>> &gs, int &tiles,
+             T *C, long long ldc, int M, int N, const std::vector<GemmSegT<T>> &segs,
+             double &flops, int tb = 64);
+
+// GEMV plan of one matrix's dense solves into the packed vectors bp / ys at
+// this matrix's segment (ld = n, nrhs columns); keys order fwd / bwd.
+template <typename T>
+void build_solve_plan(const snumat *h, T *bp, T *ys, int nrhs, GemvPlan<T> &out);
+template <typename T>
+int upload_gemv_plan(GemvPlan<T> &g);
+template <typename T>
+void free_gemv_plan(GemvPlan<T> &g);
+template <typename T>
+int upload_dense_plan(DensePlan<T> &d);
+template <typename T>
+void free_dense_plan(DensePlan<T> &d);
+
+// Merge the launch lists of several matrices: launches with equal (key,
+// kind) become one launch over the concatenated problems.
+template <typename T>
+void merge_dense(const std::vector<const DensePlan<T> *> &in, DensePlan<T> &out);
+template <typename T>
+void merge_gemv(const std::vector<const GemvPlan<T> *> &in, GemvPlan<T> &out);
+
+// Executors over a launch set.
+struct FactorSet {
+  const PrepDev *prep;
+  int nmat;
+  long long max_len, max_nnz;
+  const IrDev *ir;           // FP32 only: |A| norms into anorm_out
+  unsigned long long *anorm_out;
+  int max_n;
+  const WorkDev *wd;
+  double eps;
+};
+template <typename T>
+int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
+               const std::vector<hipEvent_t> *ev, bool profile, std::vector<hipEvent_t> *pev,
+               double *gemm_ms);
+struct SolveSet {
+  const PermDev *perm;
+  const VecDev *vec;
+  int nmat;
+  long long max_elems;
+  int max_rhs;
+  const WorkDev *wd;
+  int cb_ld;
+};
+template <typename T>
+void run_sweep(const SolveSet &ss, T *bp, T *cbuf, const GemvPlan<T> &plan, hipStream_t st);
+struct RefineSet {
+  SolveSet sweep_b;  // permutation tables b -> x (the caller's vectors)
+  SolveSet sweep_r;  // permutation tables r -> dx
+  const IrDev *ir;
+  int nmat, max_n;
+  long long max_elems;
+  unsigned long long *norms;  // 3 areas (b, r, x) of nslots
+  int nslots;
+  const std::vector<int> *slot_mat;   // matrix of each column slot
+  const std::vector<double> *anorm;   // |A|inf per matrix
+  int *d_active;                      // nmat flags
+  int max_steps;
+  double tol;
+};
+int run_refined(const RefineSet &rs, float *bp, float *cbuf, const GemvPlan<float> &plan,
+                hipStream_t st, int *steps_out, std::vector<double> *berr_mat);
+
+}  // namespace snu

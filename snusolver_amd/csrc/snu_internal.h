@@ -38,6 +38,7 @@ struct Plan {
   std::vector<int> order, sizes;
   std::vector<int> block_start, block_size, level;  // index 0..num_block
   std::vector<int> pa_rowptr, pa_colidx, perm_map;
+  std::vector<int> a_rowptr, a_colidx;  // A's own pattern (FP32 refinement only)
   std::vector<int> coo_r, coo_c;
   std::vector<int> gnnz, goff;    // merged COO layout per grid id
   std::vector<int> mstart, msize; // merge region of each block
@@ -65,6 +66,13 @@ struct Plan {
   }
   // value-arena index of separator entry (bi, lr) x (bj, lc)
   int64_t sep_index(int bi, int lr, int bj, int lc) const;
+  // Structural nonzeros of the dense separator panels: for separator p,
+  // nzL[p][r] / nzU[p][c] flag the rows of Lp_p / columns of Up_p (ancestors
+  // stacked bottom-up, A_p each) that can be nonzero; cntL / cntU are their
+  // prefix counts (A_p + 1 entries).
+  std::vector<std::vector<int>> cntL, cntU;
+  bool anyL(int p, int a, int b) const { return b > a && cntL[p][b] > cntL[p][a]; }
+  bool anyU(int p, int a, int b) const { return b > a && cntU[p][b] > cntU[p][a]; }
   double f_dense = 0, f_leaf = 0, b_leaf = 0, b_solve = 0;
 };
 

@@ -1,4 +1,7 @@
 // Device-side descriptors and kernel launchers of the MI355X SnuSolver path.
+// Value-carrying kernels are templated on the value type T (double for the
+// FP64 path, float for the FP32 factorization refined in FP64); launchers are
+// instantiated for both in snu_kernels.hip.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -6,21 +9,7 @@
 
 namespace snu {
 
-// One leaf (one reference rank): offsets into the concatenated device arrays.
-struct LeafDev {
-  long long val_off;  // leaf LU values in the value arena
-  long long idx_off;  // workspace-column array (one entry per LU slot)
-  long long S_off;    // boundary Schur block, nbr x nbc row-major
-  int rp_off;         // rowptr (core_n + 1) and diag (core_n) arrays
-  int core_n, leaf_size, nbr, nbc;
-  int lev_off, nlev, levrows_off;     // factor / forward level sets
-  int ulev_off, nulev, ulevrows_off;  // backward level sets
-  int brow_off, bcol_off;             // boundary row / col arrays
-  int gstart;                         // permuted index of the first leaf row
-  int cb_off;                         // rhs contribution rows (x nrhs)
-};
-
-// Single-copy separator storage (see the design notes):
+// Single-copy separator storage:
 //   R_p = [D_p | Up_p] : s_p x (s_p + A_p), ld s_p, ancestors bottom-up
 //   Lp_p              : A_p x s_p, ld A_p
 struct DenseDev {
@@ -30,79 +19,164 @@ struct DenseDev {
   int nk;
 };
 
-struct GemmSeg {
-  const double *A, *B;
+// One leaf (one reference rank) of one matrix: device pointers into the
+// owning matrix's arrays, so that leaves of several matrices can share a
+// launch.
+struct LeafDev {
+  const int *rowptr, *diag, *wcol;       // LU rowptr (core_n + 1), diag, workspace column per slot
+  const int *levptr, *levrows;           // factor / forward level sets
+  const int *ulevptr, *ulevrows;         // backward level sets
+  const int *brow;                       // boundary rows (core-local)
+  const int *bcol_glob, *bcol_blk, *bcol_loc;  // boundary columns: permuted index, block, offset
+  void *vals;                            // leaf LU values (value type of the launch)
+  void *S;                               // boundary Schur block, nbr x nbc row-major
+  void *dense;                           // the matrix's dense separator arena
+  const DenseDev *dd;                    // the matrix's separator tables
+  int core_n, leaf_size, nbr, nbc, nlev, nulev;
+  int gstart;                            // permuted index of the first leaf row
+  int cb_off;                            // rows of the rhs contribution buffer
+  int vec;                               // the matrix's slot in the launch's VecDev table
+  int pad;
+};
+// Segment of a packed right-hand-side buffer: columns of length n at
+// base + c ld, c < nrhs.
+struct VecDev {
+  long long base;
+  int ld, nrhs;
+};
+// Caller-side vectors of one matrix for the (un)permutation.
+struct PermDev {
+  const int *order;
+  const double *b;
+  double *x;
+  long long ldb, ldx;
+  int n, vec;
+};
+// Factor set-up of one matrix.
+struct PrepDev {
+  void *arena, *inv;
+  long long arena_len, inv_len;
+  const double *aval;
+  const long long *adest;
+  long long nnz;
+};
+// FP64 refinement of one matrix: A in its CSR order, column-major vectors.
+struct IrDev {
+  const int *rowptr, *colidx;
+  const double *val;
+  const double *b;
+  double *x, *r, *dx;
+  long long ldb, ldx, ldr, lddx;
+  int n, nrhs;
+  int norm0;  // first slot of this matrix's columns in the norm arrays
+  int pad;
+};
+
+template <typename T>
+struct GemmSegT {
+  const T *A, *B;
   long long lda, ldb;
   int K;
 };
-struct GemmProb {
-  double *C;
+template <typename T>
+struct GemmProbT {
+  T *C;
   long long ldc;
   int M, N;
   int seg0, nseg;
   int tiles_n, tile0;
 };
-struct DiagProb {
-  double *D;
+template <typename T>
+struct DiagProbT {
+  T *D;
   long long ld;
   int nb;
-  double *Linv, *Uinv;  // 64 x 64 each, ld 64
+  T *Linv, *Uinv;  // nb x nb diagonal blocks of the full inverses
+  long long ldi;
 };
-struct TriProb {
-  const double *D;   // s x s LU factors, ld s
-  long long ld;
-  int s;
-  const double *inv;  // per 64-block: [Linv | Uinv], 2 * 4096 doubles each
-  double *b;          // rhs rows of this separator, ld ldb
-  long long ldb;
+// y (M x nrhs) = / -= sum_seg A_seg (M x K) x_seg (K x nrhs); tri: 0 dense,
+// 1 lower triangular single segment (columns k <= row), 2 upper (k >= row).
+template <typename T>
+struct GemvSegT {
+  const T *A, *x;
+  long long lda, ldx;
+  int K;
 };
+template <typename T>
+struct GemvProbT {
+  T *y;
+  long long ldy;
+  int M, seg0, nseg, tri, tile0, nrhs;
+};
+using GemmSeg = GemmSegT<double>;
+using GemmProb = GemmProbT<double>;
+using DiagProb = DiagProbT<double>;
+using GemvSeg = GemvSegT<double>;
+using GemvProb = GemvProbT<double>;
 
 int device_lds_limit();
 
-void launch_fill(double *p, long long n, hipStream_t st);
-void launch_scatter(const double *aval, const long long *dest, int nnz,
-                    double *arena, hipStream_t st);
-void launch_leaf_rows(const LeafDev *leaves, int nleaves, const int *rowptr,
-                      const int *diag, const int *wcol, const int *levptr,
-                      const int *levrows, double *arena, int wsz, int waves,
-                      double eps, hipStream_t st);
-void launch_leaf_sep(const LeafDev *leaves, const int *wg_leaf,
-                     const int *wg_first, int nwg, const int *rowptr,
-                     const int *diag, const int *wcol, const int *brow,
-                     double *arena, double *S, int wsz, int waves,
+template <typename T>
+void launch_fill(T *p, long long n, hipStream_t st);
+// zero arena / inverses, scatter A's values (one matrix per table entry)
+template <typename T>
+void launch_prepare(const PrepDev *tab, int nmat, long long max_len, long long max_nnz,
+                    hipStream_t st);
+template <typename T>
+void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, double eps,
+                      hipStream_t st);
+template <typename T>
+void launch_leaf_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
+                     int wsz, int waves, hipStream_t st);
+template <typename T>
+void launch_assemble_schur(int ntarget, const int *trow_blk, const int *trow_loc,
+                           const int *tptr, const int *tsrc_leaf, const int *tsrc_ri,
+                           const LeafDev *leaves, hipStream_t st);
+// mode 0: C -= sum A B; 1: C = A B; 2: C = -A B
+template <typename T>
+void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
+                 int total_tiles, int mode, hipStream_t st);
+template <typename T>
+void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
+                    int total_tiles, hipStream_t st);
+// mode 0: y -= A x; 1: y = A x
+template <typename T>
+void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
+                 int total_tiles, int mode, hipStream_t st);
+template <typename T>
+void launch_diag_lu(const DiagProbT<T> *probs, int nprob, hipStream_t st);
+// max_elems: the largest n x nrhs of the matrices
+template <typename T>
+void launch_permute_in(const PermDev *pd, int nmat, const VecDev *vd, T *bp,
+                       long long max_elems, hipStream_t st);
+template <typename T>
+void launch_permute_out(const PermDev *pd, int nmat, const VecDev *vd, const T *bp,
+                        long long max_elems, hipStream_t st);
+template <typename T>
+void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp,
                      hipStream_t st);
-void launch_assemble_schur(int ntarget, const int *trow_blk,
-                           const int *trow_loc, const int *tptr,
-                           const int *tsrc_leaf, const int *tsrc_ri,
-                           const LeafDev *leaves, const int *bcol_blk,
-                           const int *bcol_loc, const double *S, double *dense,
-                           DenseDev dd, hipStream_t st);
-void launch_gemm(const GemmProb *probs, int nprob, const GemmSeg *segs,
-                 int total_tiles, bool assign, hipStream_t st);
-void launch_diag_lu(const DiagProb *probs, int nprob, hipStream_t st);
-void launch_tri_fwd(const TriProb *probs, int nprob, int nrhs, hipStream_t st);
-void launch_tri_bwd(const TriProb *probs, int nprob, int nrhs, hipStream_t st);
-void launch_permute_in(const double *b, long long ldb, const int *order,
-                       double *bp, int n, int nrhs, hipStream_t st);
-void launch_permute_out(const double *bp, const int *order, double *x,
-                        long long ldx, int n, int nrhs, hipStream_t st);
-void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const int *rowptr,
-                     const int *diag, const int *wcol, const int *levptr,
-                     const int *levrows, const double *arena, double *bp,
-                     int n, int nrhs, hipStream_t st);
-void launch_leaf_fwd_sep(const LeafDev *leaves, const int *wg_leaf,
-                         const int *wg_first, int nwg, int waves,
-                         const int *rowptr, const int *wcol, const int *brow,
-                         const double *arena, const double *bp, double *cbuf,
-                         int n, int nrhs, int cb_ld, hipStream_t st);
+template <typename T>
+void launch_leaf_fwd_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
+                         int waves, const VecDev *vd, const T *bp, T *cbuf, int cb_ld,
+                         hipStream_t st);
+template <typename T>
 void launch_assemble_rhs(int ntarget, const int *trow_glob, const int *tptr,
-                         const int *tsrc_leaf, const int *tsrc_ri,
-                         const LeafDev *leaves, const double *cbuf, int cb_ld,
-                         double *bp, int n, int nrhs, hipStream_t st);
-void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const int *rowptr,
-                     const int *diag, const int *wcol, const int *ulevptr,
-                     const int *ulevrows, const int *bcol_glob,
-                     const double *arena, double *bp, int n, int nrhs,
+                         const int *tsrc_leaf, const int *tsrc_ri, const LeafDev *leaves,
+                         const VecDev *vd, const T *cbuf, int cb_ld, T *bp, int max_rhs,
+                         hipStream_t st);
+template <typename T>
+void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp,
                      hipStream_t st);
+
+// ---- FP64 iterative refinement (one matrix per table entry) ----
+void launch_residual(const IrDev *tab, int nmat, int max_n, hipStream_t st);
+// which: 0 b, 1 r, 2 x; out (nout slots) is zeroed first
+void launch_colmax(const IrDev *tab, int nmat, int max_n, int which, unsigned long long *out,
+                   int nout, hipStream_t st);
+void launch_rowabs_max(const IrDev *tab, int nmat, int max_n, unsigned long long *out,
+                       hipStream_t st);
+// x += dx where active[matrix] != 0
+void launch_update(const IrDev *tab, int nmat, long long max_elems, const int *active,
+                   hipStream_t st);
 
 }  // namespace snu

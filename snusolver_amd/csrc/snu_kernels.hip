@@ -9,11 +9,26 @@
 // applies the identical per-entry sequence.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "snu_kernels.h"
 
 namespace snu {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+// Pointers read from launch descriptors are generic; casting them to the
+// global address space makes the compiler emit global_load/global_store
+// (flat accesses also count against lgkmcnt, which would make every LDS wait
+// drain the global prefetch as well).
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ gcdouble *gptr(const double *p) { return (gcdouble *)p; }
+__device__ __forceinline__ gdouble *gptr(double *p) { return (gdouble *)p; }
+typedef __attribute__((address_space(1))) const float gcfloat;
+typedef __attribute__((address_space(1))) float gfloat;
+__device__ __forceinline__ gcfloat *gptr(const float *p) { return (gcfloat *)p; }
+__device__ __forceinline__ gfloat *gptr(float *p) { return (gfloat *)p; }
 
 __device__ __forceinline__ int rl(int v, int l) {
   return __builtin_amdgcn_readlane(v, l);
@@ -24,10 +39,14 @@ __device__ __forceinline__ double rld(double v, int l) {
   int hi = rl((int)(x >> 32), l);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+__device__ __forceinline__ float rld(float v, int l) {
+  return __int_as_float(rl(__float_as_int(v), l));
+}
 __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
-__device__ __forceinline__ double wave_sum(double v) {
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
@@ -35,29 +54,20 @@ __device__ __forceinline__ double wave_sum(double v) {
 // ---------------------------------------------------------------------------
 // elementwise
 // ---------------------------------------------------------------------------
-__global__ void k_fill(double *p, long long n) {
+template <typename T>
+__global__ void k_fill(T *p, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long stride = (long long)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = 0.0;
+  for (; i < n; i += stride) p[i] = T(0);
 }
-void launch_fill(double *p, long long n, hipStream_t st) {
+template <typename T>
+void launch_fill(T *p, long long n, hipStream_t st) {
   if (n <= 0) return;
   long long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
 }
 
-__global__ void k_scatter(const double *aval, const long long *dest, int nnz,
-                          double *arena) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nnz) arena[dest[i]] = aval[i];
-}
-void launch_scatter(const double *aval, const long long *dest, int nnz,
-                    double *arena, hipStream_t st) {
-  if (nnz <= 0) return;
-  hipLaunchKernelGGL(k_scatter, dim3((nnz + 255) / 256), dim3(256), 0, st,
-                     aval, dest, nnz, arena);
-}
 
 // ---------------------------------------------------------------------------
 // leaf factorization
@@ -70,21 +80,28 @@ void launch_scatter(const double *aval, const long long *dest, int nnz,
 //   otherwise a separator row: every entry is an L entry, and the updates
 //             that land on boundary columns form the row's Schur
 //             contribution -sum_k L(r,k) U(k,c) (kernel_sparse.cu:953-971).
-template <bool LEAF_ROW>
+template <typename T, bool LEAF_ROW>
 __device__ void factor_row(int i, const int *__restrict__ rowptr,
                            const int *__restrict__ diag,
                            const int *__restrict__ wcol,
-                           double *__restrict__ val, double *w, int lane,
-                           double eps) {
+                           T *__restrict__ val, T *w, int lane,
+                           T eps) {
 #pragma clang fp contract(off)
+  // Pivot rows are streamed one pivot ahead: while pivot t is applied, the
+  // first NCH chunks (64 entries each) of pivot t+1's U row are already in
+  // flight (unconditional loads with clamped addresses; the update itself is
+  // masked).  Longer U rows fall back to loads inside the update loop.
+  constexpr int NCH = 16;
   const int p0 = rowptr[i], p1 = rowptr[i + 1];
   const int pd = LEAF_ROW ? diag[i] : p1;
   for (int p = p0 + lane; p < p1; p += 64) w[wcol[p]] = val[p];
   wave_lds_fence();
+  int cc[NCH], cn[NCH];
+  T uc[NCH], un[NCH];
   for (int pb = p0; pb < pd; pb += 64) {
     const int cnt = min(64, pd - pb);
-    int k = 0, ks = 0, ke = 0;
-    double piv = 1.0;
+    int k = 0, ks = 1, ke = 1;
+    T piv = T(1);
     if (lane < cnt) {
       k = wcol[pb + lane];
       int kd = diag[k];
@@ -92,32 +109,40 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
       ke = rowptr[k + 1];
       piv = val[kd];
     }
-    int cn = 0;
-    double un = 0.0;
     {
-      int s0 = rl(ks, 0), e0 = rl(ke, 0), q = s0 + lane;
-      if (q < e0) cn = wcol[q], un = val[q];
+      const int s0 = rl(ks, 0), e0 = rl(ke, 0);
+#pragma unroll
+      for (int j = 0; j < NCH; j++) {
+        int q = min(s0 + lane + 64 * j, e0 - 1);
+        cn[j] = wcol[q];
+        un[j] = val[q];
+      }
     }
     for (int t = 0; t < cnt; t++) {
       const int kt = rl(k, t), st = rl(ks, t), et = rl(ke, t);
-      const double pt = rld(piv, t);
-      const int c = cn;
-      const double u = un;
-      if (t + 1 < cnt) {  // prefetch the next pivot row's first chunk
-        int s1 = rl(ks, t + 1), e1 = rl(ke, t + 1), q1 = s1 + lane;
-        cn = 0;
-        un = 0.0;
-        if (q1 < e1) cn = wcol[q1], un = val[q1];
+      const T pt = rld(piv, t);
+#pragma unroll
+      for (int j = 0; j < NCH; j++) cc[j] = cn[j], uc[j] = un[j];
+      if (t + 1 < cnt) {
+        const int s1 = rl(ks, t + 1), e1 = rl(ke, t + 1);
+#pragma unroll
+        for (int j = 0; j < NCH; j++) {
+          int q = min(s1 + lane + 64 * j, e1 - 1);
+          cn[j] = wcol[q];
+          un[j] = val[q];
+        }
       }
-      const double l = w[kt] / pt;
-      int q = st + lane;
-      if (q < et) {
-        double prod = l * u;
-        w[c] = w[c] - prod;
+      const T l = w[kt] / pt;
+#pragma unroll
+      for (int j = 0; j < NCH; j++) {
+        if (st + lane + 64 * j < et) {
+          T prod = l * uc[j];
+          w[cc[j]] = w[cc[j]] - prod;
+        }
       }
-      for (q += 64; q < et; q += 64) {
+      for (int q = st + lane + 64 * NCH; q < et; q += 64) {
         int c2 = wcol[q];
-        double prod = l * val[q];
+        T prod = l * val[q];
         w[c2] = w[c2] - prod;
       }
       if (lane == 0) w[kt] = l;
@@ -126,7 +151,7 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
   }
   if (LEAF_ROW) {
     if (lane == 0) {
-      double d = w[i];
+      T d = w[i];
       if (d < eps && d > -eps) d = d < 0 ? -eps : eps;
       w[i] = d;
     }
@@ -135,88 +160,76 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
   for (int p = p0 + lane; p < p1; p += 64) {
     int c = wcol[p];
     val[p] = w[c];
-    w[c] = 0.0;
+    w[c] = T(0);
   }
   wave_lds_fence();
 }
 
 // One workgroup per leaf; rows of a level are spread over the waves, levels
 // are separated by workgroup barriers.
-__global__ __launch_bounds__(256) void k_leaf_rows(
-    const LeafDev *__restrict__ leaves, const int *__restrict__ rowptr,
-    const int *__restrict__ diag, const int *__restrict__ wcol,
-    const int *__restrict__ levptr, const int *__restrict__ levrows,
-    double *__restrict__ arena, int wsz, double eps) {
-  extern __shared__ double lds[];
+extern __shared__ __align__(16) char lds_raw[];
+template <typename T>
+__global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ leaves,
+                                                   int wsz, T eps) {
+  T *lds = (T *)lds_raw;
   const LeafDev L = leaves[blockIdx.x];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
-  double *w = lds + (size_t)wave * wsz;
-  for (int c = lane; c < L.leaf_size + L.nbc; c += 64) w[c] = 0.0;
+  T *w = lds + (size_t)wave * wsz;
+  for (int c = lane; c < L.leaf_size + L.nbc; c += 64) w[c] = T(0);
   wave_lds_fence();
-  const int *rp = rowptr + L.rp_off;
-  const int *dg = diag + L.rp_off;
-  const int *wc = wcol + L.idx_off;
-  double *v = arena + L.val_off;
-  const int *lp = levptr + L.lev_off;
-  const int *lr = levrows + L.levrows_off;
+  const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol;
+  T *v = (T *)L.vals;
+  const int *lp = L.levptr, *lr = L.levrows;
   for (int lev = 0; lev < L.nlev; lev++) {
     for (int t = lp[lev] + wave; t < lp[lev + 1]; t += W)
-      factor_row<true>(lr[t], rp, dg, wc, v, w, lane, eps);
+      factor_row<T, true>(lr[t], rp, dg, wc, v, w, lane, eps);
     __syncthreads();
   }
 }
-void launch_leaf_rows(const LeafDev *leaves, int nleaves, const int *rowptr,
-                      const int *diag, const int *wcol, const int *levptr,
-                      const int *levrows, double *arena, int wsz, int waves,
-                      double eps, hipStream_t st) {
+template <typename T>
+void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, double eps,
+                      hipStream_t st) {
   if (nleaves <= 0) return;
-  size_t lds = (size_t)wsz * waves * sizeof(double);
-  (void)hipFuncSetAttribute((const void *)k_leaf_rows,
+  size_t lds = (size_t)wsz * waves * sizeof(T);
+  (void)hipFuncSetAttribute((const void *)k_leaf_rows<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_leaf_rows, dim3(nleaves), dim3(64 * waves), lds, st,
-                     leaves, rowptr, diag, wcol, levptr, levrows, arena, wsz,
-                     eps);
+  hipLaunchKernelGGL(k_leaf_rows<T>, dim3(nleaves), dim3(64 * waves), lds, st, leaves, wsz,
+                     (T)eps);
 }
 
 // Separator rows of the leaves: each wave factors one boundary row (its L_SL
 // row) and writes its Schur contribution row into S.
-__global__ __launch_bounds__(256) void k_leaf_sep(
-    const LeafDev *__restrict__ leaves, const int *__restrict__ wg_leaf,
-    const int *__restrict__ wg_first, const int *__restrict__ rowptr,
-    const int *__restrict__ diag, const int *__restrict__ wcol,
-    const int *__restrict__ brow, double *__restrict__ arena,
-    double *__restrict__ S, int wsz) {
-  extern __shared__ double lds[];
+template <typename T>
+__global__ __launch_bounds__(512) void k_leaf_sep(const LeafDev *__restrict__ leaves,
+                                                  const int *__restrict__ wg_leaf,
+                                                  const int *__restrict__ wg_first, int wsz) {
+  T *lds = (T *)lds_raw;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const LeafDev L = leaves[wg_leaf[blockIdx.x]];
   const int ri = wg_first[blockIdx.x] + wave;
   if (ri >= L.nbr) return;
-  double *w = lds + (size_t)wave * wsz;
+  T *w = lds + (size_t)wave * wsz;
   const int ls = L.leaf_size;
-  for (int c = lane; c < ls + L.nbc; c += 64) w[c] = 0.0;
+  for (int c = lane; c < ls + L.nbc; c += 64) w[c] = T(0);
   wave_lds_fence();
-  const int r = brow[L.brow_off + ri];
-  factor_row<false>(r, rowptr + L.rp_off, diag + L.rp_off, wcol + L.idx_off,
-                    arena + L.val_off, w, lane, 0.0);
-  double *Srow = S + L.S_off + (long long)ri * L.nbc;
+  const int r = L.brow[ri];
+  factor_row<T, false>(r, L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0));
+  T *Srow = (T *)L.S + (long long)ri * L.nbc;
   for (int j = lane; j < L.nbc; j += 64) {
     Srow[j] = w[ls + j];
-    w[ls + j] = 0.0;
+    w[ls + j] = T(0);
   }
 }
-void launch_leaf_sep(const LeafDev *leaves, const int *wg_leaf,
-                     const int *wg_first, int nwg, const int *rowptr,
-                     const int *diag, const int *wcol, const int *brow,
-                     double *arena, double *S, int wsz, int waves,
-                     hipStream_t st) {
+template <typename T>
+void launch_leaf_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
+                     int wsz, int waves, hipStream_t st) {
   if (nwg <= 0) return;
-  size_t lds = (size_t)wsz * waves * sizeof(double);
-  (void)hipFuncSetAttribute((const void *)k_leaf_sep,
+  size_t lds = (size_t)wsz * waves * sizeof(T);
+  (void)hipFuncSetAttribute((const void *)k_leaf_sep<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_leaf_sep, dim3(nwg), dim3(64 * waves), lds, st, leaves,
-                     wg_leaf, wg_first, rowptr, diag, wcol, brow, arena, S,
-                     wsz);
+  hipLaunchKernelGGL(k_leaf_sep<T>, dim3(nwg), dim3(64 * waves), lds, st, leaves, wg_leaf,
+                     wg_first, wsz);
 }
 
 // Slot of separator entry (bi, lr) x (bj, lc) in the dense arena.
@@ -235,63 +248,274 @@ __device__ __forceinline__ long long sep_index(const DenseDev &d, int bi,
 
 // One wave per target separator row; contributions are added leaf by leaf
 // in ascending rank order (deterministic; no atomics).
+// A target row and all its sources belong to one matrix, whose separator
+// tables and dense arena the sources' leaf descriptors carry.
+template <typename T>
 __global__ __launch_bounds__(256) void k_assemble_schur(
     int ntarget, const int *__restrict__ trow_blk,
     const int *__restrict__ trow_loc, const int *__restrict__ tptr,
     const int *__restrict__ tsrc_leaf, const int *__restrict__ tsrc_ri,
-    const LeafDev *__restrict__ leaves, const int *__restrict__ bcol_blk,
-    const int *__restrict__ bcol_loc, const double *__restrict__ S,
-    double *dense, DenseDev dd) {
+    const LeafDev *__restrict__ leaves) {
   const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (t >= ntarget) return;
   const int bi = trow_blk[t], lr = trow_loc[t];
   for (int s = tptr[t]; s < tptr[t + 1]; s++) {
     const LeafDev L = leaves[tsrc_leaf[s]];
-    const double *Srow = S + L.S_off + (long long)tsrc_ri[s] * L.nbc;
+    const DenseDev dd = *L.dd;
+    T *dense = (T *)L.dense;
+    const T *Srow = (const T *)L.S + (long long)tsrc_ri[s] * L.nbc;
     for (int j = lane; j < L.nbc; j += 64) {
-      double v = Srow[j];
-      if (v != 0.0) {
-        long long idx = sep_index(dd, bi, lr, bcol_blk[L.bcol_off + j],
-                                  bcol_loc[L.bcol_off + j]);
+      T v = Srow[j];
+      if (v != T(0)) {
+        long long idx = sep_index(dd, bi, lr, L.bcol_blk[j], L.bcol_loc[j]);
         dense[idx] += v;
       }
     }
     __threadfence();
   }
 }
-void launch_assemble_schur(int ntarget, const int *trow_blk,
-                           const int *trow_loc, const int *tptr,
-                           const int *tsrc_leaf, const int *tsrc_ri,
-                           const LeafDev *leaves, const int *bcol_blk,
-                           const int *bcol_loc, const double *S, double *dense,
-                           DenseDev dd, hipStream_t st) {
+template <typename T>
+void launch_assemble_schur(int ntarget, const int *trow_blk, const int *trow_loc,
+                           const int *tptr, const int *tsrc_leaf, const int *tsrc_ri,
+                           const LeafDev *leaves, hipStream_t st) {
   if (ntarget <= 0) return;
-  hipLaunchKernelGGL(k_assemble_schur, dim3((ntarget + 3) / 4), dim3(256), 0,
-                     st, ntarget, trow_blk, trow_loc, tptr, tsrc_leaf, tsrc_ri,
-                     leaves, bcol_blk, bcol_loc, S, dense, dd);
+  hipLaunchKernelGGL(k_assemble_schur<T>, dim3((ntarget + 3) / 4), dim3(256), 0, st, ntarget,
+                     trow_blk, trow_loc, tptr, tsrc_leaf, tsrc_ri, leaves);
 }
 
 // ---------------------------------------------------------------------------
-// dense: grouped GEMM on MFMA f64 16x16x4
+// dense: grouped GEMM on MFMA f64 4x4x4 (4 blocks)
 // ---------------------------------------------------------------------------
-// Workgroup tile 64 x 64, 4 waves each 32 x 32 (2 x 2 MFMA blocks), K chunk
-// 16 staged through LDS.  SUB: C -= sum_seg A_seg B_seg (K concatenated over
-// the segments, in order).  ASSIGN: C = A B with one segment; C may alias A
-// (N <= 64) or B (M <= 64) because each workgroup reads its whole A row
-// panel / B column panel before it writes.
-// MFMA operands are swapped (A-operand <- B tile, B-operand <- A tile) so
-// that the 16 lanes of a group hold 16 consecutive rows of one column of C
-// (coalesced epilogue).  v_mfma_f64_16x16x4f64 layouts on gfx950 (checked
-// by a lane-layout probe): A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
-// D[i][j] in lane j + 16 (i % 4), register i / 4.
-constexpr int TB = 64, KB = 16, PADA = 4;
-template <bool ASSIGN>
-__global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs,
+// On gfx950 v_mfma_f64_4x4x4f64 sustains twice the rate of
+// v_mfma_f64_16x16x4f64 (measured 74 vs 38 TFLOP/s with register operands).
+// Lane layouts (checked by lane-layout probes): A[m][k] in lane m + 16k,
+// B[k][n] in lane n + 16k (m, n < 16) -- the same registers as the 16x16x4
+// instruction -- but the 4 blocks compute only the 4 diagonal 4 x 4 blocks of
+// that 16 x 16 product: D_blk[i][j] in lane 4 blk + 16 i + j.  A full 16 x 16
+// block therefore takes 4 instructions whose second operand is rotated by
+// 4 r lanes inside each 16-lane row (DPP row_ror:(16 - 4r)).
+// Operands are swapped (first operand <- B tile, second <- A tile), so that
+// for rotation r lane l holds C[m0 + 4((blk + r) % 4) + l % 4][n0 + 4 blk + l / 16]
+// with blk = (l / 4) % 4: 16 lanes of a row cover 16 consecutive rows of
+// one column of C (coalesced epilogue).
+//
+// Workgroup tile (32 BMB) x (32 BNB), 4 waves in 2 x 2, each (16 BMB) x
+// (16 BNB); K chunk 16; double-buffered LDS fed from a register prefetch that
+// stays in flight across the MFMA block (one barrier per chunk).
+// MODE 0: C -= sum_seg A_seg B_seg (K concatenated over the segments).
+// MODE 1 / 2: C = A B / C = -A B (one segment); C may alias A (N <= tile
+// width) or B (M <= tile height): a workgroup reads its whole A row panel /
+// B column panel before its epilogue writes.
+// Workgroup ids are remapped so that each XCD (ids go round-robin over the 8
+// XCDs) works on a contiguous run of tiles, which share operand panels in
+// that XCD's L2.
+constexpr int KB = 16;
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = bid & 7, i = bid >> 3;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+typedef float f4 __attribute__((ext_vector_type(4)));
+template <typename T, int MODE, int BMB, int BNB>
+__global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ probs,
                                               int nprob,
-                                              const GemmSeg *__restrict__ segs) {
-  __shared__ double As[KB][TB + PADA];
-  __shared__ double Bs[KB][TB + PADA];
+                                              const GemmSegT<T> *__restrict__ segs) {
+  constexpr bool F64 = std::is_same<T, double>::value;
+  constexpr int BM = 32 * BMB, BN = 32 * BNB;
+  constexpr int NA = BM * KB / 256, NB = BN * KB / 256;
+  __shared__ T As[2][KB][BM + 1];
+  __shared__ T Bs[2][KB][BN + 1];
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmProbT<T> P = probs[lo];
+  // grouped order: consecutive tile ids walk down a strip of GM tile rows
+  // before moving right, so the tiles in flight on one XCD share both their
+  // A and their B panels in its L2
+  constexpr int GM = 8;
+  const int lt = tile - P.tile0;
+  const int tiles_m = (P.M + BM - 1) / BM;
+  const int grp = lt / (GM * P.tiles_n), first = grp * GM;
+  const int gsz = min(GM, tiles_m - first), li = lt - grp * GM * P.tiles_n;
+  const int m0 = (first + li % gsz) * BM, n0 = (li / gsz) * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 16 * BMB, wn = (wave >> 1) * 16 * BNB;
+  const int l16 = lane & 15, lk = lane >> 4;
+  // FP64: four 4x4x4 products per 16 x 16 block (one per rotation r);
+  // FP32: one 16x16x4 product per block, 4 output registers
+  double acc[BMB][BNB][4];
+  f4 accf[BMB][BNB];
+#pragma unroll
+  for (int a = 0; a < BMB; a++)
+#pragma unroll
+    for (int b = 0; b < BNB; b++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[a][b][r] = 0.0;
+      accf[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  T ra[NA], rb[NB];
+  // Loads are unconditional and unmasked with addresses clamped into the
+  // operands, so they stay branch-free and in flight across the MFMA block:
+  // a clamped row of A (column of B) only feeds a row (column) of the tile
+  // that is never stored; the K tail is zeroed in LDS after the store.
+  auto load = [&](int sgi, int kk0) {
+    const GemmSegT<T> S = segs[P.seg0 + sgi];
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+      int e = tid + 256 * i;
+      int gm = min(m0 + e % BM, P.M - 1), gk = min(kk0 + e / BM, S.K - 1);
+      ra[i] = gptr(S.A)[gm + (long long)gk * S.lda];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      int e = tid + 256 * i;
+      int gk = min(kk0 + (e & 15), S.K - 1), gn = min(n0 + (e >> 4), P.N - 1);
+      rb[i] = gptr(S.B)[gk + (long long)gn * S.ldb];
+    }
+  };
+  auto store = [&](int buf, int kvalid) {
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+      int e = tid + 256 * i;
+      As[buf][e / BM][e % BM] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      int e = tid + 256 * i;
+      Bs[buf][e & 15][e >> 4] = rb[i];
+    }
+    if (kvalid < KB) {  // uniform: last chunk of a segment whose K % 16 != 0
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        int e = tid + 256 * i;
+        if ((e & 15) >= kvalid) Bs[buf][e & 15][e >> 4] = T(0);
+      }
+    }
+  };
+  auto advance = [&](int &sgi, int &kk0) {
+    kk0 += KB;
+    while (sgi < P.nseg && kk0 >= segs[P.seg0 + sgi].K) {
+      sgi++;
+      kk0 = 0;
+    }
+  };
+  int sg = 0, k0 = 0;
+  if (P.nseg == 0) return;
+  load(sg, k0);
+  store(0, segs[P.seg0].K - k0);
+  __syncthreads();
+  int buf = 0;
+  for (;;) {
+    int nsg = sg, nk0 = k0;
+    advance(nsg, nk0);
+    const bool more = nsg < P.nseg;
+    if (more) load(nsg, nk0);
+#pragma unroll
+    for (int kk = 0; kk < KB; kk += 4) {
+      T bv[BNB], av[BMB];
+#pragma unroll
+      for (int b = 0; b < BNB; b++) bv[b] = Bs[buf][kk + lk][wn + b * 16 + l16];
+#pragma unroll
+      for (int a = 0; a < BMB; a++) av[a] = As[buf][kk + lk][wm + a * 16 + l16];
+      if constexpr (F64) {
+        double avr[BMB][4];
+#pragma unroll
+        for (int a = 0; a < BMB; a++) {
+          avr[a][0] = av[a];
+          avr[a][1] = dpp_d<0x12C>(av[a]);  // lane i <- lane i + 4 (row of 16)
+          avr[a][2] = dpp_d<0x128>(av[a]);  // lane i <- lane i + 8
+          avr[a][3] = dpp_d<0x124>(av[a]);  // lane i <- lane i + 12
+        }
+#pragma unroll
+        for (int a = 0; a < BMB; a++)
+#pragma unroll
+          for (int b = 0; b < BNB; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+              acc[a][b][r] = __builtin_amdgcn_mfma_f64_4x4x4f64(bv[b], avr[a][r], acc[a][b][r], 0, 0, 0);
+      } else {
+#pragma unroll
+        for (int a = 0; a < BMB; a++)
+#pragma unroll
+          for (int b = 0; b < BNB; b++)
+            accf[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[b], av[a], accf[a][b], 0, 0, 0);
+      }
+    }
+    if (!more) break;
+    store(buf ^ 1, segs[P.seg0 + nsg].K - nk0);
+    __syncthreads();
+    buf ^= 1;
+    sg = nsg;
+    k0 = nk0;
+  }
+  const int blk = (lane >> 2) & 3;
+#pragma unroll
+  for (int a = 0; a < BMB; a++)
+#pragma unroll
+    for (int b = 0; b < BNB; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int m, n;
+        T v;
+        if constexpr (F64) {
+          m = m0 + wm + a * 16 + 4 * ((blk + r) & 3) + (lane & 3);
+          n = n0 + wn + b * 16 + 4 * blk + lk;
+          v = acc[a][b][r];
+        } else {
+          // D = B^T A^T: register r of lane l holds C[l % 16][4 (l / 16) + r]
+          m = m0 + wm + a * 16 + l16;
+          n = n0 + wn + b * 16 + 4 * lk + r;
+          v = accf[a][b][r];
+        }
+        if (m < P.M && n < P.N) {
+          auto c = gptr(P.C) + m + (long long)n * P.ldc;
+          if (MODE == 1) *c = v;
+          else if (MODE == 2) *c = -v;
+          else *c = *c - v;
+        }
+      }
+}
+template <typename T>
+void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
+                 int total_tiles, int mode, hipStream_t st) {
+  if (total_tiles <= 0 || nprob <= 0) return;
+  if (mode == 1)
+    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  else if (mode == 2)
+    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  else
+    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+}
+template <typename T>
+void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
+                    int total_tiles, hipStream_t st) {
+  if (total_tiles <= 0 || nprob <= 0) return;
+  hipLaunchKernelGGL((k_gemm<T, 0, 4, 4>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+}
+
+// ---------------------------------------------------------------------------
+// dense: grouped GEMV (the dense solves)
+// ---------------------------------------------------------------------------
+// One workgroup per 64 rows of a problem; lane = row, the 4 waves take
+// interleaved columns and are combined in LDS (fixed order: deterministic).
+// A is read once, coalesced down its columns; x through scalar loads.
+// nrhs columns are processed in groups of 8 (A is re-read per group).
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_gemv(const GemvProbT<T> *__restrict__ probs,
+                                              int nprob,
+                                              const GemvSegT<T> *__restrict__ segs) {
+  __shared__ T red[4][8][64];
   const int tile = blockIdx.x;
   int lo = 0, hi = nprob - 1;
   while (lo < hi) {
@@ -299,81 +523,92 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
     if (probs[mid].tile0 <= tile) lo = mid;
     else hi = mid - 1;
   }
-  const GemmProb P = probs[lo];
-  const int lt = tile - P.tile0;
-  const int m0 = (lt / P.tiles_n) * TB, n0 = (lt % P.tiles_n) * TB;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  const int l16 = lane & 15, lk = lane >> 4;
-  d4 acc[2][2];
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
-  for (int sg = 0; sg < P.nseg; sg++) {
-    const GemmSeg S = segs[P.seg0 + sg];
-    for (int k0 = 0; k0 < S.K; k0 += KB) {
+  const GemvProbT<T> P = probs[lo];
+  const int nrhs = P.nrhs;
+  const int m0 = (tile - P.tile0) * 64;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int m = m0 + lane, mc = min(m, P.M - 1);
+  for (int c0 = 0; c0 < nrhs; c0 += 8) {
+    const int nc = min(8, nrhs - c0);
+    T acc[8];
 #pragma unroll
-      for (int e = tid; e < TB * KB; e += 256) {
-        int m = e & 63, k = e >> 6, gm = m0 + m, gk = k0 + k;
-        As[k][m] = (gm < P.M && gk < S.K) ? S.A[gm + (long long)gk * S.lda] : 0.0;
-      }
+    for (int c = 0; c < 8; c++) acc[c] = T(0);
+    for (int sg = 0; sg < P.nseg; sg++) {
+      const GemvSegT<T> S = segs[P.seg0 + sg];
+      int kb = 0, ke = S.K;
+      if (P.tri == 1) ke = min(S.K, m0 + 64);
+      if (P.tri == 2) kb = m0;
+      auto A = gptr(S.A) + mc;
+      auto X = gptr(S.x) + (long long)c0 * S.ldx;
+      int k = kb + w;
+      for (; k + 12 < ke; k += 16) {
+        T a0 = A[(long long)k * S.lda], a1 = A[(long long)(k + 4) * S.lda];
+        T a2 = A[(long long)(k + 8) * S.lda], a3 = A[(long long)(k + 12) * S.lda];
 #pragma unroll
-      for (int e = tid; e < TB * KB; e += 256) {
-        int k = e & 15, n = e >> 4, gk = k0 + k, gn = n0 + n;
-        Bs[k][n] = (gk < S.K && gn < P.N) ? S.B[gk + (long long)gn * S.ldb] : 0.0;
+        for (int c = 0; c < 8; c++)
+          if (c < nc) {
+            auto xc = X + (long long)c * S.ldx;
+            acc[c] += a0 * xc[k] + a1 * xc[k + 4] + a2 * xc[k + 8] + a3 * xc[k + 12];
+          }
       }
-      __syncthreads();
+      for (; k < ke; k += 4) {
+        T a0 = A[(long long)k * S.lda];
 #pragma unroll
-      for (int kk = 0; kk < KB; kk += 4) {
-        double av[2], bv[2];
-        for (int b = 0; b < 2; b++) bv[b] = Bs[kk + lk][wn + b * 16 + l16];
-        for (int a = 0; a < 2; a++) av[a] = As[kk + lk][wm + a * 16 + l16];
-        for (int a = 0; a < 2; a++)
-          for (int b = 0; b < 2; b++)
-            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b], av[a],
-                                                             acc[a][b], 0, 0, 0);
+        for (int c = 0; c < 8; c++)
+          if (c < nc) acc[c] += a0 * X[(long long)c * S.ldx + k];
       }
-      __syncthreads();
     }
-  }
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++)
-      for (int r = 0; r < 4; r++) {
-        int m = m0 + wm + a * 16 + l16;
-        int n = n0 + wn + b * 16 + lk + 4 * r;
-        if (m < P.M && n < P.N) {
-          double *c = P.C + m + (long long)n * P.ldc;
-          if (ASSIGN) *c = acc[a][b][r];
-          else *c = *c - acc[a][b][r];
-        }
+#pragma unroll
+    for (int c = 0; c < 8; c++) red[w][c][lane] = acc[c];
+    __syncthreads();
+    if (w == 0 && m < P.M) {
+      for (int c = 0; c < nc; c++) {
+        T v = red[0][c][lane] + red[1][c][lane] + red[2][c][lane] + red[3][c][lane];
+        auto y = gptr(P.y) + m + (long long)(c0 + c) * P.ldy;
+        if (MODE == 1) *y = v;
+        else *y = *y - v;
       }
+    }
+    __syncthreads();
+  }
 }
-void launch_gemm(const GemmProb *probs, int nprob, const GemmSeg *segs,
-                 int total_tiles, bool assign, hipStream_t st) {
+template <typename T>
+void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
+                 int total_tiles, int mode, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
-  if (assign)
-    hipLaunchKernelGGL(k_gemm<true>, dim3(total_tiles), dim3(256), 0, st,
-                       probs, nprob, segs);
+  if (mode == 1)
+    hipLaunchKernelGGL((k_gemv<T, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
   else
-    hipLaunchKernelGGL(k_gemm<false>, dim3(total_tiles), dim3(256), 0, st,
-                       probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemv<T, 0>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
 }
 
 // ---------------------------------------------------------------------------
 // dense: diagonal block LU (no pivoting, as cusolverDnDgetrf with a null
 // ipiv / LAPACKE_mkl_dgetrfnp) and the inverses of its triangular factors.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_diag_lu(const DiagProb *__restrict__ probs) {
-  __shared__ double a[64 * 64];   // col-major, ld 64
-  __shared__ double xs[64 * 64];  // inverse being built, col-major
-  const DiagProb P = probs[blockIdx.x];
+// The block is padded to 64 x 64 with a unit diagonal, so the inverses of
+// the padded factors carry the nb x nb inverses in their leading corner.
+// Inverses by recursive doubling over 256 threads:
+//   inv([[A, B], [0, C]]) = [[A^-1, -A^-1 B C^-1], [0, C^-1]]   (upper)
+//   inv([[A, 0], [B, C]]) = [[A^-1, 0], [-C^-1 B A^-1, C^-1]]   (unit lower)
+// with block size s = 1, 2, 4, ..., 32; each level is two small matrix
+// products over all pairs of diagonal blocks at once.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_lu(const DiagProbT<T> *__restrict__ probs) {
+  __shared__ T a[64 * 64];   // LU factors, col-major, ld 64
+  __shared__ T xu[64 * 64];  // U^-1
+  __shared__ T xl[64 * 64];  // L^-1
+  __shared__ T t[64 * 64];   // products: upper-right blocks for U, lower-left for L
+  const DiagProbT<T> P = probs[blockIdx.x];
   const int nb = P.nb, tid = threadIdx.x;
   for (int e = tid; e < 64 * 64; e += 256) {
     int r = e & 63, c = e >> 6;
-    a[e] = (r < nb && c < nb) ? P.D[r + (long long)c * P.ld] : 0.0;
+    a[e] = (r < nb && c < nb) ? P.D[r + (long long)c * P.ld] : (r == c ? T(1) : T(0));
   }
   __syncthreads();
   for (int j = 0; j < nb; j++) {
-    const double piv = a[j + j * 64];
+    const T piv = a[j + j * 64];
     for (int r = j + 1 + tid; r < nb; r += 256) a[r + j * 64] /= piv;
     __syncthreads();
     const int w = nb - j - 1;
@@ -387,167 +622,122 @@ __global__ __launch_bounds__(256) void k_diag_lu(const DiagProb *__restrict__ pr
     int r = e % nb, c = e / nb;
     P.D[r + (long long)c * P.ld] = a[r + c * 64];
   }
-  // U^-1, one column per thread
-  if (tid < 64) {
-    const int c = tid;
-    double *x = xs + c * 64;
-    for (int r = 0; r < 64; r++) x[r] = 0.0;
-    if (c < nb) {
-      x[c] = 1.0 / a[c + c * 64];
-      for (int r = c - 1; r >= 0; r--) {
-        double s = 0.0;
-        for (int t = r + 1; t <= c; t++) s += a[r + t * 64] * x[t];
-        x[r] = -s / a[r + r * 64];
-      }
-    }
+  for (int e = tid; e < 64 * 64; e += 256) {
+    int r = e & 63, c = e >> 6;
+    xu[e] = r == c ? T(1) / a[e] : T(0);
+    xl[e] = r == c ? T(1) : T(0);
   }
   __syncthreads();
-  for (int e = tid; e < 64 * 64; e += 256) P.Uinv[e] = xs[e];
-  __syncthreads();
-  // L^-1 (unit lower)
-  if (tid < 64) {
-    const int c = tid;
-    double *x = xs + c * 64;
-    for (int r = 0; r < 64; r++) x[r] = 0.0;
-    if (c < nb) {
-      x[c] = 1.0;
-      for (int r = c + 1; r < nb; r++) {
-        double s = 0.0;
-        for (int t = c; t < r; t++) s += a[r + t * 64] * x[t];
-        x[r] = -s;
+  for (int sz = 1; sz < 64; sz <<= 1) {
+    const int npair = 64 / (2 * sz), per = sz * sz, tot = npair * per;
+    // t_U = B_U C^-1 (upper-right block), t_L = B_L A^-1 (lower-left block)
+    for (int e = tid; e < 2 * tot; e += 256) {
+      const bool up = e < tot;
+      const int f = up ? e : e - tot;
+      const int p = f / per, i = f % per % sz, j = f % per / sz;
+      const int r0 = p * 2 * sz, c0 = r0 + sz;
+      T acc = T(0);
+      if (up) {  // U[r0+i][c0+k] * xu[c0+k][c0+j], k <= j
+        for (int k = 0; k <= j; k++) acc += a[(r0 + i) + (c0 + k) * 64] * xu[(c0 + k) + (c0 + j) * 64];
+        t[(r0 + i) + (c0 + j) * 64] = acc;
+      } else {   // L[c0+i][r0+k] * xl[r0+k][r0+j], k >= j
+        for (int k = j; k < sz; k++) acc += a[(c0 + i) + (r0 + k) * 64] * xl[(r0 + k) + (r0 + j) * 64];
+        t[(c0 + i) + (r0 + j) * 64] = acc;
       }
     }
-  }
-  __syncthreads();
-  for (int e = tid; e < 64 * 64; e += 256) P.Linv[e] = xs[e];
-}
-void launch_diag_lu(const DiagProb *probs, int nprob, hipStream_t st) {
-  if (nprob <= 0) return;
-  hipLaunchKernelGGL(k_diag_lu, dim3(nprob), dim3(256), 0, st, probs);
-}
-
-// ---------------------------------------------------------------------------
-// dense triangular solves of one separator's diagonal block (one workgroup
-// per separator), blocked by 64 with the stored diagonal-block inverses.
-// ---------------------------------------------------------------------------
-constexpr int TRI_THREADS = 512;
-__global__ __launch_bounds__(TRI_THREADS) void k_tri_fwd(const TriProb *__restrict__ probs,
-                                                         int nrhs) {
-  __shared__ double y[64];
-  const TriProb P = probs[blockIdx.x];
-  const int tid = threadIdx.x;
-  for (int col = 0; col < nrhs; col++) {
-    double *b = P.b + (long long)col * P.ldb;
-    for (int k0 = 0; k0 < P.s; k0 += 64) {
-      const int nbk = min(64, P.s - k0);
-      const double *Li = P.inv + (long long)(k0 / 64) * 8192;
-      if (tid < nbk) {
-        double s = 0.0;
-        for (int c = 0; c <= tid; c++) s += Li[tid + c * 64] * b[k0 + c];
-        y[tid] = s;
+    __syncthreads();
+    for (int e = tid; e < 2 * tot; e += 256) {
+      const bool up = e < tot;
+      const int f = up ? e : e - tot;
+      const int p = f / per, i = f % per % sz, j = f % per / sz;
+      const int r0 = p * 2 * sz, c0 = r0 + sz;
+      T acc = T(0);
+      if (up) {  // -A^-1 t_U: xu[r0+i][r0+k] * t[r0+k][c0+j], k >= i
+        for (int k = i; k < sz; k++) acc += xu[(r0 + i) + (r0 + k) * 64] * t[(r0 + k) + (c0 + j) * 64];
+        xu[(r0 + i) + (c0 + j) * 64] = -acc;
+      } else {   // -C^-1 t_L: xl[c0+i][c0+k] * t[c0+k][r0+j], k <= i
+        for (int k = 0; k <= i; k++) acc += xl[(c0 + i) + (c0 + k) * 64] * t[(c0 + k) + (r0 + j) * 64];
+        xl[(c0 + i) + (r0 + j) * 64] = -acc;
       }
-      __syncthreads();
-      if (tid < nbk) b[k0 + tid] = y[tid];
-      for (int r = k0 + nbk + tid; r < P.s; r += TRI_THREADS) {
-        double s = 0.0;
-        const double *Dr = P.D + r + (long long)k0 * P.ld;
-        for (int c = 0; c < nbk; c++) s += Dr[(long long)c * P.ld] * y[c];
-        b[r] -= s;
-      }
-      __syncthreads();
     }
+    __syncthreads();
+  }
+  for (int e = tid; e < nb * nb; e += 256) {
+    int r = e % nb, c = e / nb;
+    P.Uinv[r + c * P.ldi] = xu[r + c * 64];
+    P.Linv[r + c * P.ldi] = xl[r + c * 64];
   }
 }
-__global__ __launch_bounds__(TRI_THREADS) void k_tri_bwd(const TriProb *__restrict__ probs,
-                                                         int nrhs) {
-  __shared__ double y[64];
-  const TriProb P = probs[blockIdx.x];
-  const int tid = threadIdx.x;
-  const int nblk = (P.s + 63) / 64;
-  for (int col = 0; col < nrhs; col++) {
-    double *b = P.b + (long long)col * P.ldb;
-    for (int kb = nblk - 1; kb >= 0; kb--) {
-      const int k0 = kb * 64, nbk = min(64, P.s - k0);
-      const double *Ui = P.inv + (long long)kb * 8192 + 4096;
-      if (tid < nbk) {
-        double s = 0.0;
-        for (int c = tid; c < nbk; c++) s += Ui[tid + c * 64] * b[k0 + c];
-        y[tid] = s;
-      }
-      __syncthreads();
-      if (tid < nbk) b[k0 + tid] = y[tid];
-      for (int r = tid; r < k0; r += TRI_THREADS) {
-        double s = 0.0;
-        const double *Dr = P.D + r + (long long)k0 * P.ld;
-        for (int c = 0; c < nbk; c++) s += Dr[(long long)c * P.ld] * y[c];
-        b[r] -= s;
-      }
-      __syncthreads();
-    }
-  }
-}
-void launch_tri_fwd(const TriProb *probs, int nprob, int nrhs, hipStream_t st) {
+template <typename T>
+void launch_diag_lu(const DiagProbT<T> *probs, int nprob, hipStream_t st) {
   if (nprob <= 0) return;
-  hipLaunchKernelGGL(k_tri_fwd, dim3(nprob), dim3(TRI_THREADS), 0, st, probs,
-                     nrhs);
-}
-void launch_tri_bwd(const TriProb *probs, int nprob, int nrhs, hipStream_t st) {
-  if (nprob <= 0) return;
-  hipLaunchKernelGGL(k_tri_bwd, dim3(nprob), dim3(TRI_THREADS), 0, st, probs,
-                     nrhs);
+  hipLaunchKernelGGL(k_diag_lu<T>, dim3(nprob), dim3(256), 0, st, probs);
 }
 
 // ---------------------------------------------------------------------------
 // solve: permutation, leaf triangular solves, boundary rhs assembly
 // ---------------------------------------------------------------------------
-__global__ void k_permute_in(const double *b, long long ldb, const int *order,
-                             double *bp, int n, int nrhs) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)n * nrhs) return;
-  int col = (int)(i / n), r = (int)(i % n);
-  bp[order[r] + (long long)col * n] = b[r + col * ldb];
+// One matrix per blockIdx.y: bp[vbase + order[r] + col ld] = b[r + col ldb].
+template <typename T>
+__global__ void k_permute_in(const PermDev *__restrict__ pd, const VecDev *__restrict__ vd,
+                             T *bp) {
+  const PermDev P = pd[blockIdx.y];
+  const VecDev V = vd[P.vec];
+  const long long tot = (long long)P.n * V.nrhs;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int col = (int)(i / P.n), r = (int)(i % P.n);
+    bp[V.base + P.order[r] + (long long)col * V.ld] = (T)P.b[r + col * P.ldb];
+  }
 }
-__global__ void k_permute_out(const double *bp, const int *order, double *x,
-                              long long ldx, int n, int nrhs) {
-  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)n * nrhs) return;
-  int col = (int)(i / n), r = (int)(i % n);
-  x[r + col * ldx] = bp[order[r] + (long long)col * n];
+template <typename T>
+__global__ void k_permute_out(const PermDev *__restrict__ pd, const VecDev *__restrict__ vd,
+                              const T *bp) {
+  const PermDev P = pd[blockIdx.y];
+  const VecDev V = vd[P.vec];
+  const long long tot = (long long)P.n * V.nrhs;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (long long)gridDim.x * blockDim.x) {
+    const int col = (int)(i / P.n), r = (int)(i % P.n);
+    P.x[r + col * P.ldx] = (double)bp[V.base + P.order[r] + (long long)col * V.ld];
+  }
 }
-void launch_permute_in(const double *b, long long ldb, const int *order,
-                       double *bp, int n, int nrhs, hipStream_t st) {
-  long long tot = (long long)n * nrhs;
-  hipLaunchKernelGGL(k_permute_in, dim3((unsigned)((tot + 255) / 256)), dim3(256),
-                     0, st, b, ldb, order, bp, n, nrhs);
+static unsigned perm_blocks(long long max_elems) {
+  long long b = (max_elems + 255) / 256;
+  return (unsigned)std::max(1LL, std::min(b, 2048LL));
 }
-void launch_permute_out(const double *bp, const int *order, double *x,
-                        long long ldx, int n, int nrhs, hipStream_t st) {
-  long long tot = (long long)n * nrhs;
-  hipLaunchKernelGGL(k_permute_out, dim3((unsigned)((tot + 255) / 256)), dim3(256),
-                     0, st, bp, order, x, ldx, n, nrhs);
+template <typename T>
+void launch_permute_in(const PermDev *pd, int nmat, const VecDev *vd, T *bp,
+                       long long max_elems, hipStream_t st) {
+  if (nmat <= 0 || max_elems <= 0) return;
+  hipLaunchKernelGGL(k_permute_in<T>, dim3(perm_blocks(max_elems), nmat), dim3(256), 0, st, pd,
+                     vd, bp);
+}
+template <typename T>
+void launch_permute_out(const PermDev *pd, int nmat, const VecDev *vd, const T *bp,
+                        long long max_elems, hipStream_t st) {
+  if (nmat <= 0 || max_elems <= 0) return;
+  hipLaunchKernelGGL(k_permute_out<T>, dim3(perm_blocks(max_elems), nmat), dim3(256), 0, st,
+                     pd, vd, bp);
 }
 
 // y_L = L_LL^-1 b_L (unit lower), level by level, one wave per row.
-__global__ __launch_bounds__(256) void k_leaf_fwd(
-    const LeafDev *__restrict__ leaves, const int *__restrict__ rowptr,
-    const int *__restrict__ diag, const int *__restrict__ wcol,
-    const int *__restrict__ levptr, const int *__restrict__ levrows,
-    const double *__restrict__ arena, double *bp, int n, int nrhs) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_leaf_fwd(const LeafDev *__restrict__ leaves,
+                                                  const VecDev *__restrict__ vd, T *bp) {
   const LeafDev L = leaves[blockIdx.x];
+  const VecDev V = vd[L.vec];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
-  const int *rp = rowptr + L.rp_off;
-  const int *dg = diag + L.rp_off;
-  const int *wc = wcol + L.idx_off;
-  const double *v = arena + L.val_off;
-  const int *lp = levptr + L.lev_off;
-  const int *lr = levrows + L.levrows_off;
+  const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol;
+  const T *v = (const T *)L.vals;
+  const int *lp = L.levptr, *lr = L.levrows;
   for (int lev = 0; lev < L.nlev; lev++) {
     for (int t = lp[lev] + wave; t < lp[lev + 1]; t += W) {
       const int i = lr[t];
-      for (int col = 0; col < nrhs; col++) {
-        double *b = bp + (long long)col * n + L.gstart;
-        double s = 0.0;
+      for (int col = 0; col < V.nrhs; col++) {
+        T *b = bp + V.base + (long long)col * V.ld + L.gstart;
+        T s = T(0);
         for (int p = rp[i] + lane; p < dg[i]; p += 64) s += v[p] * b[wc[p]];
         s = wave_sum(s);
         if (lane == 0) b[i] -= s;
@@ -556,101 +746,98 @@ __global__ __launch_bounds__(256) void k_leaf_fwd(
     __syncthreads();
   }
 }
-void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const int *rowptr,
-                     const int *diag, const int *wcol, const int *levptr,
-                     const int *levrows, const double *arena, double *bp,
-                     int n, int nrhs, hipStream_t st) {
+template <typename T>
+void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp,
+                     hipStream_t st) {
   if (nleaves <= 0) return;
-  hipLaunchKernelGGL(k_leaf_fwd, dim3(nleaves), dim3(256), 0, st, leaves,
-                     rowptr, diag, wcol, levptr, levrows, arena, bp, n, nrhs);
+  hipLaunchKernelGGL(k_leaf_fwd<T>, dim3(nleaves), dim3(256), 0, st, leaves, vd, bp);
 }
 
 // Boundary rows: cbuf[ri] = sum_k L_SL(r, k) y_k (kernel_sparse.cu:935-944).
-__global__ __launch_bounds__(256) void k_leaf_fwd_sep(
-    const LeafDev *__restrict__ leaves, const int *__restrict__ wg_leaf,
-    const int *__restrict__ wg_first, const int *__restrict__ rowptr,
-    const int *__restrict__ wcol, const int *__restrict__ brow,
-    const double *__restrict__ arena, const double *__restrict__ bp,
-    double *__restrict__ cbuf, int n, int nrhs, int cb_ld) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_leaf_fwd_sep(const LeafDev *__restrict__ leaves,
+                                                      const int *__restrict__ wg_leaf,
+                                                      const int *__restrict__ wg_first,
+                                                      const VecDev *__restrict__ vd,
+                                                      const T *__restrict__ bp,
+                                                      T *__restrict__ cbuf, int cb_ld) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const LeafDev L = leaves[wg_leaf[blockIdx.x]];
+  const VecDev V = vd[L.vec];
   const int ri = wg_first[blockIdx.x] + wave;
   if (ri >= L.nbr) return;
-  const int r = brow[L.brow_off + ri];
-  const int *rp = rowptr + L.rp_off;
-  const int *wc = wcol + L.idx_off;
-  const double *v = arena + L.val_off;
-  for (int col = 0; col < nrhs; col++) {
-    const double *b = bp + (long long)col * n + L.gstart;
-    double s = 0.0;
+  const int r = L.brow[ri];
+  const int *rp = L.rowptr, *wc = L.wcol;
+  const T *v = (const T *)L.vals;
+  for (int col = 0; col < V.nrhs; col++) {
+    const T *b = bp + V.base + (long long)col * V.ld + L.gstart;
+    T s = T(0);
     for (int p = rp[r] + lane; p < rp[r + 1]; p += 64) s += v[p] * b[wc[p]];
     s = wave_sum(s);
     if (lane == 0) cbuf[L.cb_off + ri + (long long)col * cb_ld] = s;
   }
 }
-void launch_leaf_fwd_sep(const LeafDev *leaves, const int *wg_leaf,
-                         const int *wg_first, int nwg, int waves,
-                         const int *rowptr, const int *wcol, const int *brow,
-                         const double *arena, const double *bp, double *cbuf,
-                         int n, int nrhs, int cb_ld, hipStream_t st) {
+template <typename T>
+void launch_leaf_fwd_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
+                         int waves, const VecDev *vd, const T *bp, T *cbuf, int cb_ld,
+                         hipStream_t st) {
   if (nwg <= 0) return;
-  hipLaunchKernelGGL(k_leaf_fwd_sep, dim3(nwg), dim3(64 * waves), 0, st,
-                     leaves, wg_leaf, wg_first, rowptr, wcol, brow, arena, bp,
-                     cbuf, n, nrhs, cb_ld);
+  hipLaunchKernelGGL(k_leaf_fwd_sep<T>, dim3(nwg), dim3(64 * waves), 0, st, leaves, wg_leaf,
+                     wg_first, vd, bp, cbuf, cb_ld);
 }
 
-// b_R -= contributions of the leaves touching row R, in ascending rank.
+// b_R -= contributions of the leaves touching row R, in ascending rank
+// (one thread per target row and rhs column; the target's matrix is that of
+// its sources).
+template <typename T>
 __global__ void k_assemble_rhs(int ntarget, const int *__restrict__ trow_glob,
                                const int *__restrict__ tptr,
                                const int *__restrict__ tsrc_leaf,
                                const int *__restrict__ tsrc_ri,
                                const LeafDev *__restrict__ leaves,
-                               const double *__restrict__ cbuf, int cb_ld,
-                               double *bp, int n, int nrhs) {
+                               const VecDev *__restrict__ vd,
+                               const T *__restrict__ cbuf, int cb_ld, T *bp, int max_rhs) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)ntarget * nrhs) return;
+  if (i >= (long long)ntarget * max_rhs) return;
   const int t = (int)(i % ntarget), col = (int)(i / ntarget);
-  double *b = bp + (long long)col * n + trow_glob[t];
-  double acc = *b;
+  const VecDev V = vd[leaves[tsrc_leaf[tptr[t]]].vec];
+  if (col >= V.nrhs) return;
+  T *b = bp + V.base + (long long)col * V.ld + trow_glob[t];
+  T acc = *b;
   for (int s = tptr[t]; s < tptr[t + 1]; s++)
     acc -= cbuf[leaves[tsrc_leaf[s]].cb_off + tsrc_ri[s] + (long long)col * cb_ld];
   *b = acc;
 }
+template <typename T>
 void launch_assemble_rhs(int ntarget, const int *trow_glob, const int *tptr,
-                         const int *tsrc_leaf, const int *tsrc_ri,
-                         const LeafDev *leaves, const double *cbuf, int cb_ld,
-                         double *bp, int n, int nrhs, hipStream_t st) {
-  long long tot = (long long)ntarget * nrhs;
+                         const int *tsrc_leaf, const int *tsrc_ri, const LeafDev *leaves,
+                         const VecDev *vd, const T *cbuf, int cb_ld, T *bp, int max_rhs,
+                         hipStream_t st) {
+  long long tot = (long long)ntarget * max_rhs;
   if (tot <= 0) return;
-  hipLaunchKernelGGL(k_assemble_rhs, dim3((unsigned)((tot + 255) / 256)),
-                     dim3(256), 0, st, ntarget, trow_glob, tptr, tsrc_leaf,
-                     tsrc_ri, leaves, cbuf, cb_ld, bp, n, nrhs);
+  hipLaunchKernelGGL(k_assemble_rhs<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                     ntarget, trow_glob, tptr, tsrc_leaf, tsrc_ri, leaves, vd, cbuf, cb_ld, bp,
+                     max_rhs);
 }
 
 // x_L = U_LL^-1 (y_L - U_LS x_S), level by level (SnuMat_factorize.cu:331-344).
-__global__ __launch_bounds__(256) void k_leaf_bwd(
-    const LeafDev *__restrict__ leaves, const int *__restrict__ rowptr,
-    const int *__restrict__ diag, const int *__restrict__ wcol,
-    const int *__restrict__ ulevptr, const int *__restrict__ ulevrows,
-    const int *__restrict__ bcol_glob, const double *__restrict__ arena,
-    double *bp, int n, int nrhs) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_leaf_bwd(const LeafDev *__restrict__ leaves,
+                                                  const VecDev *__restrict__ vd, T *bp) {
   const LeafDev L = leaves[blockIdx.x];
+  const VecDev V = vd[L.vec];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
   const int ls = L.leaf_size;
-  const int *rp = rowptr + L.rp_off;
-  const int *dg = diag + L.rp_off;
-  const int *wc = wcol + L.idx_off;
-  const int *bg = bcol_glob + L.bcol_off;
-  const double *v = arena + L.val_off;
-  const int *lp = ulevptr + L.ulev_off;
-  const int *lr = ulevrows + L.ulevrows_off;
+  const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol, *bg = L.bcol_glob;
+  const T *v = (const T *)L.vals;
+  const int *lp = L.ulevptr, *lr = L.ulevrows;
   for (int lev = 0; lev < L.nulev; lev++) {
     for (int t = lp[lev] + wave; t < lp[lev + 1]; t += W) {
       const int i = lr[t];
-      for (int col = 0; col < nrhs; col++) {
-        double *b = bp + (long long)col * n;
-        double s = 0.0;
+      for (int col = 0; col < V.nrhs; col++) {
+        T *b = bp + V.base + (long long)col * V.ld;
+        T s = T(0);
         for (int p = dg[i] + 1 + lane; p < rp[i + 1]; p += 64) {
           int c = wc[p];
           long long g = c < ls ? (long long)L.gstart + c : (long long)bg[c - ls];
@@ -663,15 +850,151 @@ __global__ __launch_bounds__(256) void k_leaf_bwd(
     __syncthreads();
   }
 }
-void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const int *rowptr,
-                     const int *diag, const int *wcol, const int *ulevptr,
-                     const int *ulevrows, const int *bcol_glob,
-                     const double *arena, double *bp, int n, int nrhs,
+template <typename T>
+void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp,
                      hipStream_t st) {
   if (nleaves <= 0) return;
-  hipLaunchKernelGGL(k_leaf_bwd, dim3(nleaves), dim3(256), 0, st, leaves,
-                     rowptr, diag, wcol, ulevptr, ulevrows, bcol_glob, arena,
-                     bp, n, nrhs);
+  hipLaunchKernelGGL(k_leaf_bwd<T>, dim3(nleaves), dim3(256), 0, st, leaves, vd, bp);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 iterative refinement around the FP32 factors (original CSR order)
+// ---------------------------------------------------------------------------
+// One matrix per blockIdx.y throughout; FP64 arithmetic.
+// r = b - A x with 8 lanes per row.
+__global__ __launch_bounds__(256) void k_residual(const IrDev *__restrict__ tab) {
+  const IrDev M = tab[blockIdx.y];
+  const int sub = threadIdx.x & 7;
+  for (long long row = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 3; row < M.n;
+       row += ((long long)gridDim.x * blockDim.x) >> 3) {
+    const int p0 = M.rowptr[row], p1 = M.rowptr[row + 1];
+    for (int c = 0; c < M.nrhs; c++) {
+      const double *xc = M.x + (long long)c * M.ldx;
+      double s = 0.0;
+      for (int p = p0 + sub; p < p1; p += 8) s += M.val[p] * xc[M.colidx[p]];
+      s += __shfl_xor(s, 1, 8);
+      s += __shfl_xor(s, 2, 8);
+      s += __shfl_xor(s, 4, 8);
+      if (sub == 0) M.r[row + (long long)c * M.ldr] = M.b[row + (long long)c * M.ldb] - s;
+    }
+  }
+}
+static unsigned ir_blocks(long long rows_x_lanes) {
+  long long b = (rows_x_lanes + 255) / 256;
+  return (unsigned)std::max(1LL, std::min(b, 1024LL));
+}
+void launch_residual(const IrDev *tab, int nmat, int max_n, hipStream_t st) {
+  if (nmat <= 0 || max_n <= 0) return;
+  hipLaunchKernelGGL(k_residual, dim3(ir_blocks(8LL * max_n), nmat), dim3(256), 0, st, tab);
+}
+
+__device__ __forceinline__ double block_max(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  v = red[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); i++) v = fmax(v, red[i]);
+  return v;
+}
+// out[norm0 + c] = max_i |v[i + c ld]| for v = b (which 0), r (1) or x (2).
+// Nonnegative doubles order like their bit patterns, so an integer atomic
+// max on the bits is exact.
+__global__ __launch_bounds__(256) void k_colmax(const IrDev *__restrict__ tab, int which,
+                                                unsigned long long *out) {
+  __shared__ double red[4];
+  const IrDev M = tab[blockIdx.y];
+  const double *v = which == 0 ? M.b : which == 1 ? M.r : M.x;
+  const long long ld = which == 0 ? M.ldb : which == 1 ? M.ldr : M.ldx;
+  for (int c = 0; c < M.nrhs; c++) {
+    double m = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < M.n;
+         i += (long long)gridDim.x * blockDim.x)
+      m = fmax(m, fabs(v[i + c * ld]));
+    m = block_max(m, red);
+    if (threadIdx.x == 0) atomicMax(out + M.norm0 + c, (unsigned long long)__double_as_longlong(m));
+  }
+}
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+void launch_colmax(const IrDev *tab, int nmat, int max_n, int which, unsigned long long *out,
+                   int nout, hipStream_t st) {
+  if (nout > 0)
+    hipLaunchKernelGGL(k_zero_u64, dim3((nout + 255) / 256), dim3(256), 0, st, out, nout);
+  if (nmat <= 0 || max_n <= 0) return;
+  hipLaunchKernelGGL(k_colmax, dim3(ir_blocks(max_n), nmat), dim3(256), 0, st, tab, which, out);
+}
+// out[matrix] = max_i sum_j |A_ij|
+__global__ __launch_bounds__(256) void k_rowabs_max(const IrDev *__restrict__ tab,
+                                                    unsigned long long *out) {
+  __shared__ double red[4];
+  const IrDev M = tab[blockIdx.y];
+  double m = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < M.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int p = M.rowptr[i]; p < M.rowptr[i + 1]; p++) s += fabs(M.val[p]);
+    m = fmax(m, s);
+  }
+  m = block_max(m, red);
+  if (threadIdx.x == 0) atomicMax(out + blockIdx.y, (unsigned long long)__double_as_longlong(m));
+}
+void launch_rowabs_max(const IrDev *tab, int nmat, int max_n, unsigned long long *out,
+                       hipStream_t st) {
+  if (nmat <= 0) return;
+  hipLaunchKernelGGL(k_zero_u64, dim3((nmat + 255) / 256), dim3(256), 0, st, out, nmat);
+  if (max_n <= 0) return;
+  hipLaunchKernelGGL(k_rowabs_max, dim3(ir_blocks(max_n), nmat), dim3(256), 0, st, tab, out);
+}
+// x += dx for the matrices whose flag is set
+__global__ void k_update(const IrDev *__restrict__ tab, const int *__restrict__ active) {
+  const IrDev M = tab[blockIdx.y];
+  if (!active[blockIdx.y]) return;
+  const long long tot = (long long)M.n * M.nrhs;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < tot;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long c = i / M.n, r = i % M.n;
+    M.x[r + c * M.ldx] += M.dx[r + c * M.lddx];
+  }
+}
+void launch_update(const IrDev *tab, int nmat, long long max_elems, const int *active,
+                   hipStream_t st) {
+  if (nmat <= 0 || max_elems <= 0) return;
+  hipLaunchKernelGGL(k_update, dim3(ir_blocks(max_elems), nmat), dim3(256), 0, st, tab, active);
+}
+
+// Factor set-up for a set of matrices (one per blockIdx.y): zero the value
+// arena and the inverse blocks, then scatter A's values into their slots.
+template <typename T>
+__global__ void k_prep_fill(const PrepDev *__restrict__ tab) {
+  const PrepDev M = tab[blockIdx.y];
+  T *a = (T *)M.arena, *v = (T *)M.inv;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < M.arena_len; i += stride)
+    a[i] = T(0);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < M.inv_len; i += stride)
+    v[i] = T(0);
+}
+template <typename T>
+__global__ void k_prep_scatter(const PrepDev *__restrict__ tab) {
+  const PrepDev M = tab[blockIdx.y];
+  T *a = (T *)M.arena;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < M.nnz;
+       i += (long long)gridDim.x * blockDim.x)
+    a[M.adest[i]] = (T)M.aval[i];
+}
+template <typename T>
+void launch_prepare(const PrepDev *tab, int nmat, long long max_len, long long max_nnz,
+                    hipStream_t st) {
+  if (nmat <= 0) return;
+  auto blocks = [](long long n) {
+    return (unsigned)std::max(1LL, std::min((n + 255) / 256, 4096LL));
+  };
+  hipLaunchKernelGGL(k_prep_fill<T>, dim3(blocks(max_len), nmat), dim3(256), 0, st, tab);
+  hipLaunchKernelGGL(k_prep_scatter<T>, dim3(blocks(max_nnz), nmat), dim3(256), 0, st, tab);
 }
 
 int device_lds_limit() {
@@ -681,5 +1004,38 @@ int device_lds_limit() {
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 65536;
   return (int)prop.sharedMemPerBlock;
 }
+
+
+// ---- explicit instantiations: FP64 and FP32 value paths ----
+#define SNU_INSTANTIATE(T)                                                                    \
+  template void launch_fill<T>(T *, long long, hipStream_t);                                  \
+  template void launch_prepare<T>(const PrepDev *, int, long long, long long, hipStream_t);   \
+  template void launch_leaf_rows<T>(const LeafDev *, int, int, int, double, hipStream_t);     \
+  template void launch_leaf_sep<T>(const LeafDev *, const int *, const int *, int, int, int,  \
+                                   hipStream_t);                                              \
+  template void launch_assemble_schur<T>(int, const int *, const int *, const int *,          \
+                                         const int *, const int *, const LeafDev *,           \
+                                         hipStream_t);                                        \
+  template void launch_gemm<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int, int,     \
+                               hipStream_t);                                                  \
+  template void launch_gemm128<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,       \
+                                  hipStream_t);                                               \
+  template void launch_gemv<T>(const GemvProbT<T> *, int, const GemvSegT<T> *, int, int,     \
+                               hipStream_t);                                                  \
+  template void launch_diag_lu<T>(const DiagProbT<T> *, int, hipStream_t);                   \
+  template void launch_permute_in<T>(const PermDev *, int, const VecDev *, T *, long long,    \
+                                     hipStream_t);                                            \
+  template void launch_permute_out<T>(const PermDev *, int, const VecDev *, const T *,        \
+                                      long long, hipStream_t);                                \
+  template void launch_leaf_fwd<T>(const LeafDev *, int, const VecDev *, T *, hipStream_t);   \
+  template void launch_leaf_fwd_sep<T>(const LeafDev *, const int *, const int *, int, int,   \
+                                       const VecDev *, const T *, T *, int, hipStream_t);     \
+  template void launch_assemble_rhs<T>(int, const int *, const int *, const int *,            \
+                                       const int *, const LeafDev *, const VecDev *,          \
+                                       const T *, int, T *, int, hipStream_t);                \
+  template void launch_leaf_bwd<T>(const LeafDev *, int, const VecDev *, T *, hipStream_t);
+SNU_INSTANTIATE(double)
+SNU_INSTANTIATE(float)
+#undef SNU_INSTANTIATE
 
 }  // namespace snu

@@ -99,16 +99,17 @@ def test_gemm_kernel_vs_numpy(S):
     import ctypes as C
     L = S.lib()
     rng = np.random.default_rng(0)
-    for (M, N, K) in [(1, 1, 1), (64, 64, 16), (65, 63, 17), (130, 7, 200), (5, 200, 64), (257, 129, 3)]:
+    for (M, N, K) in [(1, 1, 1), (64, 64, 16), (65, 63, 17), (130, 7, 200), (5, 200, 64), (257, 129, 3),
+                      (128, 128, 16), (300, 260, 100), (129, 383, 64), (1000, 999, 33)]:
         A = np.asfortranarray(rng.standard_normal((M, K)))
         B = np.asfortranarray(rng.standard_normal((K, N)))
         C0 = np.asfortranarray(rng.standard_normal((M, N)))
-        for assign in (0, 1):
+        for assign in (0, 1, 2, 3):
             Cg = C0.copy(order="F")
             rc = L.snu_test_gemm(M, N, K, A.ctypes.data_as(C.c_void_p), M, B.ctypes.data_as(C.c_void_p), K,
                                  Cg.ctypes.data_as(C.c_void_p), M, assign)
             assert rc == 0
-            ref = A @ B if assign else C0 - A @ B
+            ref = A @ B if assign == 1 else (-(A @ B) if assign == 3 else C0 - A @ B)
             err = np.max(np.abs(Cg - ref)) / max(1.0, np.max(np.abs(ref)))
             assert err < 1e-13, (M, N, K, assign, err)
 
@@ -132,3 +133,131 @@ def test_diag_lu_kernel_vs_doolittle(S):
         Ui = Ui.reshape(64, 64, order="F")[:nb, :nb]
         assert np.max(np.abs(Li @ Lf - np.eye(nb))) < 1e-12
         assert np.max(np.abs(Ui @ Uf - np.eye(nb))) < 1e-12
+
+
+def test_gemm_kernel_f32_vs_numpy(S):
+    """FP32 MFMA GEMM (16x16x4 f32, FP32 accumulation) against an FP64
+    product of the same FP32 operands, elementwise within the standard
+    dot-product rounding bound (K + 2) u32 (|A||B| + |C|)."""
+    import ctypes as C
+    L = S.lib()
+    rng = np.random.default_rng(2)
+    u32 = 2.0 ** -24
+    for (M, N, K) in [(1, 1, 1), (64, 64, 16), (65, 63, 17), (130, 7, 200), (5, 200, 64),
+                      (128, 128, 16), (300, 260, 100), (129, 383, 64), (1000, 999, 33)]:
+        A = np.asfortranarray(rng.standard_normal((M, K)).astype(np.float32))
+        B = np.asfortranarray(rng.standard_normal((K, N)).astype(np.float32))
+        C0 = np.asfortranarray(rng.standard_normal((M, N)).astype(np.float32))
+        A64, B64, C64 = A.astype(np.float64), B.astype(np.float64), C0.astype(np.float64)
+        absprod = np.abs(A64) @ np.abs(B64)
+        for assign in (0, 1, 2, 3):
+            Cg = C0.copy(order="F")
+            rc = L.snu_test_gemm_f32(M, N, K, A.ctypes.data_as(C.c_void_p), M, B.ctypes.data_as(C.c_void_p), K,
+                                     Cg.ctypes.data_as(C.c_void_p), M, assign)
+            assert rc == 0
+            P = A64 @ B64
+            ref = P if assign == 1 else (-P if assign == 3 else C64 - P)
+            bound = (K + 2) * u32 * (absprod + (np.abs(C64) if assign in (0, 2) else 0)) + 1e-30
+            assert np.all(np.abs(Cg - ref) <= bound), (M, N, K, assign, np.max(np.abs(Cg - ref) / bound))
+
+
+def test_diag_lu_kernel_f32(S):
+    import ctypes as C
+    L = S.lib()
+    rng = np.random.default_rng(3)
+    for nb in (1, 2, 17, 64):
+        A = (rng.standard_normal((nb, nb)) + nb * np.eye(nb)).astype(np.float32)
+        D = np.asfortranarray(A.copy())
+        Li = np.zeros(4096, np.float32)
+        Ui = np.zeros(4096, np.float32)
+        rc = L.snu_test_diag_lu_f32(nb, D.ctypes.data_as(C.c_void_p), nb, Li.ctypes.data_as(C.c_void_p),
+                                    Ui.ctypes.data_as(C.c_void_p))
+        assert rc == 0
+        Lf = np.tril(D, -1).astype(np.float64) + np.eye(nb)
+        Uf = np.triu(D).astype(np.float64)
+        assert np.max(np.abs(Lf @ Uf - A)) <= 1e-5 * np.max(np.abs(A))
+        Li = Li.reshape(64, 64, order="F")[:nb, :nb].astype(np.float64)
+        Ui = Ui.reshape(64, 64, order="F")[:nb, :nb].astype(np.float64)
+        assert np.max(np.abs(Li @ Lf - np.eye(nb))) < 1e-4
+        assert np.max(np.abs(Ui @ Uf - np.eye(nb))) < 1e-4
+
+
+def test_fp32_refined_vs_oracle(S, oracle_mod, cases):
+    """FP32 factors + FP64 refinement: same bars as the FP64 path (x within
+    1e-9 of the FP64 oracle, backward error <= 1e-12), in <= 10 steps."""
+    for name, A, np_, (order, sizes) in cases:
+        print("case", name, flush=True)
+        _, b = S.gen_rhs(A, 1, 1)
+        b = b[:, 0]
+        xo = oracle_mod.Oracle(A, np_, order, sizes).factor_solve(b)
+        with S.SnuMat(A, np_, order=order, sizes=sizes, device=0, precision="fp32") as M:
+            x = M.factorize().solve(b)
+            t = M.timings()
+        rel = np.max(np.abs(x - xo)) / np.max(np.abs(xo))
+        berr = S.backward_error(A, x, b)
+        print(f"  steps {t['ir_steps']:.0f}  rel diff vs oracle {rel:.2e}  backward error {berr:.2e}")
+        assert rel <= REL_TOL, (name, rel)
+        assert berr <= BERR_TOL, (name, berr)
+        assert t["ir_steps"] <= 10 and t["ir_berr"] <= BERR_TOL
+
+
+def test_fp32_refined_multi_rhs(S):
+    A = S.gen_grid(16, 16, 12, 7, 0, 6.1, 0)
+    order, sizes = S.order_grid(16, 16, 12, 16)
+    _, B = S.gen_rhs(A, 9, 1)
+    with S.SnuMat(A, 16, order=order, sizes=sizes, device=0) as M64:
+        X64 = M64.factorize().solve(B)
+    with S.SnuMat(A, 16, order=order, sizes=sizes, device=0, precision="fp32") as M:
+        M.factorize()
+        X = M.solve(B)
+        for j in (0, 4, 8):
+            xj = M.solve(B[:, j])
+            assert np.max(np.abs(X[:, j] - xj)) <= 1e-9 * np.max(np.abs(xj))
+    assert S.backward_error(A, X, B) <= BERR_TOL
+    assert np.max(np.abs(X - X64)) <= REL_TOL * np.max(np.abs(X64))
+
+
+def _batch_members(S):
+    """Uneven matrices: 2D 9-point grids of several sides and leaf counts
+    (np = 1 included), two 3D 7-point grids."""
+    out = []
+    for i, (side, np_) in enumerate([(12, 1), (17, 2), (24, 4), (40, 4), (33, 8), (64, 16), (20, 1), (50, 8)]):
+        A = S.gen_grid(side, side, 1, 9, 0, 8.1, i)
+        out.append((A, np_, S.order_grid(side, side, 1, np_)))
+    for i, (d, np_) in enumerate([(10, 4), (14, 8)]):
+        A = S.gen_grid(d, d, d, 7, 0, 6.1, 20 + i)
+        out.append((A, np_, S.order_grid(d, d, d, np_)))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_equals_standalone(S, precision):
+    """A batch factorizes and solves every member with the arithmetic of its
+    standalone run: the solutions are bitwise equal, with uneven right-hand
+    side counts (0 skips a member) in one packed solve."""
+    mem = _batch_members(S)
+    rng = np.random.default_rng(7)
+    counts = [1, 3, 0, 2, 1, 5, 1, 0, 2, 1]
+    bs = [None if k == 0 else rng.standard_normal((A.n, k)) for (A, _, _), k in zip(mem, counts)]
+    ref = []
+    for (A, np_, (order, sizes)), b in zip(mem, bs):
+        with S.SnuMat(A, np_, order=order, sizes=sizes, device=0, precision=precision) as M:
+            ref.append(None if b is None else M.factorize().solve(b))
+    mats = [S.SnuMat(A, np_, order=order, sizes=sizes, device=0, precision=precision)
+            for (A, np_, (order, sizes)) in mem]
+    with S.Batch(mats) as B:
+        B.factorize()
+        xs = B.solve(bs)
+        for (A, _, _), b, x, r in zip(mem, bs, xs, ref):
+            if b is None:
+                assert x is None
+                continue
+            assert np.array_equal(x, r)
+            assert S.backward_error(A, x, b) <= BERR_TOL
+        # members are factored after the batch and solve on their own too
+        x5 = mats[5].solve(bs[5])
+        assert np.array_equal(x5, ref[5])
+        fl, sl = B.launches()
+        print(f"batch of {len(mats)}: {fl} dense factor launches, {sl} dense solve launches")
+    for M in mats:
+        M.close()
