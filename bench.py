@@ -22,7 +22,12 @@ Two further legs (skippable) report the other forms of the path:
      plain FP64 path (factor and solve times, refinement steps, errors);
   "batch": 512 2D 9-point matrices (sides log-uniform in [16, 192]) factorized
      and solved as one batch, FP64 and FP32 + refinement, against the same
-     matrices run one by one.
+     matrices run one by one;
+  "serving": 16 resident FP32 + refinement factorizations (8 3D 7-point grids,
+     sides in 32..80; 8 2D 5-point grids, sides in 256..1024), a Poisson
+     stream of solve requests with 2^k columns (k uniform in 0..6), 5% of
+     requests new factorizations admitted under a 200 GB budget; p50 / p99
+     latency and solves/s at the stated rate.
 """
 import argparse
 import json
@@ -47,6 +52,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-precision", action="store_true", help="skip the 96^3 FP32+IR leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 512-matrix batch leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also launch a plain 1 GiB copy kernel (snu::k_copy), the known-size "
+                         "reference for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="requests per second")
+    ap.add_argument("--serve-requests", type=int, default=400)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -59,6 +70,12 @@ def main():
 
     import snusolver_amd as S
 
+    if args.calibrate_copy:
+        import ctypes as C
+        ms = C.c_double()
+        S._check(S.lib().snu_bench_copy(C.c_size_t(1 << 30), 3, C.byref(ms)))
+        print(f"copy kernel: 1 GiB in {ms.value:.3f} ms ({2 * (1 << 30) / ms.value / 1e6:.0f} GB/s read+write)",
+              file=sys.stderr, flush=True)
     A, grid, np_ = S.config_matrix(args.config)
     n = A.n
     _, b1 = S.gen_rhs(A, 1, 1)
@@ -170,7 +187,9 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"], xo = cpu_baseline(A, grid, np_, b1[:, 0], args.cpu_threads)
         line["rel_diff_vs_oracle"] = float(np.max(np.abs(x - xo)) / np.max(np.abs(xo)))
-    for flag, key, leg in ((args.no_precision, "precision", precision_leg), (args.no_batch, "batch", batch_leg)):
+    legs = ((args.no_precision, "precision", precision_leg), (args.no_batch, "batch", batch_leg),
+            (args.no_serve, "serving", lambda S_, np_: serve_leg(S_, np_, args.serve_rate, args.serve_requests)))
+    for flag, key, leg in legs:
         if flag:
             continue
         try:
@@ -220,6 +239,83 @@ def precision_leg(S, np, reps=3):
         d_x.free()
     out["rel_diff_fp32ir_vs_fp64"] = float(np.max(np.abs(xs["fp32"] - xs["fp64"])) / np.max(np.abs(xs["fp64"])))
     out["speedup_fp32ir_vs_fp64"] = round(out["fp32"]["factorize_solve_per_s"] / out["fp64"]["factorize_solve_per_s"], 3)
+    return out
+
+
+def serve_matrix(S, rng, kind):
+    """One matrix of the serving mix: kind 0 a 3D 7-point grid (side 32..80),
+    kind 1 a 2D 5-point grid (side 256..1024), convection-diffusion recipe."""
+    if kind == 0:
+        d = int(rng.integers(32, 81))
+        A = S.gen_grid(d, d, d, 7, 0, 6.1, int(rng.integers(1 << 30)))
+        return A, (d, d, d), S.batch_np(A.n)
+    d = int(rng.integers(256, 1025))
+    A = S.gen_grid(d, d, 1, 5, 0, 4.1, int(rng.integers(1 << 30)))
+    return A, (d, d, 1), S.batch_np(A.n)
+
+
+def serve_leg(S, np, rate, nreq, budget=200e9):
+    import threading
+    rng = np.random.default_rng(4)
+    mix = [serve_matrix(S, rng, kind) for kind in (0, 1) for _ in range(8)]
+    out = {"workload": "16 resident FP32+IR factorizations (8 3D 7-pt sides 32..80, 8 2D 5-pt sides 256..1024), "
+                       f"Poisson {rate:g} req/s, nrhs 2^U(0..6), 5% new factorizations, 200 GB budget",
+           "requests": nreq, "rate_per_s": rate}
+    print("serving leg: admitting 16 matrices", file=sys.stderr, flush=True)
+    srv = S.Server(budget_bytes=budget, slots=2, max_cols=256)
+    t0 = time.perf_counter()
+    ids = [srv.add(A, np_, grid=g, precision="fp32") for A, g, np_ in mix]
+    out["admit_s"] = round(time.perf_counter() - t0, 2)
+    mats = {i: A for i, (A, _, _) in zip(ids, mix)}
+    arr = np.random.default_rng(5)
+    gaps = arr.exponential(1.0 / rate, nreq)
+    tickets, adds, lock = [], [], threading.Lock()
+
+    def admit(A, g, np_):
+        mid = srv.add(A, np_, grid=g, precision="fp32")
+        with lock:
+            mats[mid] = A
+
+    print("serving leg: request stream", file=sys.stderr, flush=True)
+    start = time.perf_counter()
+    due = start
+    for q in range(nreq):
+        due += gaps[q]
+        dt = due - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        if arr.random() < 0.05:
+            A, g, np_ = serve_matrix(S, arr, int(arr.integers(2)))
+            th = threading.Thread(target=admit, args=(A, g, np_))
+            th.start()
+            adds.append(th)
+            continue
+        with lock:
+            live = list(mats.keys())
+        mid = live[int(arr.integers(len(live)))]
+        k = 1 << int(arr.integers(0, 7))
+        b = arr.standard_normal((mats[mid].n, k))
+        try:
+            tickets.append((mid, b, srv.submit(mid, b)))
+        except S.SnuError:
+            pass  # evicted between the pick and the submit
+    worst = 0.0
+    cols = 0
+    for mid, b, t in tickets:
+        x, _ = srv.wait(t)
+        cols += b.shape[1]
+        if len(tickets) < 50 or t % 10 == 0:
+            worst = max(worst, S.backward_error(mats[mid], x, b))
+    for th in adds:
+        th.join()
+    wall = time.perf_counter() - start
+    st = srv.stats()
+    srv.close()
+    out.update({"answered": int(st["requests"]), "columns": cols, "packed_solves": int(st["packed_solves"]),
+                "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "mean_ms": round(st["mean_ms"], 3),
+                "solves_per_s": round(st["requests"] / wall, 2), "columns_per_s": round(cols / wall, 2),
+                "new_factorizations": len(adds), "evictions": int(st["evictions"]),
+                "worst_backward_error_sampled": worst, "wall_s": round(wall, 2)})
     return out
 
 

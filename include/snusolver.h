@@ -90,6 +90,16 @@ int snu_initialize(int device);
 int snumat_construct(const snu_csr_t *A, const snu_opts_t *opts,
                      snumat_t **out);
 
+/* Device bytes an upload of this handle takes (value arrays, structure and
+ * one-column solve buffers; an estimate from the symbolic structure, valid
+ * for host-only handles too).  No reference counterpart: used by callers that
+ * admit matrices under a memory budget. */
+int snumat_bytes_needed(const snumat_t *h, size_t *bytes);
+
+/* Move a host-only handle (constructed with device < 0) onto a device with
+ * A's values (same pattern as at construct).  No reference counterpart. */
+int snumat_attach(snumat_t *h, int device, const double *values);
+
 /* Numeric factorization on the GPU from the resident copy of A's values. */
 int snumat_factorize(snumat_t *h);
 
@@ -139,6 +149,37 @@ int snubatch_timings(const snubatch_t *b, double *t16);
 int snubatch_launches(const snubatch_t *b, int *factor_launches,
                       int *solve_launches);
 int snubatch_destroy(snubatch_t *b);
+
+/* ---------------- serving (no reference counterpart) ----------------
+ * A server keeps factorizations resident in HBM under budget_bytes and
+ * answers solve requests from any thread.  `slots` worker threads each pack
+ * every pending request (up to max_cols columns) into one batched solve --
+ * several requests for one matrix become columns of one multi-rhs solve,
+ * different matrices share one launch per sweep step and tree height -- and
+ * slots run concurrently on their own streams.  snuserve_add analyses a
+ * matrix on the host, evicts idle least-recently-used matrices until it
+ * fits the budget, then uploads and factorizes it (one precision per
+ * server).  Each answer equals the matrix's standalone solve of the same
+ * columns up to the refinement tolerance (bitwise for SNU_FP64). */
+typedef struct snuserve snuserve_t;
+int snuserve_create(int device, size_t budget_bytes, int slots, int max_cols,
+                    snuserve_t **out);
+int snuserve_add(snuserve_t *s, const snu_csr_t *A, const snu_opts_t *opts,
+                 int *id);
+int snuserve_remove(snuserve_t *s, int id);
+/* Queue x = A_id^-1 b (host pointers, column-major, ld >= n); b and x must
+ * stay valid until snuserve_wait returns for the ticket. */
+int snuserve_submit(snuserve_t *s, int id, const double *b, double *x,
+                    int nrhs, int ld, long long *ticket);
+/* Block until the request is answered; returns its status. */
+int snuserve_wait(snuserve_t *s, long long ticket, double *latency_ms);
+/* st[0] requests answered, st[1] columns, st[2] packed solves, st[3] p50
+ * latency ms, st[4] p99 ms, st[5] mean ms, st[6] max ms, st[7] evictions,
+ * st[8] admissions, st[9] resident matrices, st[10] bytes reserved, st[11]
+ * wall ms first submit -> last answer, st[12] summed slot busy ms; reset
+ * clears the latency record. */
+int snuserve_stats(snuserve_t *s, double *st16, int reset);
+int snuserve_destroy(snuserve_t *s);
 
 /* Reference-style one shot: construct + factorize + solve + destroy
  * (<- void solve(csr_matrix, double *b, double *x)). */
@@ -240,6 +281,13 @@ int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,
  * prec SNU_FP64 or SNU_FP32. */
 int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                    double *ms_out);
+/* tiles3: 64 or 128, the tile of the grouped C -= A B kernel per size bucket
+ * of min(M, N) ([128, 512), [512, 2048), >= 2048), chosen by the
+ * autotuner's timings on this device (run once per process). */
+int snu_gemm_tiles(int device, int precision, int *tiles3);
+/* average time (ms) of a plain device-to-device copy kernel of `bytes`
+ * (multiple of 16): the known-size reference for HBM byte counters. */
+int snu_bench_copy(size_t bytes, int reps, double *ms_out);
 /* in-place no-pivot LU of an nb x nb block (nb <= 64); Linv/Uinv: 64 x 64. */
 int snu_test_diag_lu(int nb, double *D, int ld, double *Linv, double *Uinv);
 int snu_test_diag_lu_f32(int nb, float *D, int ld, float *Linv, float *Uinv);

@@ -25,7 +25,7 @@ import numpy as np
 __all__ = [
     "SnuError", "CSR", "SnuMat", "lib", "gen_grid", "gen_rhs", "order_grid",
     "order_nd", "read_mtx", "backward_error", "solve", "config_matrix", "Batch",
-    "config_batch", "batch_sides", "batch_np",
+    "config_batch", "batch_sides", "batch_np", "Server",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -433,6 +433,86 @@ class Batch:
         f, s_ = C.c_int(), C.c_int()
         _check(lib().snubatch_launches(self._h, C.byref(f), C.byref(s_)))
         return f.value, s_.value
+
+
+class Server:
+    """Serving loop: resident factorizations under an HBM byte budget,
+    solve requests packed across matrices by `slots` worker threads."""
+
+    def __init__(self, budget_bytes: int, slots: int = 2, max_cols: int = 256, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().snuserve_create(device, C.c_size_t(int(budget_bytes)), slots, max_cols,
+                                     C.byref(self._h)))
+        self._keep = {}
+
+    def add(self, A: CSR, np_: int, grid=None, precision: str = "fp32", order=None, sizes=None,
+            ir_max_steps: int = 10, ir_tol: float = 1e-15) -> int:
+        o = _Opts()
+        lib().snu_default_opts(C.byref(o))
+        o.np = np_
+        o.precision = FP32 if precision == "fp32" else FP64
+        o.ir_max_steps = ir_max_steps
+        o.ir_tol = ir_tol
+        keep = []
+        if order is not None:
+            o.ordering = ORDER_GIVEN
+            order = np.ascontiguousarray(order, np.int32)
+            sizes = np.ascontiguousarray(sizes, np.int32)
+            keep += [order, sizes]
+            o.order, o.sizes = _ip(order), _ip(sizes)
+        elif grid is not None:
+            o.ordering = ORDER_GRID
+            g = list(grid) + [1] * (3 - len(grid))
+            o.grid[0], o.grid[1], o.grid[2] = g
+        a = A.c()
+        mid = C.c_int()
+        _check(lib().snuserve_add(self._h, C.byref(a), C.byref(o), C.byref(mid)))
+        return mid.value
+
+    def remove(self, mid: int):
+        _check(lib().snuserve_remove(self._h, mid))
+
+    def submit(self, mid: int, b: np.ndarray):
+        """Queue a solve; returns a ticket holding the (b, x) buffers."""
+        b = np.asarray(b, np.float64)
+        B = np.asfortranarray(b.reshape(b.shape[0], -1))
+        X = np.zeros_like(B, order="F")
+        t = C.c_longlong()
+        _check(lib().snuserve_submit(self._h, mid, _dp(B), _dp(X), B.shape[1], B.shape[0], C.byref(t)))
+        self._keep[t.value] = (B, X, b.ndim == 1)
+        return t.value
+
+    def wait(self, ticket: int):
+        """Block until answered; returns (x, latency_ms)."""
+        lat = C.c_double()
+        rc = lib().snuserve_wait(self._h, C.c_longlong(ticket), C.byref(lat))
+        B, X, one = self._keep.pop(ticket)
+        _check(rc)
+        return (X[:, 0].copy() if one else X), lat.value
+
+    def stats(self, reset: bool = False) -> dict:
+        st = (C.c_double * 16)()
+        _check(lib().snuserve_stats(self._h, st, 1 if reset else 0))
+        keys = ["requests", "columns", "packed_solves", "p50_ms", "p99_ms", "mean_ms", "max_ms",
+                "evictions", "admissions", "resident", "bytes_reserved", "wall_ms", "slot_busy_ms"]
+        return {k: float(st[i]) for i, k in enumerate(keys)}
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().snuserve_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def solve(A: CSR, b: np.ndarray, np_: int = 4, **kw) -> np.ndarray:

@@ -180,6 +180,38 @@ extern "C" int snumat_construct(const snu_csr_t *A, const snu_opts_t *opts,
   return SNU_OK;
 }
 
+extern "C" int snumat_bytes_needed(const snumat_t *h, size_t *bytes) {
+  if (!h || !bytes) return set_error(SNU_ERR_BAD_INPUT, "null");
+  *bytes = estimate_device_bytes(h);
+  return SNU_OK;
+}
+
+extern "C" int snumat_attach(snumat_t *h, int device, const double *values) {
+  if (!h || !values) return set_error(SNU_ERR_BAD_INPUT, "null handle/values");
+  if (h->on_device) return set_error(SNU_ERR_BAD_INPUT, "handle is already on a device");
+  int rc = snu_initialize(device);
+  if (rc) return rc;
+  h->device = device;
+  h->opts.device = device;
+  h->on_device = true;
+  rc = upload(h);
+  if (!rc) {
+    hipError_t e = hipMemcpy(h->d_aval, values, sizeof(double) * (size_t)h->P.nnz, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = set_error(SNU_ERR_HIP, hipGetErrorString(e));
+  }
+  if (!rc && h->prec == SNU_FP32) {
+    IrDev e = ir_entry(h, nullptr, 0, nullptr, 0, 0);
+    hipError_t he = hipMemcpy(h->d_ir, &e, sizeof(e), hipMemcpyHostToDevice);
+    if (he != hipSuccess) rc = set_error(SNU_ERR_HIP, hipGetErrorString(he));
+  }
+  if (rc) {
+    free_device(h);
+    h->on_device = false;
+    h->device = -1;
+  }
+  return rc;
+}
+
 extern "C" int snumat_update_values(snumat_t *h, const double *values) {
   if (!h || !values) return set_error(SNU_ERR_BAD_INPUT, "null handle/values");
   if (!h->on_device) return set_error(SNU_ERR_NO_DEVICE, "handle was built host-only");
@@ -259,6 +291,7 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
   pd[1] = PermDev{h->d_order, h->d_r, h->d_dx, (long long)n, (long long)n, n, 0};
   HIPCHK(hipMemcpyAsync(h->d_vec, &vd, sizeof(vd), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(h->d_perm, pd, sizeof(pd), hipMemcpyHostToDevice, st));
+  launch_zero_i32(h->wd.stall, 1, st);
   SolveSet ss;
   ss.perm = h->d_perm;
   ss.vec = h->d_vec;
@@ -303,6 +336,9 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
   HIPCHK(hipEventRecord(h->ev[7], st));
   HIPCHK(hipEventSynchronize(h->ev[7]));
   h->t[6] = ev_ms(h->ev[6], h->ev[7]);
+  int stalled = 0;
+  HIPCHK(hipMemcpy(&stalled, h->wd.stall, sizeof(int), hipMemcpyDeviceToHost));
+  if (stalled) return set_error(SNU_ERR_HIP, "leaf solve: a row's wait for its rows timed out");
   h->t[13] = h->ir_steps;
   h->t[14] = h->ir_berr;
   return rc;
@@ -672,4 +708,45 @@ extern "C" int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec,
     return set_error(SNU_ERR_BAD_INPUT, "bad gemm bench shape");
   return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, tile, reps, ms_out)
                           : bench_gemm<double>(M, N, K, nseg, tile, reps, ms_out);
+}
+
+// Average time (ms) of a plain device copy of `bytes` (multiple of 16) over
+// reps launches: the known-size kernel against which HBM byte counters are
+// scaled.
+extern "C" int snu_bench_copy(size_t bytes, int reps, double *ms_out) {
+  if (bytes < 16 || bytes % 16 || reps < 1 || !ms_out) return set_error(SNU_ERR_BAD_INPUT, "bad copy bench");
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, bytes));
+  HIPCHK(hipMalloc(&b, bytes));
+  launch_copy(a, b, (long long)bytes, nullptr);
+  HIPCHK(hipDeviceSynchronize());
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < reps; r++) launch_copy(a, b, (long long)bytes, nullptr);
+  HIPCHK(hipEventRecord(e1, nullptr));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *ms_out = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return SNU_OK;
+}
+
+// Tile table of the grouped C -= AB GEMM per size bucket of min(M, N)
+// ([128, 512), [512, 2048), >= 2048; below 128 always 64), as measured by the
+// autotuner (run now if it has not run in this process).
+extern "C" int snu_gemm_tiles(int device, int precision, int *tiles3) {
+  if (!tiles3 || (precision != SNU_FP64 && precision != SNU_FP32))
+    return set_error(SNU_ERR_BAD_INPUT, "bad arguments");
+  int rc = snu_initialize(device);
+  if (rc) return rc;
+  if (precision == SNU_FP32) autotune_tiles<float>();
+  else autotune_tiles<double>();
+  tile_table(precision, tiles3);
+  return SNU_OK;
 }

@@ -52,6 +52,9 @@ struct snubatch {
   std::vector<double> anorm;
   int ir_steps = 0;
   std::vector<double> berr;
+  double *stage = nullptr;  // host-pointer solves: packed b then x
+  size_t stage_cap = 0;
+  bool factor_plan = false;  // merged dense plan built (on first factorize)
   std::vector<hipEvent_t> ev;
   double t[16] = {0};
 };
@@ -223,6 +226,7 @@ int batch_solve(snubatch *b, const double *const *d_b, double *const *d_x, const
     max_n = std::max(max_n, n);
   }
   HIPCHK(hipMemcpyAsync(b->d_perm, pd.data(), sizeof(PermDev) * pd.size(), hipMemcpyHostToDevice, st));
+  launch_zero_i32(b->wd.stall, 1, st);
   SolveSet ss;
   ss.perm = b->d_perm;
   ss.vec = b->d_vec;
@@ -267,6 +271,8 @@ int batch_solve(snubatch *b, const double *const *d_b, double *const *d_x, const
     rs.norms = b->norms;
     rs.nslots = (int)slot_mat.size();
     rs.slot_mat = &slot_mat;
+    b->anorm.assign(nm, 0.0);  // |A| of each member, from its last factorization
+    for (int m = 0; m < nm; m++) b->anorm[m] = b->mats[m]->anorm;
     rs.anorm = &b->anorm;
     rs.d_active = b->d_active;
     rs.max_steps = o.ir_max_steps > 0 ? o.ir_max_steps : 10;
@@ -282,6 +288,9 @@ int batch_solve(snubatch *b, const double *const *d_b, double *const *d_x, const
   HIPCHK(hipEventRecord(b->ev[7], st));
   HIPCHK(hipEventSynchronize(b->ev[7]));
   b->t[6] = ev_ms(b->ev[6], b->ev[7]);
+  int stalled = 0;
+  HIPCHK(hipMemcpy(&stalled, b->wd.stall, sizeof(int), hipMemcpyDeviceToHost));
+  if (stalled) return set_error(SNU_ERR_HIP, "leaf solve: a row's wait for its rows timed out");
   b->t[13] = b->ir_steps;
   double worst = 0;
   for (double e : b->berr) worst = std::max(worst, e);
@@ -320,10 +329,10 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   std::vector<LeafDev> leaves;
   WorkLists W;
   W.tptr.assign(1, 0);
-  int wsz = 0;
-  for (snumat *h : b->mats) wsz = std::max(wsz, h->wd.wsz);
+  int wsz = 0, max_leaf = 0;
+  for (snumat *h : b->mats) wsz = std::max(wsz, h->wd.wsz), max_leaf = std::max(max_leaf, h->wd.max_leaf);
   const int vsize = b->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
-  const int waves = waves_for(wsz, vsize);
+  const int waves = waves_for(wsz, vsize, max_leaf);
   if (waves < 1) return fail(set_error(SNU_ERR_OOM, "leaf workspace exceeds the LDS of one workgroup"));
   int cb = 0;
   std::vector<PrepDev> prep;
@@ -377,6 +386,7 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   wd.cb_total = cb;
   wd.wsz = wsz;
   wd.waves = waves;
+  wd.max_leaf = max_leaf;
   int rc;
   if ((rc = bupload(b, &wd.leaves, leaves)) || (rc = bupload(b, &wd.wg_leaf, W.wg_leaf)) ||
       (rc = bupload(b, &wd.wg_first, W.wg_first)) || (rc = bupload(b, &wd.fwg_leaf, W.fwg_leaf)) ||
@@ -386,14 +396,13 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
       (rc = bupload(b, &wd.tsrc_ri, W.tsrc_ri)) || (rc = bupload(b, &b->d_prep, prep)) ||
       (rc = bupload(b, &b->d_ir_a, ira)))
     return fail(rc);
-  if ((rc = balloc(b, (void **)&b->d_anorm, sizeof(unsigned long long) * count)) ||
+  if ((rc = balloc(b, (void **)&wd.stall, sizeof(int))) ||
+      (rc = balloc(b, (void **)&b->d_anorm, sizeof(unsigned long long) * count)) ||
       (rc = balloc(b, (void **)&b->d_active, sizeof(int) * count)) ||
       (rc = balloc(b, (void **)&b->d_vec, sizeof(VecDev) * count)) ||
       (rc = balloc(b, (void **)&b->d_perm, sizeof(PermDev) * 2 * count)) ||
       (rc = balloc(b, (void **)&b->d_ir, sizeof(IrDev) * count)))
     return fail(rc);
-  rc = b->prec == SNU_FP32 ? build_factor_set<float>(b) : build_factor_set<double>(b);
-  if (rc) return fail(rc);
   *out = b;
   return SNU_OK;
 }
@@ -401,6 +410,11 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
 extern "C" int snubatch_factorize(snubatch_t *b) {
   if (!b) return set_error(SNU_ERR_BAD_INPUT, "null batch");
   HIPCHK(hipSetDevice(b->device));
+  if (!b->factor_plan) {
+    int rc = b->prec == SNU_FP32 ? build_factor_set<float>(b) : build_factor_set<double>(b);
+    if (rc) return rc;
+    b->factor_plan = true;
+  }
   for (snumat *h : b->mats) h->factored = false;
   return b->prec == SNU_FP32 ? batch_factorize<float>(b) : batch_factorize<double>(b);
 }
@@ -431,8 +445,15 @@ extern "C" int snubatch_solve(snubatch_t *b, const double *const *hb, double *co
       return set_error(SNU_ERR_BAD_INPUT, "bad batch solve arguments for matrix " + std::to_string(m));
     tot += (size_t)b->mats[m]->P.n * nrhs[m];
   }
-  double *stage = nullptr;
-  HIPCHK(hipMalloc((void **)&stage, sizeof(double) * 2 * std::max<size_t>(1, tot)));
+  if (2 * tot > b->stage_cap) {
+    bfree(b, b->stage);
+    b->stage = nullptr;
+    b->stage_cap = 0;
+    int arc = balloc(b, (void **)&b->stage, sizeof(double) * 2 * std::max<size_t>(1, tot));
+    if (arc) return arc;
+    b->stage_cap = 2 * std::max<size_t>(1, tot);
+  }
+  double *stage = b->stage;
   std::vector<const double *> db(nm, nullptr);
   std::vector<double *> dx(nm, nullptr);
   std::vector<int> dld(nm);
@@ -444,8 +465,8 @@ extern "C" int snubatch_solve(snubatch_t *b, const double *const *hb, double *co
     if (!nrhs[m]) continue;
     double *pb = stage + off, *px = stage + tot + off;
     off += (size_t)n * nrhs[m];
-    if (hipMemcpy2D(pb, sizeof(double) * n, hb[m], sizeof(double) * ld[m], sizeof(double) * n, nrhs[m],
-                    hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy2DAsync(pb, sizeof(double) * n, hb[m], sizeof(double) * ld[m], sizeof(double) * n,
+                         nrhs[m], hipMemcpyHostToDevice, b->stream) != hipSuccess)
       rc = set_error(SNU_ERR_HIP, "batch rhs upload failed");
     db[m] = pb;
     dx[m] = px;
@@ -455,14 +476,14 @@ extern "C" int snubatch_solve(snubatch_t *b, const double *const *hb, double *co
     for (int m = 0; m < nm; m++) {
       if (!nrhs[m]) continue;
       const int n = b->mats[m]->P.n;
-      if (hipMemcpy2D(hx[m], sizeof(double) * ld[m], dx[m], sizeof(double) * n, sizeof(double) * n,
-                      nrhs[m], hipMemcpyDeviceToHost) != hipSuccess) {
+      if (hipMemcpy2DAsync(hx[m], sizeof(double) * ld[m], dx[m], sizeof(double) * n, sizeof(double) * n,
+                           nrhs[m], hipMemcpyDeviceToHost, b->stream) != hipSuccess) {
         rc = set_error(SNU_ERR_HIP, "batch solution download failed");
         break;
       }
     }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) rc = set_error(SNU_ERR_HIP, "batch solve sync failed");
   }
-  (void)hipFree(stage);
   return rc;
 }
 

@@ -85,6 +85,8 @@ struct WorkDev {
   int *tsrc_leaf = nullptr, *tsrc_ri = nullptr, ntarget = 0;
   int cb_total = 0;
   int wsz = 0, waves = 1;
+  int max_leaf = 0;  // rows of the largest leaf (completion flags in LDS)
+  int *stall = nullptr;  // leaf-row wait gave up (device flag)
 };
 
 }  // namespace snu
@@ -149,10 +151,17 @@ const Values<T> &vals(const snumat *h) {
 }
 
 int upload(snumat *h);
+// GEMM tile (64 or 128) for a C block of M x N, by precision; autotune_tiles
+// measures the table once per process (on the current device)
+template <typename T>
+void autotune_tiles();
+int tile_for(int prec, int M, int N);
+int tile_table(int prec, int *out3);
 void free_device(snumat *h);
+size_t estimate_device_bytes(const snumat *h);
 int device_alloc(snumat *h, void **p, size_t bytes);
 float ev_ms(hipEvent_t a, hipEvent_t b);
-int waves_for(int wsz, int vsize);
+int waves_for(int wsz, int vsize, int max_leaf);
 
 template <typename T>
 int upload_list(T **p, const std::vector<T> &v);

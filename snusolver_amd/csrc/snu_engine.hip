@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 
 #include "snu_engine.h"
@@ -72,10 +73,10 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-// waves per leaf workgroup: the LDS of one workgroup holds one workspace
-// row (wsz values) per wave
-int waves_for(int wsz, int vsize) {
-  int fit = device_lds_limit() / std::max(1, wsz * vsize);
+// waves per leaf workgroup: the LDS of one workgroup holds the leaf's row
+// completion flags and one workspace row (wsz values) per wave
+int waves_for(int wsz, int vsize, int max_leaf) {
+  int fit = (device_lds_limit() - leaf_flag_bytes(max_leaf)) / std::max(1, wsz * vsize);
   return std::min(8, fit);
 }
 
@@ -114,10 +115,93 @@ template int add_gemm<float>(std::vector<GemmProbT<float>> &, std::vector<GemmSe
                              int &, float *, long long, int, int,
                              const std::vector<GemmSegT<float>> &, double &, int);
 
+// ---------------------------------------------------------------------------
+// tile-shape autotuner: per precision and size bucket of min(M, N), the
+// faster of the 64 x 64 and 128 x 128 C -= AB kernels, measured once per
+// process on device-resident operands (both tiles compute every element with
+// the same instruction sequence, so the choice never changes results)
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+bool g_tuned[2] = {false, false};
+int g_tile[2][3] = {{128, 128, 128}, {128, 128, 128}};
+const int kBucketEdge[3] = {128, 512, 2048};
+const int kBucketProbe[3] = {256, 1024, 4096};
+
+template <typename T>
+double time_gemm(int M, int N, int K, int tile, int reps) {
+  T *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  GemmProbT<T> *dgp = nullptr;
+  GemmSegT<T> *dgs = nullptr;
+  double ms_out = -1;
+  if (hipMalloc((void **)&dA, sizeof(T) * (size_t)M * K) == hipSuccess &&
+      hipMalloc((void **)&dB, sizeof(T) * (size_t)K * N) == hipSuccess &&
+      hipMalloc((void **)&dC, sizeof(T) * (size_t)M * N) == hipSuccess) {
+    launch_fill(dA, (long long)M * K, nullptr);
+    launch_fill(dB, (long long)K * N, nullptr);
+    launch_fill(dC, (long long)M * N, nullptr);
+    std::vector<GemmProbT<T>> gp;
+    std::vector<GemmSegT<T>> gs;
+    int tiles = 0;
+    double fl = 0;
+    add_gemm<T>(gp, gs, tiles, dC, M, M, N, {GemmSegT<T>{dA, dB, M, K, K}}, fl, tile);
+    if (upload_list(&dgp, gp) == SNU_OK && upload_list(&dgs, gs) == SNU_OK) {
+      auto run = [&]() {
+        if (tile == 128) launch_gemm128<T>(dgp, 1, dgs, tiles, nullptr);
+        else launch_gemm<T>(dgp, 1, dgs, tiles, 0, nullptr);
+      };
+      hipEvent_t e0, e1;
+      if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        run();
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; r++) run();
+        (void)hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) == hipSuccess) ms_out = ev_ms(e0, e1) / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+      }
+    }
+  }
+  (void)hipDeviceSynchronize();
+  if (dA) (void)hipFree(dA);
+  if (dB) (void)hipFree(dB);
+  if (dC) (void)hipFree(dC);
+  if (dgp) (void)hipFree(dgp);
+  if (dgs) (void)hipFree(dgs);
+  return ms_out;
+}
+}  // namespace
+
+template <typename T>
+void autotune_tiles() {
+  const int pi = std::is_same<T, float>::value ? 1 : 0;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (g_tuned[pi]) return;
+  for (int b = 0; b < 3; b++) {
+    const int n = kBucketProbe[b];
+    const double t64 = time_gemm<T>(n, n, 256, 64, 3), t128 = time_gemm<T>(n, n, 256, 128, 3);
+    if (t64 > 0 && t128 > 0) g_tile[pi][b] = t128 < t64 ? 128 : 64;
+  }
+  g_tuned[pi] = true;
+}
+template void autotune_tiles<double>();
+template void autotune_tiles<float>();
+
+int tile_for(int prec, int M, int N) {
+  const int m = std::min(M, N);
+  if (m < kBucketEdge[0]) return 64;
+  const int b = m < kBucketEdge[1] ? 0 : m < kBucketEdge[2] ? 1 : 2;
+  return g_tile[prec == SNU_FP32 ? 1 : 0][b];
+}
+int tile_table(int prec, int *out3) {
+  for (int b = 0; b < 3; b++) out3[b] = g_tile[prec == SNU_FP32 ? 1 : 0][b];
+  return g_tuned[prec == SNU_FP32 ? 1 : 0] ? 1 : 0;
+}
+
 namespace {
 
-// A grouped C -= sum A B launch, split by tile size: problems with M, N >=
-// 128 go to the 128 x 128 kernel (launch kind 5), the rest to the 64 x 64 one.
+// A grouped C -= sum A B launch, split by tile size (chosen per size bucket
+// by the autotuner): 128 x 128 tiles are launch kind 5, 64 x 64 kind 2.
 template <typename T>
 struct GemmBatch {
   std::vector<GemmProbT<T>> p[2];
@@ -126,7 +210,7 @@ struct GemmBatch {
   int add(std::vector<GemmSegT<T>> &gs, T *C, long long ldc, int M, int N,
           const std::vector<GemmSegT<T>> &segs) {
     if (M <= 0 || N <= 0) return 0;
-    int t = (M >= 128 && N >= 128) ? 1 : 0;
+    int t = tile_for(std::is_same<T, float>::value ? SNU_FP32 : SNU_FP64, M, N) == 128 ? 1 : 0;
     return add_gemm(p[t], gs, tiles[t], C, ldc, M, N, segs, flops[t], t ? 128 : 64);
   }
   void flush(std::vector<GemmProbT<T>> &gp, std::vector<Launch> &out, long long key) {
@@ -623,6 +707,8 @@ int upload(snumat *h) {
   Plan &P = h->P;
   int rc;
   HIPCHK(hipSetDevice(h->device));
+  if (h->prec == SNU_FP32) autotune_tiles<float>();
+  else autotune_tiles<double>();
   const int np = P.np;
   // ---- leaf arrays (concatenated; descriptors hold pointers into them) ----
   std::vector<int> rowptr, diag, levptr, levrows, ulevptr, ulevrows;
@@ -678,8 +764,10 @@ int upload(snumat *h) {
   h->max_leaf_w = maxw;
   h->wd.cb_total = cb;
   h->wd.wsz = (maxw + 1) & ~1;
+  h->wd.max_leaf = 0;
+  for (int r = 0; r < np; r++) h->wd.max_leaf = std::max(h->wd.max_leaf, P.leaves[r].leaf_size);
   const int vsize = h->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
-  h->wd.waves = waves_for(h->wd.wsz, vsize);
+  h->wd.waves = waves_for(h->wd.wsz, vsize, h->wd.max_leaf);
   if (h->wd.waves < 1)
     return set_error(SNU_ERR_OOM, "leaf workspace (" + std::to_string(maxw) +
                                       " columns) exceeds the LDS of one workgroup");
@@ -787,7 +875,7 @@ int upload(snumat *h) {
     D.bcol_loc = d_bcol_loc + o.bcol;
     D.dd = h->d_dd;
   }
-  if ((rc = dalloc(h, &h->d_vec, 1)) || (rc = dalloc(h, &h->d_perm, 2)) ||
+  if ((rc = dalloc(h, &wd.stall, 1)) || (rc = dalloc(h, &h->d_vec, 1)) || (rc = dalloc(h, &h->d_perm, 2)) ||
       (rc = dalloc(h, &h->d_prep, 1)) || (rc = dalloc(h, &h->d_ir, 1)) ||
       (rc = dalloc(h, &h->d_active, 1)) || (rc = dalloc(h, &h->d_norms, 1)))
     return rc;
@@ -816,6 +904,26 @@ int upload(snumat *h) {
   pd.nnz = P.nnz;
   HIPCHK(hipMemcpy(h->d_prep, &pd, sizeof(pd), hipMemcpyHostToDevice));
   return SNU_OK;
+}
+
+// Device bytes that upload() and a one-column solve take for this plan (the
+// launch descriptor lists excluded).
+size_t estimate_device_bytes(const snumat *h) {
+  const Plan &P = h->P;
+  const double vs = h->prec == SNU_FP32 ? 4.0 : 8.0;
+  double S_total = 0, idx = 0, inv = 0;
+  for (const LeafPlan &L : P.leaves) {
+    S_total += (double)L.nbr * L.nbc;
+    idx += 2.0 * (L.core_n + 1) + L.lu_nnz + L.lev_ptr.size() + L.lev_rows.size() + L.ulev_ptr.size() +
+           L.ulev_rows.size() + L.brow.size() + 3.0 * L.bcol.size() + 2.0 * L.nbr / 4 + 4;
+  }
+  for (int p = 1; p < P.np; p++) inv += 2.0 * P.block_size[p] * P.block_size[p];
+  double bytes = vs * ((double)P.leaf_total + (double)P.dense_total + S_total + inv +
+                       (double)inversion_tmp_size(P));
+  bytes += 16.0 * P.nnz + 4.0 * (idx + P.n) + 48.0 * P.num_block;
+  if (h->prec == SNU_FP32) bytes += 4.0 * (P.n + 1 + P.nnz);
+  bytes += 2 * vs * P.n + 16.0 * P.n + (h->prec == SNU_FP32 ? 16.0 * P.n : 0.0);
+  return (size_t)bytes;
 }
 
 void free_device(snumat *h) {
@@ -865,7 +973,8 @@ int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
   launch_prepare<T>(fs.prep, fs.nmat, fs.max_len, fs.max_nnz, st);
   if (fs.ir) launch_rowabs_max(fs.ir, fs.nmat, fs.max_n, fs.anorm_out, st);
   if (ev) HIPCHK(hipEventRecord((*ev)[1], st));
-  launch_leaf_rows<T>(wd.leaves, wd.nleaves, wd.wsz, wd.waves, fs.eps, st);
+  launch_zero_i32(wd.stall, 1, st);
+  launch_leaf_rows<T>(wd.leaves, wd.nleaves, wd.wsz, wd.waves, wd.max_leaf, fs.eps, wd.stall, st);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[2], st));
   launch_leaf_sep<T>(wd.leaves, wd.wg_leaf, wd.wg_first, wd.nwg, wd.wsz, wd.waves, st);
@@ -894,6 +1003,10 @@ int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
   }
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[5], st));
+  int stalled = 0;
+  HIPCHK(hipMemcpyAsync(&stalled, wd.stall, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (stalled) return set_error(SNU_ERR_HIP, "leaf factorization: a row's wait for its pivot rows timed out");
   if (profile && pev && gemm_ms) {
     HIPCHK(hipStreamSynchronize(st));
     gemm_ms[0] = gemm_ms[1] = 0;
@@ -911,14 +1024,14 @@ template <typename T>
 void run_sweep(const SolveSet &ss, T *bp, T *cbuf, const GemvPlan<T> &plan, hipStream_t st) {
   const WorkDev &wd = *ss.wd;
   launch_permute_in(ss.perm, ss.nmat, ss.vec, bp, ss.max_elems, st);
-  launch_leaf_fwd(wd.leaves, wd.nleaves, ss.vec, bp, st);
+  launch_leaf_fwd(wd.leaves, wd.nleaves, ss.vec, bp, wd.max_leaf, wd.stall, st);
   launch_leaf_fwd_sep(wd.leaves, wd.fwg_leaf, wd.fwg_first, wd.nfwg, 4, ss.vec, bp, cbuf, ss.cb_ld,
                       st);
   launch_assemble_rhs(wd.ntarget, wd.trow_glob, wd.tptr, wd.tsrc_leaf, wd.tsrc_ri, wd.leaves,
                       ss.vec, cbuf, ss.cb_ld, bp, ss.max_rhs, st);
   for (const Launch &L : plan.fwd) run_launch<T>(L, nullptr, nullptr, nullptr, plan.d_vp, plan.d_vs, st);
   for (const Launch &L : plan.bwd) run_launch<T>(L, nullptr, nullptr, nullptr, plan.d_vp, plan.d_vs, st);
-  launch_leaf_bwd(wd.leaves, wd.nleaves, ss.vec, bp, st);
+  launch_leaf_bwd(wd.leaves, wd.nleaves, ss.vec, bp, wd.max_leaf, wd.stall, st);
   launch_permute_out(ss.perm, ss.nmat, ss.vec, bp, ss.max_elems, st);
 }
 

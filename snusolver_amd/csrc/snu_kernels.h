@@ -122,9 +122,17 @@ void launch_fill(T *p, long long n, hipStream_t st);
 template <typename T>
 void launch_prepare(const PrepDev *tab, int nmat, long long max_len, long long max_nnz,
                     hipStream_t st);
+// max_leaf: the largest leaf of the launch (its rows' completion flags
+// share the LDS with the waves' workspaces)
+// stall: set to 1 if a row gave up waiting for a pivot row (never expected;
+// the factors are then invalid and the caller reports an error)
 template <typename T>
-void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, double eps,
-                      hipStream_t st);
+void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, int max_leaf,
+                      double eps, int *stall, hipStream_t st);
+int leaf_flag_bytes(int max_leaf);
+void launch_zero_i32(int *p, int n, hipStream_t st);
+// dst = src, bytes a multiple of 16
+void launch_copy(const void *src, void *dst, long long bytes, hipStream_t st);
 template <typename T>
 void launch_leaf_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
                      int wsz, int waves, hipStream_t st);
@@ -153,8 +161,8 @@ template <typename T>
 void launch_permute_out(const PermDev *pd, int nmat, const VecDev *vd, const T *bp,
                         long long max_elems, hipStream_t st);
 template <typename T>
-void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp,
-                     hipStream_t st);
+void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
+                     int *stall, hipStream_t st);
 template <typename T>
 void launch_leaf_fwd_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
                          int waves, const VecDev *vd, const T *bp, T *cbuf, int cb_ld,
@@ -165,8 +173,8 @@ void launch_assemble_rhs(int ntarget, const int *trow_glob, const int *tptr,
                          const VecDev *vd, const T *cbuf, int cb_ld, T *bp, int max_rhs,
                          hipStream_t st);
 template <typename T>
-void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp,
-                     hipStream_t st);
+void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
+                     int *stall, hipStream_t st);
 
 // ---- FP64 iterative refinement (one matrix per table entry) ----
 void launch_residual(const IrDev *tab, int nmat, int max_n, hipStream_t st);

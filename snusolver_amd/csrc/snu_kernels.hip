@@ -80,12 +80,18 @@ void launch_fill(T *p, long long n, hipStream_t st) {
 //   otherwise a separator row: every entry is an L entry, and the updates
 //             that land on boundary columns form the row's Schur
 //             contribution -sum_k L(r,k) U(k,c) (kernel_sparse.cu:953-971).
+// done (leaf rows only): per-row completion flags in LDS.  A pivot row is
+// used only after its flag is seen (then an acquire fence orders the loads
+// of its values after the owner's release), so rows of one leaf run
+// concurrently on the waves of the workgroup, each waiting only for the
+// pivots it has reached; the row's own flag is raised after its values are
+// stored.
 template <typename T, bool LEAF_ROW>
 __device__ void factor_row(int i, const int *__restrict__ rowptr,
                            const int *__restrict__ diag,
                            const int *__restrict__ wcol,
                            T *__restrict__ val, T *w, int lane,
-                           T eps) {
+                           T eps, volatile unsigned char *done, int *stall) {
 #pragma clang fp contract(off)
   // Pivot rows are streamed one pivot ahead: while pivot t is applied, the
   // first NCH chunks (64 entries each) of pivot t+1's U row are already in
@@ -98,12 +104,35 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
   wave_lds_fence();
   int cc[NCH], cn[NCH];
   T uc[NCH], un[NCH];
-  for (int pb = p0; pb < pd; pb += 64) {
-    const int cnt = min(64, pd - pb);
+  for (int pb = p0; pb < pd;) {
+    int cnt = min(64, pd - pb);
     int k = 0, ks = 1, ke = 1;
     T piv = T(1);
+    if (lane < cnt) k = wcol[pb + lane];
+    if (LEAF_ROW) {
+      // the leading pivots of this batch whose rows are complete (at least
+      // one: the first pivot is waited for).  The wait is bounded: past
+      // kSpinLimit polls the row proceeds and the stall is reported, so the
+      // kernel always ends.
+      constexpr int kSpinLimit = 1 << 26;
+      for (int spin = 0;; spin++) {
+        const bool ready = lane >= cnt || done[k] != 0;
+        const unsigned long long notready = __ballot(!ready);
+        const int f = notready ? __builtin_ctzll(notready) : cnt;
+        if (f > 0) {
+          cnt = f;
+          break;
+        }
+        if (spin == kSpinLimit) {
+          if (lane == 0) *stall = 1;
+          cnt = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
     if (lane < cnt) {
-      k = wcol[pb + lane];
       int kd = diag[k];
       ks = kd + 1;
       ke = rowptr[k + 1];
@@ -148,6 +177,7 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
       if (lane == 0) w[kt] = l;
       wave_lds_fence();
     }
+    pb += cnt;
   }
   if (LEAF_ROW) {
     if (lane == 0) {
@@ -163,39 +193,49 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
     w[c] = T(0);
   }
   wave_lds_fence();
+  if (LEAF_ROW) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) done[i] = 1;
+  }
 }
 
-// One workgroup per leaf; rows of a level are spread over the waves, levels
-// are separated by workgroup barriers.
+// One workgroup per leaf.  Wave w factors leaf rows w, w + W, w + 2W, ...
+// in ascending order; a row waits only for the pivot rows it reaches (LDS
+// completion flags), so the rows of a dense chain overlap instead of running
+// one level at a time.  Every wait ends: the lowest unfinished row depends
+// only on finished rows and its wave is working on it.
+// LDS: completion flags (fl bytes), then one workspace row (wsz values) per
+// wave.
 extern __shared__ __align__(16) char lds_raw[];
 template <typename T>
 __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ leaves,
-                                                   int wsz, T eps) {
-  T *lds = (T *)lds_raw;
+                                                   int wsz, int fl, T eps, int *stall) {
   const LeafDev L = leaves[blockIdx.x];
+  volatile unsigned char *done = (volatile unsigned char *)lds_raw;
+  T *lds = (T *)(lds_raw + fl);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
+  for (int r = threadIdx.x; r < L.leaf_size; r += blockDim.x) done[r] = 0;
   T *w = lds + (size_t)wave * wsz;
   for (int c = lane; c < L.leaf_size + L.nbc; c += 64) w[c] = T(0);
-  wave_lds_fence();
+  __syncthreads();
   const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol;
   T *v = (T *)L.vals;
-  const int *lp = L.levptr, *lr = L.levrows;
-  for (int lev = 0; lev < L.nlev; lev++) {
-    for (int t = lp[lev] + wave; t < lp[lev + 1]; t += W)
-      factor_row<T, true>(lr[t], rp, dg, wc, v, w, lane, eps);
-    __syncthreads();
-  }
+  for (int r = wave; r < L.leaf_size; r += W)
+    factor_row<T, true>(r, rp, dg, wc, v, w, lane, eps, done, stall);
 }
+// LDS bytes of the completion flags for leaves of up to max_leaf rows
+int leaf_flag_bytes(int max_leaf) { return (max_leaf + 15) & ~15; }
 template <typename T>
-void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, double eps,
-                      hipStream_t st) {
+void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, int max_leaf,
+                      double eps, int *stall, hipStream_t st) {
   if (nleaves <= 0) return;
-  size_t lds = (size_t)wsz * waves * sizeof(T);
+  const int fl = leaf_flag_bytes(max_leaf);
+  size_t lds = (size_t)fl + (size_t)wsz * waves * sizeof(T);
   (void)hipFuncSetAttribute((const void *)k_leaf_rows<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_leaf_rows<T>, dim3(nleaves), dim3(64 * waves), lds, st, leaves, wsz,
-                     (T)eps);
+  hipLaunchKernelGGL(k_leaf_rows<T>, dim3(nleaves), dim3(64 * waves), lds, st, leaves, wsz, fl,
+                     (T)eps, stall);
 }
 
 // Separator rows of the leaves: each wave factors one boundary row (its L_SL
@@ -214,7 +254,7 @@ __global__ __launch_bounds__(512) void k_leaf_sep(const LeafDev *__restrict__ le
   for (int c = lane; c < ls + L.nbc; c += 64) w[c] = T(0);
   wave_lds_fence();
   const int r = L.brow[ri];
-  factor_row<T, false>(r, L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0));
+  factor_row<T, false>(r, L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0), nullptr, nullptr);
   T *Srow = (T *)L.S + (long long)ri * L.nbc;
   for (int j = lane; j < L.nbc; j += 64) {
     Srow[j] = w[ls + j];
@@ -315,11 +355,13 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   const int q = nb >> 3, r = nb & 7, x = bid & 7, i = bid >> 3;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
+// row_ror within rows of 16 lanes reads a valid lane for every lane, so the
+// destination's old value is never kept (no zeroing needed)
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
   int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xf, 0xf, false);
-  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xf, 0xf, false);
+  lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, false);
   return __hiloint2double(hi, lo);
 }
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -421,13 +463,18 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
     advance(nsg, nk0);
     const bool more = nsg < P.nseg;
     if (more) load(nsg, nk0);
+    // all LDS operands of the chunk first (one wait), then the MFMA block
+    T bvk[KB / 4][BNB], avk[KB / 4][BMB];
 #pragma unroll
-    for (int kk = 0; kk < KB; kk += 4) {
-      T bv[BNB], av[BMB];
+    for (int kq = 0; kq < KB / 4; kq++) {
 #pragma unroll
-      for (int b = 0; b < BNB; b++) bv[b] = Bs[buf][kk + lk][wn + b * 16 + l16];
+      for (int b = 0; b < BNB; b++) bvk[kq][b] = Bs[buf][4 * kq + lk][wn + b * 16 + l16];
 #pragma unroll
-      for (int a = 0; a < BMB; a++) av[a] = As[buf][kk + lk][wm + a * 16 + l16];
+      for (int a = 0; a < BMB; a++) avk[kq][a] = As[buf][4 * kq + lk][wm + a * 16 + l16];
+    }
+#pragma unroll
+    for (int kq = 0; kq < KB / 4; kq++) {
+      const T *bv = bvk[kq], *av = avk[kq];
       if constexpr (F64) {
         double avr[BMB][4];
 #pragma unroll
@@ -721,36 +768,63 @@ void launch_permute_out(const PermDev *pd, int nmat, const VecDev *vd, const T *
                      pd, vd, bp);
 }
 
-// y_L = L_LL^-1 b_L (unit lower), level by level, one wave per row.
+// Wait until every row named in cols[p0, p1) (leaf rows) has its flag
+// raised; bounded like the factorization's wait.
+__device__ __forceinline__ void wait_rows(const int *__restrict__ cols, int p0, int p1, int lane,
+                                          volatile unsigned char *done, int limit_col,
+                                          int *stall) {
+  constexpr int kSpinLimit = 1 << 26;
+  for (int p = p0; p < p1; p += 64) {
+    const int q = p + lane;
+    int c = q < p1 ? cols[q] : -1;
+    for (int spin = 0;; spin++) {
+      const bool ready = c < 0 || c >= limit_col || done[c] != 0;
+      if (__ballot(!ready) == 0) break;
+      if (spin == kSpinLimit) {
+        if (lane == 0) *stall = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// y_L = L_LL^-1 b_L (unit lower).  Wave w solves leaf rows w, w + W, ... in
+// ascending order, each once the rows it reads are solved (LDS flags), all
+// right-hand-side columns of the matrix per row.
 template <typename T>
 __global__ __launch_bounds__(256) void k_leaf_fwd(const LeafDev *__restrict__ leaves,
-                                                  const VecDev *__restrict__ vd, T *bp) {
+                                                  const VecDev *__restrict__ vd, T *bp, int *stall) {
+  extern __shared__ __align__(16) char flags_raw[];
+  volatile unsigned char *done = (volatile unsigned char *)flags_raw;
   const LeafDev L = leaves[blockIdx.x];
   const VecDev V = vd[L.vec];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
+  for (int r = threadIdx.x; r < L.leaf_size; r += blockDim.x) done[r] = 0;
+  __syncthreads();
   const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol;
   const T *v = (const T *)L.vals;
-  const int *lp = L.levptr, *lr = L.levrows;
-  for (int lev = 0; lev < L.nlev; lev++) {
-    for (int t = lp[lev] + wave; t < lp[lev + 1]; t += W) {
-      const int i = lr[t];
-      for (int col = 0; col < V.nrhs; col++) {
-        T *b = bp + V.base + (long long)col * V.ld + L.gstart;
-        T s = T(0);
-        for (int p = rp[i] + lane; p < dg[i]; p += 64) s += v[p] * b[wc[p]];
-        s = wave_sum(s);
-        if (lane == 0) b[i] -= s;
-      }
+  for (int i = wave; i < L.leaf_size; i += W) {
+    wait_rows(wc, rp[i], dg[i], lane, done, L.leaf_size, stall);
+    for (int col = 0; col < V.nrhs; col++) {
+      T *b = bp + V.base + (long long)col * V.ld + L.gstart;
+      T s = T(0);
+      for (int p = rp[i] + lane; p < dg[i]; p += 64) s += v[p] * b[wc[p]];
+      s = wave_sum(s);
+      if (lane == 0) b[i] -= s;
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) done[i] = 1;
   }
 }
 template <typename T>
-void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp,
-                     hipStream_t st) {
+void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
+                     int *stall, hipStream_t st) {
   if (nleaves <= 0) return;
-  hipLaunchKernelGGL(k_leaf_fwd<T>, dim3(nleaves), dim3(256), 0, st, leaves, vd, bp);
+  hipLaunchKernelGGL(k_leaf_fwd<T>, dim3(nleaves), dim3(256), leaf_flag_bytes(max_leaf), st,
+                     leaves, vd, bp, stall);
 }
 
 // Boundary rows: cbuf[ri] = sum_k L_SL(r, k) y_k (kernel_sparse.cu:935-944).
@@ -820,41 +894,46 @@ void launch_assemble_rhs(int ntarget, const int *trow_glob, const int *tptr,
                      max_rhs);
 }
 
-// x_L = U_LL^-1 (y_L - U_LS x_S), level by level (SnuMat_factorize.cu:331-344).
+// x_L = U_LL^-1 (y_L - U_LS x_S) (SnuMat_factorize.cu:331-344).  Wave w
+// solves leaf rows ls-1-w, ls-1-w-W, ... in descending order, each once the
+// leaf rows its U part reads are solved (boundary columns are final).
 template <typename T>
 __global__ __launch_bounds__(256) void k_leaf_bwd(const LeafDev *__restrict__ leaves,
-                                                  const VecDev *__restrict__ vd, T *bp) {
+                                                  const VecDev *__restrict__ vd, T *bp, int *stall) {
+  extern __shared__ __align__(16) char flags_raw[];
+  volatile unsigned char *done = (volatile unsigned char *)flags_raw;
   const LeafDev L = leaves[blockIdx.x];
   const VecDev V = vd[L.vec];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
   const int ls = L.leaf_size;
+  for (int r = threadIdx.x; r < ls; r += blockDim.x) done[r] = 0;
+  __syncthreads();
   const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol, *bg = L.bcol_glob;
   const T *v = (const T *)L.vals;
-  const int *lp = L.ulevptr, *lr = L.ulevrows;
-  for (int lev = 0; lev < L.nulev; lev++) {
-    for (int t = lp[lev] + wave; t < lp[lev + 1]; t += W) {
-      const int i = lr[t];
-      for (int col = 0; col < V.nrhs; col++) {
-        T *b = bp + V.base + (long long)col * V.ld;
-        T s = T(0);
-        for (int p = dg[i] + 1 + lane; p < rp[i + 1]; p += 64) {
-          int c = wc[p];
-          long long g = c < ls ? (long long)L.gstart + c : (long long)bg[c - ls];
-          s += v[p] * b[g];
-        }
-        s = wave_sum(s);
-        if (lane == 0) b[L.gstart + i] = (b[L.gstart + i] - s) / v[dg[i]];
+  for (int i = ls - 1 - wave; i >= 0; i -= W) {
+    wait_rows(wc, dg[i] + 1, rp[i + 1], lane, done, ls, stall);
+    for (int col = 0; col < V.nrhs; col++) {
+      T *b = bp + V.base + (long long)col * V.ld;
+      T s = T(0);
+      for (int p = dg[i] + 1 + lane; p < rp[i + 1]; p += 64) {
+        int c = wc[p];
+        long long g = c < ls ? (long long)L.gstart + c : (long long)bg[c - ls];
+        s += v[p] * b[g];
       }
+      s = wave_sum(s);
+      if (lane == 0) b[L.gstart + i] = (b[L.gstart + i] - s) / v[dg[i]];
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) done[i] = 1;
   }
 }
 template <typename T>
-void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp,
-                     hipStream_t st) {
+void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
+                     int *stall, hipStream_t st) {
   if (nleaves <= 0) return;
-  hipLaunchKernelGGL(k_leaf_bwd<T>, dim3(nleaves), dim3(256), 0, st, leaves, vd, bp);
+  hipLaunchKernelGGL(k_leaf_bwd<T>, dim3(nleaves), dim3(256), leaf_flag_bytes(max_leaf), st,
+                     leaves, vd, bp, stall);
 }
 
 // ---------------------------------------------------------------------------
@@ -915,6 +994,26 @@ __global__ __launch_bounds__(256) void k_colmax(const IrDev *__restrict__ tab, i
     m = block_max(m, red);
     if (threadIdx.x == 0) atomicMax(out + M.norm0 + c, (unsigned long long)__double_as_longlong(m));
   }
+}
+// Plain streaming copy (16 B per lane per iteration): the reference kernel
+// for scaling the HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src,
+                                              double2 *__restrict__ dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+void launch_copy(const void *src, void *dst, long long bytes, hipStream_t st) {
+  const long long n = bytes / 16;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, st, (const double2 *)src, (double2 *)dst, n);
+}
+__global__ void k_zero_i32(int *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+void launch_zero_i32(int *p, int n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_zero_i32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
 }
 __global__ void k_zero_u64(unsigned long long *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1010,7 +1109,8 @@ int device_lds_limit() {
 #define SNU_INSTANTIATE(T)                                                                    \
   template void launch_fill<T>(T *, long long, hipStream_t);                                  \
   template void launch_prepare<T>(const PrepDev *, int, long long, long long, hipStream_t);   \
-  template void launch_leaf_rows<T>(const LeafDev *, int, int, int, double, hipStream_t);     \
+  template void launch_leaf_rows<T>(const LeafDev *, int, int, int, int, double, int *,       \
+                                    hipStream_t);                                              \
   template void launch_leaf_sep<T>(const LeafDev *, const int *, const int *, int, int, int,  \
                                    hipStream_t);                                              \
   template void launch_assemble_schur<T>(int, const int *, const int *, const int *,          \
@@ -1027,13 +1127,15 @@ int device_lds_limit() {
                                      hipStream_t);                                            \
   template void launch_permute_out<T>(const PermDev *, int, const VecDev *, const T *,        \
                                       long long, hipStream_t);                                \
-  template void launch_leaf_fwd<T>(const LeafDev *, int, const VecDev *, T *, hipStream_t);   \
+  template void launch_leaf_fwd<T>(const LeafDev *, int, const VecDev *, T *, int, int *,     \
+                                   hipStream_t);                                              \
   template void launch_leaf_fwd_sep<T>(const LeafDev *, const int *, const int *, int, int,   \
                                        const VecDev *, const T *, T *, int, hipStream_t);     \
   template void launch_assemble_rhs<T>(int, const int *, const int *, const int *,            \
                                        const int *, const LeafDev *, const VecDev *,          \
                                        const T *, int, T *, int, hipStream_t);                \
-  template void launch_leaf_bwd<T>(const LeafDev *, int, const VecDev *, T *, hipStream_t);
+  template void launch_leaf_bwd<T>(const LeafDev *, int, const VecDev *, T *, int, int *,     \
+                                   hipStream_t);
 SNU_INSTANTIATE(double)
 SNU_INSTANTIATE(float)
 #undef SNU_INSTANTIATE

@@ -1,0 +1,381 @@
+// Serving loop over resident factorizations (no reference counterpart).
+//
+// A server keeps factorized matrices resident in HBM under a byte budget and
+// answers solve requests (matrix id, b, nrhs) submitted from any thread.
+// Worker threads ("slots") each own a batch over the resident set and its
+// stream: a free slot takes every pending request (up to a column cap),
+// packs them -- several requests for one matrix become columns of one
+// multi-rhs solve, different matrices with uneven column counts share one
+// launch per sweep step and tree height -- and runs the packed solve.
+// Requests that arrive while slots are busy wait for the next free slot;
+// slots run concurrently, so their packed solves overlap on the device.
+// New factorizations (snuserve_add) are analysed on the host, admitted
+// against the budget by evicting least-recently-used idle matrices, then
+// uploaded and factorized on the caller's thread while slots keep serving.
+//
+// Built from the public C-ABI only (snumat_*, snubatch_*).
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <shared_mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "snu_internal.h"
+
+namespace {
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  int id = 0;
+  const double *b = nullptr;
+  double *x = nullptr;
+  int nrhs = 0, ld = 0;
+  double t_submit = 0, t_done = 0;
+  int status = 0;
+  bool done = false;
+};
+
+struct Resident {
+  snumat_t *h = nullptr;
+  int n = 0;
+  size_t bytes = 0;
+  double last_use = 0;
+  int busy = 0;  // pending or in-flight requests
+};
+
+}  // namespace
+
+struct snuserve {
+  int device = 0, prec = -1, max_cols = 256;
+  size_t budget = 0, used = 0;
+  std::mutex mu;                 // requests, resident bookkeeping, stats
+  std::condition_variable cv_work, cv_done;
+  std::shared_mutex set_mu;      // shared: a packed solve; exclusive: eviction
+  std::mutex add_mu;             // one admission at a time
+  std::map<int, Resident> res;
+  int next_id = 1;
+  unsigned long long generation = 0;
+  std::deque<long long> pending;
+  std::unordered_map<long long, Request> reqs;
+  long long next_ticket = 1;
+  std::vector<std::thread> workers;
+  bool stop = false;
+  // statistics
+  std::vector<double> lat;
+  double cols = 0, packs = 0, evictions = 0, admissions = 0, busy_ms = 0;
+  double t_first = 0, t_last = 0;
+};
+
+namespace {
+
+// One slot: a batch over the resident set (rebuilt when the set changes).
+void worker(snuserve *s) {
+  snubatch_t *batch = nullptr;
+  unsigned long long gen = ~0ull;
+  std::vector<int> ids;          // batch member order
+  std::vector<snumat_t *> mats;  // batch members
+  for (;;) {
+    std::vector<long long> take;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+      if (s->pending.empty()) break;  // stop requested and nothing left
+      int cols = 0;
+      while (!s->pending.empty()) {
+        const Request &r = s->reqs[s->pending.front()];
+        if (!take.empty() && cols + r.nrhs > s->max_cols) break;
+        cols += r.nrhs;
+        take.push_back(s->pending.front());
+        s->pending.pop_front();
+      }
+    }
+    const double t0 = now_ms();
+    int rc = SNU_OK;
+    std::map<int, std::vector<long long>> by_id;
+    {
+      std::shared_lock<std::shared_mutex> sl(s->set_mu);
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (gen != s->generation) {
+          ids.clear();
+          mats.clear();
+          for (auto &kv : s->res) ids.push_back(kv.first), mats.push_back(kv.second.h);
+          gen = s->generation;
+          if (batch) snubatch_destroy(batch);
+          batch = nullptr;
+        }
+        for (long long t : take) by_id[s->reqs[t].id].push_back(t);
+      }
+      if (!batch && !ids.empty()) rc = snubatch_create(mats.data(), (int)mats.size(), &batch);
+      if (!rc && batch) {
+        // pack: per member, the columns of its requests side by side
+        const int nm = (int)ids.size();
+        std::vector<std::vector<double>> B(nm), X(nm);
+        std::vector<const double *> pb(nm, nullptr);
+        std::vector<double *> px(nm, nullptr);
+        std::vector<int> nrhs(nm, 0), ld(nm, 1);
+        std::unordered_map<int, int> slot;
+        for (int m = 0; m < nm; m++) slot[ids[m]] = m;
+        std::vector<std::pair<int, int>> where(take.size(), {-1, 0});  // (member, first column)
+        {
+          std::lock_guard<std::mutex> lk(s->mu);
+          for (auto &kv : by_id) {
+            auto it = slot.find(kv.first);
+            if (it == slot.end()) continue;  // evicted: answered below
+            const int m = it->second, n = s->res[kv.first].n;
+            int k = 0;
+            for (long long t : kv.second) k += s->reqs[t].nrhs;
+            B[m].resize((size_t)n * k);
+            X[m].resize((size_t)n * k);
+            int c = 0;
+            for (long long t : kv.second) {
+              const Request &r = s->reqs[t];
+              for (int j = 0; j < r.nrhs; j++)
+                std::memcpy(&B[m][(size_t)(c + j) * n], r.b + (size_t)j * r.ld, sizeof(double) * n);
+              const size_t pos = std::find(take.begin(), take.end(), t) - take.begin();
+              where[pos] = {m, c};
+              c += r.nrhs;
+            }
+            nrhs[m] = k;
+            ld[m] = n;
+            pb[m] = B[m].data();
+            px[m] = X[m].data();
+          }
+        }
+        rc = snubatch_solve(batch, pb.data(), px.data(), nrhs.data(), ld.data());
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (size_t q = 0; q < take.size(); q++) {
+          Request &r = s->reqs[take[q]];
+          if (where[q].first < 0) continue;
+          const int m = where[q].first, c0 = where[q].second, n = s->res[ids[m]].n;
+          if (!rc || rc == SNU_ERR_NOT_CONVERGED)
+            for (int j = 0; j < r.nrhs; j++)
+              std::memcpy(r.x + (size_t)j * r.ld, &X[m][(size_t)(c0 + j) * n], sizeof(double) * n);
+        }
+      }
+    }
+    const double t1 = now_ms();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (long long t : take) {
+        Request &r = s->reqs[t];
+        auto it = s->res.find(r.id);
+        if (it == s->res.end()) {
+          r.status = SNU_ERR_BAD_INPUT;
+        } else {
+          r.status = rc;
+          it->second.busy--;
+          it->second.last_use = t1;
+        }
+        r.t_done = t1;
+        r.done = true;
+        s->lat.push_back(r.t_done - r.t_submit);
+        s->cols += r.nrhs;
+      }
+      s->packs += 1;
+      s->busy_ms += t1 - t0;
+      s->t_last = std::max(s->t_last, t1);
+    }
+    s->cv_done.notify_all();
+  }
+  if (batch) snubatch_destroy(batch);
+}
+
+}  // namespace
+
+extern "C" int snuserve_create(int device, size_t budget_bytes, int slots, int max_cols,
+                               snuserve_t **out) {
+  if (!out || slots < 1 || slots > 8 || max_cols < 1) return snu::set_error(SNU_ERR_BAD_INPUT, "bad server arguments");
+  int rc = snu_initialize(device);
+  if (rc) return rc;
+  snuserve *s = new snuserve();
+  s->device = device;
+  s->budget = budget_bytes;
+  s->max_cols = max_cols;
+  for (int i = 0; i < slots; i++) s->workers.emplace_back(worker, s);
+  *out = s;
+  return SNU_OK;
+}
+
+// Admission: host analysis, eviction of idle least-recently-used matrices
+// until the new one fits the budget, upload, factorization.
+extern "C" int snuserve_add(snuserve_t *s, const snu_csr_t *A, const snu_opts_t *opts, int *id) {
+  if (!s || !A || !opts || !id) return snu::set_error(SNU_ERR_BAD_INPUT, "null");
+  std::lock_guard<std::mutex> admit(s->add_mu);
+  if (s->prec >= 0 && opts->precision != s->prec)
+    return snu::set_error(SNU_ERR_BAD_INPUT, "a server holds one precision");
+  snu_opts_t o = *opts;
+  o.device = -1;
+  snumat_t *h = nullptr;
+  int rc = snumat_construct(A, &o, &h);
+  if (rc) return rc;
+  size_t need = 0;
+  snumat_bytes_needed(h, &need);
+  if (need > s->budget) {
+    snumat_destroy(h);
+    return snu::set_error(SNU_ERR_OOM, "matrix needs " + std::to_string(need) + " bytes, budget " +
+                                           std::to_string(s->budget));
+  }
+  for (;;) {
+    std::vector<snumat_t *> victims;
+    {
+      std::unique_lock<std::shared_mutex> ex(s->set_mu);  // no packed solve in flight
+      std::lock_guard<std::mutex> lk(s->mu);
+      while (s->used + need > s->budget) {
+        int best = -1;
+        double t = 0;
+        for (auto &kv : s->res)
+          if (kv.second.busy == 0 && (best < 0 || kv.second.last_use < t)) best = kv.first, t = kv.second.last_use;
+        if (best < 0) break;
+        victims.push_back(s->res[best].h);
+        s->used -= s->res[best].bytes;
+        s->res.erase(best);
+        s->evictions += 1;
+        s->generation++;
+      }
+      for (snumat_t *v : victims) snumat_destroy(v);
+      if (s->used + need <= s->budget) {
+        s->used += need;  // reserved
+        break;
+      }
+    }
+    // every remaining matrix is busy: wait for a request to finish
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait_for(lk, std::chrono::milliseconds(5));
+  }
+  rc = snumat_attach(h, s->device, A->data);
+  if (!rc) rc = snumat_factorize(h);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (rc) {
+    s->used -= need;
+    snumat_destroy(h);
+    return rc;
+  }
+  Resident r;
+  r.h = h;
+  r.n = A->n;
+  r.bytes = need;
+  r.last_use = now_ms();
+  *id = s->next_id++;
+  s->res[*id] = r;
+  s->prec = opts->precision;
+  s->generation++;
+  s->admissions += 1;
+  return SNU_OK;
+}
+
+extern "C" int snuserve_remove(snuserve_t *s, int id) {
+  if (!s) return snu::set_error(SNU_ERR_BAD_INPUT, "null");
+  std::unique_lock<std::shared_mutex> ex(s->set_mu);
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->res.find(id);
+  if (it == s->res.end()) return snu::set_error(SNU_ERR_BAD_INPUT, "unknown matrix id");
+  if (it->second.busy) return snu::set_error(SNU_ERR_BAD_INPUT, "matrix has requests in flight");
+  snumat_destroy(it->second.h);
+  s->used -= it->second.bytes;
+  s->res.erase(it);
+  s->generation++;
+  return SNU_OK;
+}
+
+extern "C" int snuserve_submit(snuserve_t *s, int id, const double *b, double *x, int nrhs, int ld,
+                               long long *ticket) {
+  if (!s || !b || !x || nrhs < 1 || !ticket) return snu::set_error(SNU_ERR_BAD_INPUT, "bad request");
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->res.find(id);
+    if (it == s->res.end()) return snu::set_error(SNU_ERR_BAD_INPUT, "unknown or evicted matrix id");
+    if (ld < it->second.n) return snu::set_error(SNU_ERR_BAD_INPUT, "ld < n");
+    if (nrhs > s->max_cols) return snu::set_error(SNU_ERR_BAD_INPUT, "nrhs above the server's column cap");
+    it->second.busy++;
+    Request r;
+    r.id = id;
+    r.b = b;
+    r.x = x;
+    r.nrhs = nrhs;
+    r.ld = ld;
+    r.t_submit = now_ms();
+    if (s->t_first == 0) s->t_first = r.t_submit;
+    *ticket = s->next_ticket++;
+    s->reqs[*ticket] = r;
+    s->pending.push_back(*ticket);
+  }
+  s->cv_work.notify_one();
+  return SNU_OK;
+}
+
+extern "C" int snuserve_wait(snuserve_t *s, long long ticket, double *latency_ms) {
+  if (!s) return snu::set_error(SNU_ERR_BAD_INPUT, "null");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return snu::set_error(SNU_ERR_BAD_INPUT, "unknown ticket");
+  s->cv_done.wait(lk, [&] { return it->second.done; });
+  const int st = it->second.status;
+  if (latency_ms) *latency_ms = it->second.t_done - it->second.t_submit;
+  s->reqs.erase(it);
+  if (st) return snu::set_error(st, "request failed with status " + std::to_string(st));
+  return SNU_OK;
+}
+
+// st[0] requests answered, st[1] columns solved, st[2] packed solves,
+// st[3] p50 latency ms, st[4] p99 ms, st[5] mean ms, st[6] max ms,
+// st[7] evictions, st[8] admissions, st[9] resident matrices, st[10] bytes
+// reserved, st[11] wall ms from the first submit to the last answer,
+// st[12] summed slot busy ms.  reset != 0 clears the latency record.
+extern "C" int snuserve_stats(snuserve_t *s, double *st16, int reset) {
+  if (!s || !st16) return snu::set_error(SNU_ERR_BAD_INPUT, "null");
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::memset(st16, 0, 16 * sizeof(double));
+  std::vector<double> l = s->lat;
+  std::sort(l.begin(), l.end());
+  auto pct = [&](double p) {
+    if (l.empty()) return 0.0;
+    size_t k = (size_t)std::min<double>(l.size() - 1, std::max(0.0, p * l.size() - 1e-9));
+    return l[k];
+  };
+  double sum = 0;
+  for (double v : l) sum += v;
+  st16[0] = (double)l.size();
+  st16[1] = s->cols;
+  st16[2] = s->packs;
+  st16[3] = pct(0.50);
+  st16[4] = pct(0.99);
+  st16[5] = l.empty() ? 0 : sum / l.size();
+  st16[6] = l.empty() ? 0 : l.back();
+  st16[7] = s->evictions;
+  st16[8] = s->admissions;
+  st16[9] = (double)s->res.size();
+  st16[10] = (double)s->used;
+  st16[11] = s->t_last > s->t_first ? s->t_last - s->t_first : 0;
+  st16[12] = s->busy_ms;
+  if (reset) {
+    s->lat.clear();
+    s->cols = s->packs = s->busy_ms = 0;
+    s->t_first = s->t_last = 0;
+  }
+  return SNU_OK;
+}
+
+extern "C" int snuserve_destroy(snuserve_t *s) {
+  if (!s) return SNU_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (auto &t : s->workers) t.join();
+  for (auto &kv : s->res) snumat_destroy(kv.second.h);
+  delete s;
+  return SNU_OK;
+}

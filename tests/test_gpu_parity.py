@@ -261,3 +261,77 @@ def test_batch_equals_standalone(S, precision):
         print(f"batch of {len(mats)}: {fl} dense factor launches, {sl} dense solve launches")
     for M in mats:
         M.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_serve_alone_vs_mix(S, precision):
+    """Requests against several resident matrices, with uneven column counts,
+    submitted together and packed by two slots: every answer equals the
+    matrix's standalone solve of the same columns (bitwise for FP64; within
+    1e-10 relative for FP32 + refinement, whose step count depends on the
+    columns packed together), backward error <= 1e-12."""
+    mem = _batch_members(S)[2:7]
+    rng = np.random.default_rng(11)
+    reqs = []
+    for q in range(30):
+        m = int(rng.integers(len(mem)))
+        k = int(2 ** rng.integers(0, 4))
+        reqs.append((m, rng.standard_normal((mem[m][0].n, k))))
+    alone = []
+    for m, b in reqs:
+        A, np_, (order, sizes) = mem[m]
+        with S.SnuMat(A, np_, order=order, sizes=sizes, device=0, precision=precision) as M:
+            alone.append(M.factorize().solve(b))
+    with S.Server(budget_bytes=64 << 30, slots=2, max_cols=64) as srv:
+        ids = [srv.add(A, np_, order=order, sizes=sizes, precision=precision) for A, np_, (order, sizes) in mem]
+        tickets = [srv.submit(ids[m], b) for m, b in reqs]
+        for (m, b), t, xa in zip(reqs, tickets, alone):
+            x, lat = srv.wait(t)
+            A = mem[m][0]
+            assert S.backward_error(A, x, b) <= BERR_TOL
+            if precision == "fp64":
+                assert np.array_equal(x, xa)
+            else:
+                assert np.max(np.abs(x - xa)) <= 1e-10 * np.max(np.abs(xa))
+        st = srv.stats()
+        print(f"served {st['requests']:.0f} requests in {st['packed_solves']:.0f} packed solves, "
+              f"p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms")
+        assert st["requests"] == len(reqs)
+        assert st["packed_solves"] <= len(reqs)
+
+
+def test_serve_admission_evicts_lru(S):
+    """Admission under a budget evicts the least recently used idle matrix;
+    requests against an evicted id are refused."""
+    mats = []
+    for i, side in enumerate((48, 50, 52)):
+        A = S.gen_grid(side, side, 1, 5, 0, 4.1, i)
+        mats.append((A, S.order_grid(side, side, 1, 4)))
+    need = []
+    for A, (order, sizes) in mats:
+        with S.SnuMat(A, 4, order=order, sizes=sizes, device=-1, precision="fp32") as M:
+            nb = C_size()
+            S.lib().snumat_bytes_needed(M._h, C_ref(nb))
+            need.append(nb.value)
+    budget = need[0] + need[1] + need[2] // 2  # room for two of the three
+    with S.Server(budget_bytes=budget, slots=1) as srv:
+        a = srv.add(mats[0][0], 4, order=mats[0][1][0], sizes=mats[0][1][1])
+        b = srv.add(mats[1][0], 4, order=mats[1][1][0], sizes=mats[1][1][1])
+        x, _ = srv.wait(srv.submit(a, np.ones(mats[0][0].n)))  # a is now the most recent
+        c = srv.add(mats[2][0], 4, order=mats[2][1][0], sizes=mats[2][1][1])
+        st = srv.stats()
+        assert st["evictions"] == 1 and st["resident"] == 2
+        with pytest.raises(S.SnuError):
+            srv.submit(b, np.ones(mats[1][0].n))
+        y, _ = srv.wait(srv.submit(c, np.ones(mats[2][0].n)))
+        assert S.backward_error(mats[2][0], y, np.ones(mats[2][0].n)) <= BERR_TOL
+
+
+def C_size():
+    import ctypes
+    return ctypes.c_size_t()
+
+
+def C_ref(v):
+    import ctypes
+    return ctypes.byref(v)
