@@ -276,9 +276,9 @@ int snu_test_gemm(int M, int N, int K, const double *A, int lda,
                   const double *B, int ldb, double *C, int ldc, int assign);
 int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,
                       const float *B, int ldb, float *C, int ldc, int assign);
-/* average time (ms) of the grouped C -= A B kernel with tile 64 or 128 on
- * device-resident operands, K split into nseg concatenated segments;
- * prec SNU_FP64 or SNU_FP32. */
+/* average time (ms) of the grouped C -= A B kernel on device-resident
+ * operands, K split into nseg concatenated segments; tile 64 or 128 (K chunk
+ * 16) or 6432 / 12832 (K chunk 32); prec SNU_FP64 or SNU_FP32. */
 int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                    double *ms_out);
 /* tiles3: 64 or 128, the tile of the grouped C -= A B kernel per size bucket

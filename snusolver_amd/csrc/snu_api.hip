@@ -88,9 +88,8 @@ template <typename T>
 int ensure_plan(snumat *h, int nrhs) {
   Values<T> &VV = vals<T>(h);
   if (VV.plan_nrhs == nrhs) return SNU_OK;
-  free_gemv_plan(VV.sp);
-  build_solve_plan<T>(h, VV.bp, VV.ys, nrhs, VV.sp);
-  int rc = upload_gemv_plan(VV.sp);
+  build_solve_plan<T>(h, VV.bp, VV.ys, nrhs, VV.sp);  // device arrays grow only
+  int rc = upload_gemv_plan(VV.sp, h->stream);
   if (rc) return rc;
   VV.plan_nrhs = nrhs;
   return SNU_OK;
@@ -291,7 +290,6 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
   pd[1] = PermDev{h->d_order, h->d_r, h->d_dx, (long long)n, (long long)n, n, 0};
   HIPCHK(hipMemcpyAsync(h->d_vec, &vd, sizeof(vd), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(h->d_perm, pd, sizeof(pd), hipMemcpyHostToDevice, st));
-  launch_zero_i32(h->wd.stall, 1, st);
   SolveSet ss;
   ss.perm = h->d_perm;
   ss.vec = h->d_vec;
@@ -336,9 +334,6 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
   HIPCHK(hipEventRecord(h->ev[7], st));
   HIPCHK(hipEventSynchronize(h->ev[7]));
   h->t[6] = ev_ms(h->ev[6], h->ev[7]);
-  int stalled = 0;
-  HIPCHK(hipMemcpy(&stalled, h->wd.stall, sizeof(int), hipMemcpyDeviceToHost));
-  if (stalled) return set_error(SNU_ERR_HIP, "leaf solve: a row's wait for its rows timed out");
   h->t[13] = h->ir_steps;
   h->t[14] = h->ir_berr;
   return rc;
@@ -652,7 +647,7 @@ extern "C" int snumat_set_profiling(snumat_t *h, int on) {
 // time over reps launches (HIP events).
 namespace {
 template <typename T>
-int bench_gemm(int M, int N, int K, int nseg, int tile, int reps, double *ms_out) {
+int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int reps, double *ms_out) {
   T *dA, *dB, *dC;
   HIPCHK(hipMalloc((void **)&dA, sizeof(T) * (size_t)M * K));
   HIPCHK(hipMalloc((void **)&dB, sizeof(T) * (size_t)K * N));
@@ -678,10 +673,7 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int reps, double *ms_out
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  auto run = [&]() {
-    if (tile == 128) launch_gemm128<T>(dgp, 1, dgs, tiles, nullptr);
-    else launch_gemm<T>(dgp, 1, dgs, tiles, 0, nullptr);
-  };
+  auto run = [&]() { launch_gemm_variant<T>(dgp, 1, dgs, tiles, tile, kb, nullptr); };
   run();
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipEventRecord(e0, nullptr));
@@ -703,11 +695,13 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int reps, double *ms_out
 }  // namespace
 extern "C" int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                               double *ms_out) {
-  if (M < 1 || N < 1 || K < 1 || nseg < 1 || reps < 1 || (tile != 64 && tile != 128) ||
+  // tile: 64 or 128 (K chunk 16), or 6432 / 12832 (K chunk 32)
+  const int kb = tile > 1000 ? 32 : 16, t = tile > 1000 ? tile / 100 : tile;
+  if (M < 1 || N < 1 || K < 1 || nseg < 1 || reps < 1 || (t != 64 && t != 128) ||
       (prec != SNU_FP64 && prec != SNU_FP32) || !ms_out)
     return set_error(SNU_ERR_BAD_INPUT, "bad gemm bench shape");
-  return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, tile, reps, ms_out)
-                          : bench_gemm<double>(M, N, K, nseg, tile, reps, ms_out);
+  return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, t, kb, reps, ms_out)
+                          : bench_gemm<double>(M, N, K, nseg, t, kb, reps, ms_out);
 }
 
 // Average time (ms) of a plain device copy of `bytes` (multiple of 16) over

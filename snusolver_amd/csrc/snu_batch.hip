@@ -153,9 +153,8 @@ int ensure_pattern(snubatch *b, const std::vector<int> &nrhs) {
     b->h_vec[m] = VecDev{base, b->mats[m]->P.n, nrhs[m]};
     base += (long long)b->mats[m]->P.n * nrhs[m];
   }
-  HIPCHK(hipMemcpy(b->d_vec, b->h_vec.data(), sizeof(VecDev) * nm, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(b->d_vec, b->h_vec.data(), sizeof(VecDev) * nm, hipMemcpyHostToDevice, b->stream));
   GemvPlan<T> &out = std::is_same<T, double>::value ? (GemvPlan<T> &)b->s64 : (GemvPlan<T> &)b->s32;
-  free_gemv_plan(out);
   std::vector<GemvPlan<T>> parts(nm);
   std::vector<const GemvPlan<T> *> ptrs;
   T *bp = (T *)b->bp, *ys = (T *)b->ys;
@@ -164,8 +163,8 @@ int ensure_pattern(snubatch *b, const std::vector<int> &nrhs) {
     build_solve_plan<T>(b->mats[m], bp + b->h_vec[m].base, ys + b->h_vec[m].base, nrhs[m], parts[m]);
     ptrs.push_back(&parts[m]);
   }
-  merge_gemv(ptrs, out);
-  if ((rc = upload_gemv_plan(out))) return rc;
+  merge_gemv(ptrs, out);  // keeps out's device arrays, which grow only
+  if ((rc = upload_gemv_plan(out, b->stream))) return rc;
   b->pattern = nrhs;
   return SNU_OK;
 }
@@ -226,7 +225,6 @@ int batch_solve(snubatch *b, const double *const *d_b, double *const *d_x, const
     max_n = std::max(max_n, n);
   }
   HIPCHK(hipMemcpyAsync(b->d_perm, pd.data(), sizeof(PermDev) * pd.size(), hipMemcpyHostToDevice, st));
-  launch_zero_i32(b->wd.stall, 1, st);
   SolveSet ss;
   ss.perm = b->d_perm;
   ss.vec = b->d_vec;
@@ -288,9 +286,6 @@ int batch_solve(snubatch *b, const double *const *d_b, double *const *d_x, const
   HIPCHK(hipEventRecord(b->ev[7], st));
   HIPCHK(hipEventSynchronize(b->ev[7]));
   b->t[6] = ev_ms(b->ev[6], b->ev[7]);
-  int stalled = 0;
-  HIPCHK(hipMemcpy(&stalled, b->wd.stall, sizeof(int), hipMemcpyDeviceToHost));
-  if (stalled) return set_error(SNU_ERR_HIP, "leaf solve: a row's wait for its rows timed out");
   b->t[13] = b->ir_steps;
   double worst = 0;
   for (double e : b->berr) worst = std::max(worst, e);

@@ -52,6 +52,7 @@ struct GemvPlan {
   std::vector<Launch> fwd, bwd;
   GemvProbT<T> *d_vp = nullptr;
   GemvSegT<T> *d_vs = nullptr;
+  size_t cap_vp = 0, cap_vs = 0;  // device capacities (grow-only)
 };
 
 // Value-typed device state of one matrix (FP64, or FP32 refined in FP64).
@@ -177,8 +178,11 @@ int add_gemm(std::vector<GemmProbT<T>> &gp, std::vector<GemmSegT<T>> &gs, int &t
 // this matrix's segment (ld = n, nrhs columns); keys order fwd / bwd.
 template <typename T>
 void build_solve_plan(const snumat *h, T *bp, T *ys, int nrhs, GemvPlan<T> &out);
+// Copy the plan's descriptors to the device on stream st, growing the
+// device arrays only when they are too small (the host vectors must stay
+// unchanged until st has passed the copy).
 template <typename T>
-int upload_gemv_plan(GemvPlan<T> &g);
+int upload_gemv_plan(GemvPlan<T> &g, hipStream_t st = nullptr);
 template <typename T>
 void free_gemv_plan(GemvPlan<T> &g);
 template <typename T>

@@ -435,19 +435,45 @@ void build_dense_plan(snumat *h) {
       int first = p << (l - lp), last = ((p + 1) << (l - lp)) - 1;
       int sp = P.block_size[p], ap = P.anc_sum[p];
       for (int pass = 0; pass < 2; pass++) {  // 0: R_p = [D_p | Up_p], 1: Lp_p
-        int M = pass == 0 ? sp : ap, N = pass == 0 ? sp + ap : sp;
-        for (int m0 = 0; m0 < M; m0 += 64)
-          for (int n0 = 0; n0 < N; n0 += 64) {
-            int mt = std::min(64, M - m0), nt = std::min(64, N - n0);
-            std::vector<GemmSeg> segs;
+        const int M = pass == 0 ? sp : ap, N = pass == 0 ? sp + ap : sp;
+        const int tm = (M + 63) / 64, tn = (N + 63) / 64;
+        // contributor set of every 64 x 64 target tile
+        std::vector<std::vector<int>> who((size_t)tm * tn);
+        for (int a = 0; a < tm; a++)
+          for (int b = 0; b < tn; b++) {
+            const int m0 = 64 * a, n0 = 64 * b, mt = std::min(64, M - m0), nt = std::min(64, N - n0);
             for (int i = first; i <= last; i++) {
-              int si = P.block_size[i], ai = P.anc_sum[i];
-              int off = P.anc_off(i, p);
-              int r0 = off + (pass == 0 ? 0 : sp) + m0, c0 = off + n0;
-              if (!P.anyL(i, r0, r0 + mt) || !P.anyU(i, c0, c0 + nt)) continue;
+              const int off = P.anc_off(i, p);
+              const int r0 = off + (pass == 0 ? 0 : sp) + m0, c0 = off + n0;
+              if (P.anyL(i, r0, r0 + mt) && P.anyU(i, c0, c0 + nt)) who[(size_t)a * tn + b].push_back(i);
+            }
+          }
+        // rectangles of tiles with one contributor set: runs along a tile
+        // row, then equal runs of consecutive tile rows stacked
+        std::vector<char> used((size_t)tm * tn, 0);
+        for (int a = 0; a < tm; a++)
+          for (int b = 0; b < tn; b++) {
+            const std::vector<int> &w = who[(size_t)a * tn + b];
+            if (used[(size_t)a * tn + b] || w.empty()) continue;
+            int b1 = b + 1;
+            while (b1 < tn && !used[(size_t)a * tn + b1] && who[(size_t)a * tn + b1] == w) b1++;
+            int a1 = a + 1;
+            for (; a1 < tm; a1++) {
+              bool same = true;
+              for (int q = b; q < b1 && same; q++)
+                same = !used[(size_t)a1 * tn + q] && who[(size_t)a1 * tn + q] == w;
+              if (!same) break;
+            }
+            for (int x = a; x < a1; x++)
+              for (int q = b; q < b1; q++) used[(size_t)x * tn + q] = 1;
+            const int m0 = 64 * a, n0 = 64 * b;
+            const int mt = std::min(M, 64 * a1) - m0, nt = std::min(N, 64 * b1) - n0;
+            std::vector<GemmSeg> segs;
+            for (int i : w) {
+              const int si = P.block_size[i], ai = P.anc_sum[i], off = P.anc_off(i, p);
+              const int r0 = off + (pass == 0 ? 0 : sp) + m0, c0 = off + n0;
               segs.push_back(GemmSeg{Lp(i) + r0, R(i) + (long long)(si + c0) * si, ai, si, si});
             }
-            if (segs.empty()) continue;
             T *C = pass == 0 ? R(p) + m0 + (long long)n0 * sp : Lp(p) + m0 + (long long)n0 * ap;
             sc.add(h_gs, C, pass == 0 ? sp : ap, mt, nt, segs);
           }
@@ -552,9 +578,22 @@ void build_solve_plan(const snumat *h, T *bp, T *ys, int nrhs, GemvPlan<T> &S) {
 }
 
 template <typename T>
-int upload_gemv_plan(GemvPlan<T> &g) {
+int upload_gemv_plan(GemvPlan<T> &g, hipStream_t st) {
+  auto grow = [](auto **p, size_t &cap, size_t need) -> int {
+    if (need <= cap && *p) return SNU_OK;
+    if (*p) HIPCHK(hipFree(*p));
+    *p = nullptr;
+    const size_t c = std::max<size_t>(need + need / 2, 64);
+    HIPCHK(hipMalloc((void **)p, c * sizeof(**p)));
+    cap = c;
+    return SNU_OK;
+  };
   int rc;
-  if ((rc = upload_list(&g.d_vp, g.vp)) || (rc = upload_list(&g.d_vs, g.vs))) return rc;
+  if ((rc = grow(&g.d_vp, g.cap_vp, g.vp.size())) || (rc = grow(&g.d_vs, g.cap_vs, g.vs.size()))) return rc;
+  if (!g.vp.empty())
+    HIPCHK(hipMemcpyAsync(g.d_vp, g.vp.data(), g.vp.size() * sizeof(g.vp[0]), hipMemcpyHostToDevice, st));
+  if (!g.vs.empty())
+    HIPCHK(hipMemcpyAsync(g.d_vs, g.vs.data(), g.vs.size() * sizeof(g.vs[0]), hipMemcpyHostToDevice, st));
   return SNU_OK;
 }
 template <typename T>
@@ -1024,14 +1063,14 @@ template <typename T>
 void run_sweep(const SolveSet &ss, T *bp, T *cbuf, const GemvPlan<T> &plan, hipStream_t st) {
   const WorkDev &wd = *ss.wd;
   launch_permute_in(ss.perm, ss.nmat, ss.vec, bp, ss.max_elems, st);
-  launch_leaf_fwd(wd.leaves, wd.nleaves, ss.vec, bp, wd.max_leaf, wd.stall, st);
+  launch_leaf_fwd(wd.leaves, wd.nleaves, ss.vec, bp, st);
   launch_leaf_fwd_sep(wd.leaves, wd.fwg_leaf, wd.fwg_first, wd.nfwg, 4, ss.vec, bp, cbuf, ss.cb_ld,
                       st);
   launch_assemble_rhs(wd.ntarget, wd.trow_glob, wd.tptr, wd.tsrc_leaf, wd.tsrc_ri, wd.leaves,
                       ss.vec, cbuf, ss.cb_ld, bp, ss.max_rhs, st);
   for (const Launch &L : plan.fwd) run_launch<T>(L, nullptr, nullptr, nullptr, plan.d_vp, plan.d_vs, st);
   for (const Launch &L : plan.bwd) run_launch<T>(L, nullptr, nullptr, nullptr, plan.d_vp, plan.d_vs, st);
-  launch_leaf_bwd(wd.leaves, wd.nleaves, ss.vec, bp, wd.max_leaf, wd.stall, st);
+  launch_leaf_bwd(wd.leaves, wd.nleaves, ss.vec, bp, st);
   launch_permute_out(ss.perm, ss.nmat, ss.vec, bp, ss.max_elems, st);
 }
 
@@ -1097,7 +1136,7 @@ int run_refined(const RefineSet &rs, float *bp, float *cbuf, const GemvPlan<floa
 
 #define SNU_ENGINE_INST(T)                                                                    \
   template void build_solve_plan<T>(const snumat *, T *, T *, int, GemvPlan<T> &);            \
-  template int upload_gemv_plan<T>(GemvPlan<T> &);                                            \
+  template int upload_gemv_plan<T>(GemvPlan<T> &, hipStream_t);                               \
   template void free_gemv_plan<T>(GemvPlan<T> &);                                             \
   template int upload_dense_plan<T>(DensePlan<T> &);                                          \
   template void free_dense_plan<T>(DensePlan<T> &);                                           \

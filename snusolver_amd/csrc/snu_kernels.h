@@ -147,6 +147,9 @@ void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
 template <typename T>
 void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                     int total_tiles, hipStream_t st);
+template <typename T>
+void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
+                         int total_tiles, int tile, int kb, hipStream_t st);
 // mode 0: y -= A x; 1: y = A x
 template <typename T>
 void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
@@ -161,8 +164,7 @@ template <typename T>
 void launch_permute_out(const PermDev *pd, int nmat, const VecDev *vd, const T *bp,
                         long long max_elems, hipStream_t st);
 template <typename T>
-void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
-                     int *stall, hipStream_t st);
+void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, hipStream_t st);
 template <typename T>
 void launch_leaf_fwd_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
                          int waves, const VecDev *vd, const T *bp, T *cbuf, int cb_ld,
@@ -173,8 +175,7 @@ void launch_assemble_rhs(int ntarget, const int *trow_glob, const int *tptr,
                          const VecDev *vd, const T *cbuf, int cb_ld, T *bp, int max_rhs,
                          hipStream_t st);
 template <typename T>
-void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
-                     int *stall, hipStream_t st);
+void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, hipStream_t st);
 
 // ---- FP64 iterative refinement (one matrix per table entry) ----
 void launch_residual(const IrDev *tab, int nmat, int max_n, hipStream_t st);

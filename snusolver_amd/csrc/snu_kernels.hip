@@ -350,7 +350,6 @@ void launch_assemble_schur(int ntarget, const int *trow_blk, const int *trow_loc
 // Workgroup ids are remapped so that each XCD (ids go round-robin over the 8
 // XCDs) works on a contiguous run of tiles, which share operand panels in
 // that XCD's L2.
-constexpr int KB = 16;
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   const int q = nb >> 3, r = nb & 7, x = bid & 7, i = bid >> 3;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
@@ -365,7 +364,7 @@ __device__ __forceinline__ double dpp_d(double v) {
   return __hiloint2double(hi, lo);
 }
 typedef float f4 __attribute__((ext_vector_type(4)));
-template <typename T, int MODE, int BMB, int BNB>
+template <typename T, int MODE, int BMB, int BNB, int KB>
 __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ probs,
                                               int nprob,
                                               const GemmSegT<T> *__restrict__ segs) {
@@ -422,7 +421,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
 #pragma unroll
     for (int i = 0; i < NB; i++) {
       int e = tid + 256 * i;
-      int gk = min(kk0 + (e & 15), S.K - 1), gn = min(n0 + (e >> 4), P.N - 1);
+      int gk = min(kk0 + e % KB, S.K - 1), gn = min(n0 + e / KB, P.N - 1);
       rb[i] = gptr(S.B)[gk + (long long)gn * S.ldb];
     }
   };
@@ -435,13 +434,13 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
 #pragma unroll
     for (int i = 0; i < NB; i++) {
       int e = tid + 256 * i;
-      Bs[buf][e & 15][e >> 4] = rb[i];
+      Bs[buf][e % KB][e / KB] = rb[i];
     }
-    if (kvalid < KB) {  // uniform: last chunk of a segment whose K % 16 != 0
+    if (kvalid < KB) {  // uniform: last chunk of a segment whose K % KB != 0
 #pragma unroll
       for (int i = 0; i < NB; i++) {
         int e = tid + 256 * i;
-        if ((e & 15) >= kvalid) Bs[buf][e & 15][e >> 4] = T(0);
+        if (e % KB >= kvalid) Bs[buf][e % KB][e / KB] = T(0);
       }
     }
   };
@@ -538,17 +537,32 @@ void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                  int total_tiles, int mode, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
   if (mode == 1)
-    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
   else
-    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
 }
 template <typename T>
 void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                     int total_tiles, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
-  hipLaunchKernelGGL((k_gemm<T, 0, 4, 4>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+}
+// C -= AB variants for tuning: tile 64 or 128, K chunk 16 or 32 (the tile
+// count is that of the tile size)
+template <typename T>
+void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
+                         int total_tiles, int tile, int kb, hipStream_t st) {
+  if (total_tiles <= 0 || nprob <= 0) return;
+  if (tile == 128 && kb == 32)
+    hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 32>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  else if (tile == 128)
+    hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  else if (kb == 32)
+    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 32>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  else
+    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
 }
 
 // ---------------------------------------------------------------------------
@@ -557,12 +571,13 @@ void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *seg
 // One workgroup per 64 rows of a problem; lane = row, the 4 waves take
 // interleaved columns and are combined in LDS (fixed order: deterministic).
 // A is read once, coalesced down its columns; x through scalar loads.
-// nrhs columns are processed in groups of 8 (A is re-read per group).
+// nrhs columns are processed in groups of 16 (A is re-read per group).
 template <typename T, int MODE>
 __global__ __launch_bounds__(256) void k_gemv(const GemvProbT<T> *__restrict__ probs,
                                               int nprob,
                                               const GemvSegT<T> *__restrict__ segs) {
-  __shared__ T red[4][8][64];
+  constexpr int CG = 16;  // right-hand-side columns per pass over A
+  __shared__ T red[4][CG][64];
   const int tile = blockIdx.x;
   int lo = 0, hi = nprob - 1;
   while (lo < hi) {
@@ -576,11 +591,11 @@ __global__ __launch_bounds__(256) void k_gemv(const GemvProbT<T> *__restrict__ p
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int m = m0 + lane, mc = min(m, P.M - 1);
-  for (int c0 = 0; c0 < nrhs; c0 += 8) {
-    const int nc = min(8, nrhs - c0);
-    T acc[8];
+  for (int c0 = 0; c0 < nrhs; c0 += CG) {
+    const int nc = min(CG, nrhs - c0);
+    T acc[CG];
 #pragma unroll
-    for (int c = 0; c < 8; c++) acc[c] = T(0);
+    for (int c = 0; c < CG; c++) acc[c] = T(0);
     for (int sg = 0; sg < P.nseg; sg++) {
       const GemvSegT<T> S = segs[P.seg0 + sg];
       int kb = 0, ke = S.K;
@@ -593,7 +608,7 @@ __global__ __launch_bounds__(256) void k_gemv(const GemvProbT<T> *__restrict__ p
         T a0 = A[(long long)k * S.lda], a1 = A[(long long)(k + 4) * S.lda];
         T a2 = A[(long long)(k + 8) * S.lda], a3 = A[(long long)(k + 12) * S.lda];
 #pragma unroll
-        for (int c = 0; c < 8; c++)
+        for (int c = 0; c < CG; c++)
           if (c < nc) {
             auto xc = X + (long long)c * S.ldx;
             acc[c] += a0 * xc[k] + a1 * xc[k + 4] + a2 * xc[k + 8] + a3 * xc[k + 12];
@@ -602,12 +617,12 @@ __global__ __launch_bounds__(256) void k_gemv(const GemvProbT<T> *__restrict__ p
       for (; k < ke; k += 4) {
         T a0 = A[(long long)k * S.lda];
 #pragma unroll
-        for (int c = 0; c < 8; c++)
+        for (int c = 0; c < CG; c++)
           if (c < nc) acc[c] += a0 * X[(long long)c * S.ldx + k];
       }
     }
 #pragma unroll
-    for (int c = 0; c < 8; c++) red[w][c][lane] = acc[c];
+    for (int c = 0; c < CG; c++) red[w][c][lane] = acc[c];
     __syncthreads();
     if (w == 0 && m < P.M) {
       for (int c = 0; c < nc; c++) {
@@ -647,41 +662,64 @@ __global__ __launch_bounds__(256) void k_diag_lu(const DiagProbT<T> *__restrict_
   __shared__ T xu[64 * 64];  // U^-1
   __shared__ T xl[64 * 64];  // L^-1
   __shared__ T t[64 * 64];   // products: upper-right blocks for U, lower-left for L
+  __shared__ T ub[2][64], lb[2][64];  // pivot row / pivot column, double-buffered
   const DiagProbT<T> P = probs[blockIdx.x];
   const int nb = P.nb, tid = threadIdx.x;
-  for (int e = tid; e < 64 * 64; e += 256) {
-    int r = e & 63, c = e >> 6;
-    a[e] = (r < nb && c < nb) ? P.D[r + (long long)c * P.ld] : (r == c ? T(1) : T(0));
+  // LU in registers: thread (r, g) holds row r, columns g, g + 4, ..., g + 60
+  // (padding: identity beyond nb).  Per pivot j one barrier: the pivot row
+  // and column are published in LDS, every thread forms its row's
+  // multiplier and updates its 16 entries.
+  const int r = tid & 63, g = tid >> 6;
+  T x[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int c = g + 4 * q;
+    x[q] = (r < nb && c < nb) ? P.D[r + (long long)c * P.ld] : (r == c ? T(1) : T(0));
   }
-  __syncthreads();
   for (int j = 0; j < nb; j++) {
-    const T piv = a[j + j * 64];
-    for (int r = j + 1 + tid; r < nb; r += 256) a[r + j * 64] /= piv;
-    __syncthreads();
-    const int w = nb - j - 1;
-    for (int e = tid; e < w * w; e += 256) {
-      int r = j + 1 + e % w, c = j + 1 + e / w;
-      a[r + c * 64] -= a[r + j * 64] * a[j + c * 64];
+    const int b = j & 1;
+    if (r == j) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) ub[b][g + 4 * q] = x[q];
+    }
+    if (g == (j & 3)) {
+      T v = x[0];
+#pragma unroll
+      for (int q = 1; q < 16; q++)
+        if (4 * q + g == j) v = x[q];
+      lb[b][r] = v;
     }
     __syncthreads();
+    if (r > j) {
+      const T l = lb[b][r] / ub[b][j];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int c = g + 4 * q;
+        if (c > j) x[q] -= l * ub[b][c];
+        else if (c == j) x[q] = l;
+      }
+    }
   }
-  for (int e = tid; e < nb * nb; e += 256) {
-    int r = e % nb, c = e / nb;
-    P.D[r + (long long)c * P.ld] = a[r + c * 64];
+#pragma unroll
+  for (int q = 0; q < 16; q++) {
+    const int c = g + 4 * q;
+    a[r + c * 64] = x[q];
+    if (r < nb && c < nb) P.D[r + (long long)c * P.ld] = x[q];
   }
+  __syncthreads();
   for (int e = tid; e < 64 * 64; e += 256) {
     int r = e & 63, c = e >> 6;
     xu[e] = r == c ? T(1) / a[e] : T(0);
     xl[e] = r == c ? T(1) : T(0);
   }
   __syncthreads();
-  for (int sz = 1; sz < 64; sz <<= 1) {
-    const int npair = 64 / (2 * sz), per = sz * sz, tot = npair * per;
+  for (int lg = 0; lg < 6; lg++) {
+    const int sz = 1 << lg, tot = 32 * sz;  // 64 / (2 sz) pairs of sz x sz blocks
     // t_U = B_U C^-1 (upper-right block), t_L = B_L A^-1 (lower-left block)
     for (int e = tid; e < 2 * tot; e += 256) {
       const bool up = e < tot;
       const int f = up ? e : e - tot;
-      const int p = f / per, i = f % per % sz, j = f % per / sz;
+      const int p = f >> (2 * lg), i = f & (sz - 1), j = (f >> lg) & (sz - 1);
       const int r0 = p * 2 * sz, c0 = r0 + sz;
       T acc = T(0);
       if (up) {  // U[r0+i][c0+k] * xu[c0+k][c0+j], k <= j
@@ -696,7 +734,7 @@ __global__ __launch_bounds__(256) void k_diag_lu(const DiagProbT<T> *__restrict_
     for (int e = tid; e < 2 * tot; e += 256) {
       const bool up = e < tot;
       const int f = up ? e : e - tot;
-      const int p = f / per, i = f % per % sz, j = f % per / sz;
+      const int p = f >> (2 * lg), i = f & (sz - 1), j = (f >> lg) & (sz - 1);
       const int r0 = p * 2 * sz, c0 = r0 + sz;
       T acc = T(0);
       if (up) {  // -A^-1 t_U: xu[r0+i][r0+k] * t[r0+k][c0+j], k >= i
@@ -709,10 +747,12 @@ __global__ __launch_bounds__(256) void k_diag_lu(const DiagProbT<T> *__restrict_
     }
     __syncthreads();
   }
-  for (int e = tid; e < nb * nb; e += 256) {
-    int r = e % nb, c = e / nb;
-    P.Uinv[r + c * P.ldi] = xu[r + c * 64];
-    P.Linv[r + c * P.ldi] = xl[r + c * 64];
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int rr = e & 63, cc = e >> 6;
+    if (rr < nb && cc < nb) {
+      P.Uinv[rr + cc * P.ldi] = xu[e];
+      P.Linv[rr + cc * P.ldi] = xl[e];
+    }
   }
 }
 template <typename T>
@@ -768,63 +808,35 @@ void launch_permute_out(const PermDev *pd, int nmat, const VecDev *vd, const T *
                      pd, vd, bp);
 }
 
-// Wait until every row named in cols[p0, p1) (leaf rows) has its flag
-// raised; bounded like the factorization's wait.
-__device__ __forceinline__ void wait_rows(const int *__restrict__ cols, int p0, int p1, int lane,
-                                          volatile unsigned char *done, int limit_col,
-                                          int *stall) {
-  constexpr int kSpinLimit = 1 << 26;
-  for (int p = p0; p < p1; p += 64) {
-    const int q = p + lane;
-    int c = q < p1 ? cols[q] : -1;
-    for (int spin = 0;; spin++) {
-      const bool ready = c < 0 || c >= limit_col || done[c] != 0;
-      if (__ballot(!ready) == 0) break;
-      if (spin == kSpinLimit) {
-        if (lane == 0) *stall = 1;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-// y_L = L_LL^-1 b_L (unit lower).  Wave w solves leaf rows w, w + W, ... in
-// ascending order, each once the rows it reads are solved (LDS flags), all
-// right-hand-side columns of the matrix per row.
+// y_L = L_LL^-1 b_L (unit lower), level by level; the (row, rhs column)
+// pairs of a level are spread over the waves, one wave per pair.
 template <typename T>
 __global__ __launch_bounds__(256) void k_leaf_fwd(const LeafDev *__restrict__ leaves,
-                                                  const VecDev *__restrict__ vd, T *bp, int *stall) {
-  extern __shared__ __align__(16) char flags_raw[];
-  volatile unsigned char *done = (volatile unsigned char *)flags_raw;
+                                                  const VecDev *__restrict__ vd, T *bp) {
   const LeafDev L = leaves[blockIdx.x];
   const VecDev V = vd[L.vec];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
-  for (int r = threadIdx.x; r < L.leaf_size; r += blockDim.x) done[r] = 0;
-  __syncthreads();
   const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol;
   const T *v = (const T *)L.vals;
-  for (int i = wave; i < L.leaf_size; i += W) {
-    wait_rows(wc, rp[i], dg[i], lane, done, L.leaf_size, stall);
-    for (int col = 0; col < V.nrhs; col++) {
+  const int *lp = L.levptr, *lr = L.levrows;
+  for (int lev = 0; lev < L.nlev; lev++) {
+    const int r0 = lp[lev], nr = lp[lev + 1] - r0;
+    for (int t = wave; t < nr * V.nrhs; t += W) {
+      const int i = lr[r0 + t % nr], col = t / nr;
       T *b = bp + V.base + (long long)col * V.ld + L.gstart;
       T s = T(0);
       for (int p = rp[i] + lane; p < dg[i]; p += 64) s += v[p] * b[wc[p]];
       s = wave_sum(s);
       if (lane == 0) b[i] -= s;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    if (lane == 0) done[i] = 1;
+    __syncthreads();
   }
 }
 template <typename T>
-void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
-                     int *stall, hipStream_t st) {
+void launch_leaf_fwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, hipStream_t st) {
   if (nleaves <= 0) return;
-  hipLaunchKernelGGL(k_leaf_fwd<T>, dim3(nleaves), dim3(256), leaf_flag_bytes(max_leaf), st,
-                     leaves, vd, bp, stall);
+  hipLaunchKernelGGL(k_leaf_fwd<T>, dim3(nleaves), dim3(256), 0, st, leaves, vd, bp);
 }
 
 // Boundary rows: cbuf[ri] = sum_k L_SL(r, k) y_k (kernel_sparse.cu:935-944).
@@ -894,26 +906,23 @@ void launch_assemble_rhs(int ntarget, const int *trow_glob, const int *tptr,
                      max_rhs);
 }
 
-// x_L = U_LL^-1 (y_L - U_LS x_S) (SnuMat_factorize.cu:331-344).  Wave w
-// solves leaf rows ls-1-w, ls-1-w-W, ... in descending order, each once the
-// leaf rows its U part reads are solved (boundary columns are final).
+// x_L = U_LL^-1 (y_L - U_LS x_S), level by level (SnuMat_factorize.cu:331-344);
+// the (row, rhs column) pairs of a level are spread over the waves.
 template <typename T>
 __global__ __launch_bounds__(256) void k_leaf_bwd(const LeafDev *__restrict__ leaves,
-                                                  const VecDev *__restrict__ vd, T *bp, int *stall) {
-  extern __shared__ __align__(16) char flags_raw[];
-  volatile unsigned char *done = (volatile unsigned char *)flags_raw;
+                                                  const VecDev *__restrict__ vd, T *bp) {
   const LeafDev L = leaves[blockIdx.x];
   const VecDev V = vd[L.vec];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
   const int ls = L.leaf_size;
-  for (int r = threadIdx.x; r < ls; r += blockDim.x) done[r] = 0;
-  __syncthreads();
   const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol, *bg = L.bcol_glob;
   const T *v = (const T *)L.vals;
-  for (int i = ls - 1 - wave; i >= 0; i -= W) {
-    wait_rows(wc, dg[i] + 1, rp[i + 1], lane, done, ls, stall);
-    for (int col = 0; col < V.nrhs; col++) {
+  const int *lp = L.ulevptr, *lr = L.ulevrows;
+  for (int lev = 0; lev < L.nulev; lev++) {
+    const int r0 = lp[lev], nr = lp[lev + 1] - r0;
+    for (int t = wave; t < nr * V.nrhs; t += W) {
+      const int i = lr[r0 + t % nr], col = t / nr;
       T *b = bp + V.base + (long long)col * V.ld;
       T s = T(0);
       for (int p = dg[i] + 1 + lane; p < rp[i + 1]; p += 64) {
@@ -924,16 +933,13 @@ __global__ __launch_bounds__(256) void k_leaf_bwd(const LeafDev *__restrict__ le
       s = wave_sum(s);
       if (lane == 0) b[L.gstart + i] = (b[L.gstart + i] - s) / v[dg[i]];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    if (lane == 0) done[i] = 1;
+    __syncthreads();
   }
 }
 template <typename T>
-void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
-                     int *stall, hipStream_t st) {
+void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, hipStream_t st) {
   if (nleaves <= 0) return;
-  hipLaunchKernelGGL(k_leaf_bwd<T>, dim3(nleaves), dim3(256), leaf_flag_bytes(max_leaf), st,
-                     leaves, vd, bp, stall);
+  hipLaunchKernelGGL(k_leaf_bwd<T>, dim3(nleaves), dim3(256), 0, st, leaves, vd, bp);
 }
 
 // ---------------------------------------------------------------------------
@@ -1120,6 +1126,8 @@ int device_lds_limit() {
                                hipStream_t);                                                  \
   template void launch_gemm128<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,       \
                                   hipStream_t);                                               \
+  template void launch_gemm_variant<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,  \
+                                       int, int, hipStream_t);                                \
   template void launch_gemv<T>(const GemvProbT<T> *, int, const GemvSegT<T> *, int, int,     \
                                hipStream_t);                                                  \
   template void launch_diag_lu<T>(const DiagProbT<T> *, int, hipStream_t);                   \
@@ -1127,15 +1135,13 @@ int device_lds_limit() {
                                      hipStream_t);                                            \
   template void launch_permute_out<T>(const PermDev *, int, const VecDev *, const T *,        \
                                       long long, hipStream_t);                                \
-  template void launch_leaf_fwd<T>(const LeafDev *, int, const VecDev *, T *, int, int *,     \
-                                   hipStream_t);                                              \
+  template void launch_leaf_fwd<T>(const LeafDev *, int, const VecDev *, T *, hipStream_t);   \
   template void launch_leaf_fwd_sep<T>(const LeafDev *, const int *, const int *, int, int,   \
                                        const VecDev *, const T *, T *, int, hipStream_t);     \
   template void launch_assemble_rhs<T>(int, const int *, const int *, const int *,            \
                                        const int *, const LeafDev *, const VecDev *,          \
                                        const T *, int, T *, int, hipStream_t);                \
-  template void launch_leaf_bwd<T>(const LeafDev *, int, const VecDev *, T *, int, int *,     \
-                                   hipStream_t);
+  template void launch_leaf_bwd<T>(const LeafDev *, int, const VecDev *, T *, hipStream_t);
 SNU_INSTANTIATE(double)
 SNU_INSTANTIATE(float)
 #undef SNU_INSTANTIATE

@@ -85,6 +85,8 @@ void worker(snuserve *s) {
   unsigned long long gen = ~0ull;
   std::vector<int> ids;          // batch member order
   std::vector<snumat_t *> mats;  // batch members
+  void *dbuf = nullptr;          // device staging of packed b and x (grow-only)
+  size_t dcap = 0;
   for (;;) {
     std::vector<long long> take;
     {
@@ -102,7 +104,6 @@ void worker(snuserve *s) {
     }
     const double t0 = now_ms();
     int rc = SNU_OK;
-    std::map<int, std::vector<long long>> by_id;
     {
       std::shared_lock<std::shared_mutex> sl(s->set_mu);
       {
@@ -115,54 +116,68 @@ void worker(snuserve *s) {
           if (batch) snubatch_destroy(batch);
           batch = nullptr;
         }
-        for (long long t : take) by_id[s->reqs[t].id].push_back(t);
       }
       if (!batch && !ids.empty()) rc = snubatch_create(mats.data(), (int)mats.size(), &batch);
       if (!rc && batch) {
-        // pack: per member, the columns of its requests side by side
+        // pack: per member, the columns of its requests side by side in a
+        // device staging buffer (b, then x), copied straight from and to
+        // the requests' host arrays
         const int nm = (int)ids.size();
-        std::vector<std::vector<double>> B(nm), X(nm);
-        std::vector<const double *> pb(nm, nullptr);
-        std::vector<double *> px(nm, nullptr);
-        std::vector<int> nrhs(nm, 0), ld(nm, 1);
         std::unordered_map<int, int> slot;
         for (int m = 0; m < nm; m++) slot[ids[m]] = m;
-        std::vector<std::pair<int, int>> where(take.size(), {-1, 0});  // (member, first column)
+        struct Item { int m, c0, nrhs, ld, n; const double *b; double *x; };
+        std::vector<Item> items(take.size(), Item{-1, 0, 0, 0, 0, nullptr, nullptr});
+        std::vector<int> nrhs(nm, 0), ld(nm, 1), nn(nm, 0);
         {
           std::lock_guard<std::mutex> lk(s->mu);
-          for (auto &kv : by_id) {
-            auto it = slot.find(kv.first);
-            if (it == slot.end()) continue;  // evicted: answered below
-            const int m = it->second, n = s->res[kv.first].n;
-            int k = 0;
-            for (long long t : kv.second) k += s->reqs[t].nrhs;
-            B[m].resize((size_t)n * k);
-            X[m].resize((size_t)n * k);
-            int c = 0;
-            for (long long t : kv.second) {
-              const Request &r = s->reqs[t];
-              for (int j = 0; j < r.nrhs; j++)
-                std::memcpy(&B[m][(size_t)(c + j) * n], r.b + (size_t)j * r.ld, sizeof(double) * n);
-              const size_t pos = std::find(take.begin(), take.end(), t) - take.begin();
-              where[pos] = {m, c};
-              c += r.nrhs;
-            }
-            nrhs[m] = k;
-            ld[m] = n;
-            pb[m] = B[m].data();
-            px[m] = X[m].data();
+          for (int m = 0; m < nm; m++) nn[m] = s->res[ids[m]].n, ld[m] = nn[m];
+          for (size_t q = 0; q < take.size(); q++) {
+            const Request &r = s->reqs[take[q]];
+            auto it = slot.find(r.id);
+            if (it == slot.end()) continue;
+            const int m = it->second;
+            items[q] = Item{m, nrhs[m], r.nrhs, r.ld, nn[m], r.b, r.x};
+            nrhs[m] += r.nrhs;
           }
         }
-        rc = snubatch_solve(batch, pb.data(), px.data(), nrhs.data(), ld.data());
-        std::lock_guard<std::mutex> lk(s->mu);
-        for (size_t q = 0; q < take.size(); q++) {
-          Request &r = s->reqs[take[q]];
-          if (where[q].first < 0) continue;
-          const int m = where[q].first, c0 = where[q].second, n = s->res[ids[m]].n;
-          if (!rc || rc == SNU_ERR_NOT_CONVERGED)
-            for (int j = 0; j < r.nrhs; j++)
-              std::memcpy(r.x + (size_t)j * r.ld, &X[m][(size_t)(c0 + j) * n], sizeof(double) * n);
+        std::vector<size_t> off(nm, 0);
+        size_t tot = 0;
+        for (int m = 0; m < nm; m++) off[m] = tot, tot += (size_t)nn[m] * nrhs[m];
+        if (2 * tot * sizeof(double) > dcap) {
+          if (dbuf) snu_dev_free(dbuf);
+          dbuf = nullptr;
+          dcap = 0;
+          rc = snu_dev_alloc(2 * tot * sizeof(double) + 1024, &dbuf);
+          if (!rc) dcap = 2 * tot * sizeof(double) + 1024;
         }
+        double *db = (double *)dbuf, *dx = db + tot;
+        auto copy_cols = [&](const Item &it, bool up) -> int {
+          double *dev = (up ? db : dx) + off[it.m] + (size_t)it.c0 * it.n;
+          if (it.ld == it.n)
+            return up ? snu_dev_copy(dev, it.b, sizeof(double) * it.n * it.nrhs, 1)
+                      : snu_dev_copy(it.x, dev, sizeof(double) * it.n * it.nrhs, 2);
+          for (int j = 0; j < it.nrhs; j++) {
+            int e = up ? snu_dev_copy(dev + (size_t)j * it.n, it.b + (size_t)j * it.ld, sizeof(double) * it.n, 1)
+                       : snu_dev_copy(it.x + (size_t)j * it.ld, dev + (size_t)j * it.n, sizeof(double) * it.n, 2);
+            if (e) return e;
+          }
+          return SNU_OK;
+        };
+        for (size_t q = 0; q < items.size() && !rc; q++)
+          if (items[q].m >= 0) rc = copy_cols(items[q], true);
+        if (!rc) {
+          std::vector<const double *> pb(nm, nullptr);
+          std::vector<double *> px(nm, nullptr);
+          for (int m = 0; m < nm; m++)
+            if (nrhs[m]) pb[m] = db + off[m], px[m] = dx + off[m];
+          rc = snubatch_solve_device(batch, pb.data(), px.data(), nrhs.data(), ld.data());
+        }
+        if (!rc || rc == SNU_ERR_NOT_CONVERGED)
+          for (size_t q = 0; q < items.size(); q++)
+            if (items[q].m >= 0) {
+              int e = copy_cols(items[q], false);
+              if (e) rc = e;
+            }
       }
     }
     const double t1 = now_ms();
@@ -190,6 +205,7 @@ void worker(snuserve *s) {
     s->cv_done.notify_all();
   }
   if (batch) snubatch_destroy(batch);
+  if (dbuf) snu_dev_free(dbuf);
 }
 
 }  // namespace
