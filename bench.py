@@ -40,6 +40,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "factor time & solve/s at 1e-12 backward error, FP64 vs FP32+IR; p99 latency"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_BYTES = 8.0e12
 
 
 def main():
@@ -139,7 +140,7 @@ def main():
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-        "traffic": None,
+        "traffic": profiled_traffic(),
         "kernel": "snu::k_gemm<double, 0, 2, 2> (grouped FP64 MFMA 4x4x4, C -= sum A B)",
         "launches": int(gemm_launches),
         "avg_launch_ms": round(gemm_ms / max(gemm_launches, 1), 4),
@@ -176,6 +177,13 @@ def main():
         "phases_ms": {k: round(tim[k], 3) for k in
                       ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms"]},
         "factor_gflops": round(factor_flops / (tim["factor_ms"] * 1e6), 2),
+        # whole-phase fractions of roofline: algorithmic factor flops (dense
+        # nominal, zero tiles included although they are skipped, + leaf
+        # structural) over the FP64 MFMA peak, and the solve's compulsory
+        # bytes (factors + vectors) over HBM bandwidth
+        "factor_nominal_frac_of_fp64_peak": round(factor_flops / (tim["factor_ms"] * 1e-3) / (PEAK_FP64_TFLOPS * 1e12), 4),
+        "solve_hbm_gbs": round(counts["b_solve"] / (tim["solve_ms"] * 1e6), 1),
+        "solve_frac_of_hbm": round(counts["b_solve"] / (tim["solve_ms"] * 1e-3) / PEAK_HBM_BYTES, 4),
         "factor_flops_algorithmic": {"dense": counts["f_dense"], "leaf": counts["f_leaf"]},
         "solves_per_s_nrhs1": round(solves1, 2),
         "solves_per_s_nrhs16": round(solves16, 2),
@@ -198,6 +206,19 @@ def main():
         except Exception as e:  # report the failure of a side leg, keep the main line
             line[key] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(line), flush=True)
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the dominant GEMM on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the counters'
+    reading for a copy kernel of known size); None if not recorded."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_gemm_64cube_7pt_fp64.md")
+    try:
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def precision_leg(S, np, reps=3):

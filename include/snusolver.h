@@ -148,6 +148,9 @@ int snubatch_timings(const snubatch_t *b, double *t16);
 /* number of dense factor launches and dense solve launches of the batch */
 int snubatch_launches(const snubatch_t *b, int *factor_launches,
                       int *solve_launches);
+/* Replay the factorization's launch sequence from a HIP graph captured on
+ * the next factorize (launch overhead only; same kernels and results). */
+int snubatch_use_graph(snubatch_t *b, int on);
 int snubatch_destroy(snubatch_t *b);
 
 /* ---------------- serving (no reference counterpart) ----------------
@@ -266,6 +269,10 @@ int snu_dev_sync(void);
 /* Time every dense GEMM launch of the next factorizations with HIP events on
  * the handle's stream (timings t[7]); adds event records between launches. */
 int snumat_set_profiling(snumat_t *h, int on);
+/* Factorize by replaying a HIP graph of the launch sequence, captured on the
+ * next factorize (no reference counterpart; results unchanged; phase timings
+ * then report the total only). */
+int snumat_use_graph(snumat_t *h, int on);
 
 /* ---------------- kernel-level test hooks (host pointers) ----------------
  * Not part of the reference interface: they expose single kernels of this
@@ -278,7 +285,8 @@ int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,
                       const float *B, int ldb, float *C, int ldc, int assign);
 /* average time (ms) of the grouped C -= A B kernel on device-resident
  * operands, K split into nseg concatenated segments; tile 64 or 128 (K chunk
- * 16) or 6432 / 12832 (K chunk 32); prec SNU_FP64 or SNU_FP32. */
+ * 16), 6432 / 12832 (K chunk 32) or 642 / 1282 (K chunk 16, loads two
+ * chunks ahead); prec SNU_FP64 or SNU_FP32. */
 int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                    double *ms_out);
 /* tiles3: 64 or 128, the tile of the grouped C -= A B kernel per size bucket

@@ -290,6 +290,12 @@ class SnuMat:
     def set_profiling(self, on: bool):
         _check(lib().snumat_set_profiling(self._h, 1 if on else 0))
 
+    def use_graph(self, on: bool = True):
+        """Factorize by replaying a HIP graph of the launch sequence (captured
+        on the next factorize)."""
+        _check(lib().snumat_use_graph(self._h, 1 if on else 0))
+        return self
+
     # -- introspection --
     def info(self) -> dict:
         a = (C.c_int64 * 16)()
@@ -388,6 +394,10 @@ class Batch:
 
     def factorize(self):
         _check(lib().snubatch_factorize(self._h))
+        return self
+
+    def use_graph(self, on: bool = True):
+        _check(lib().snubatch_use_graph(self._h, 1 if on else 0))
         return self
 
     def solve(self, bs):

@@ -237,7 +237,7 @@ int factorize_values(snumat *h, int stage) {
   static const DensePlan<T> empty;
   double gms[2] = {0, 0};
   int rc = run_factor<T>(fs, stage >= 2 ? VV.fact : empty, h->stream, &h->ev, h->profile && stage >= 2,
-                         &h->pev, gms);
+                         &h->pev, gms, h->use_graph && stage >= 2 ? &h->graph : nullptr);
   if (rc) return rc;
   HIPCHK(hipEventSynchronize(h->ev[5]));
   if (h->prec == SNU_FP32) {
@@ -635,6 +635,11 @@ extern "C" int snu_dev_sync(void) {
   HIPCHK(hipDeviceSynchronize());
   return SNU_OK;
 }
+extern "C" int snumat_use_graph(snumat_t *h, int on) {
+  if (!h) return set_error(SNU_ERR_BAD_INPUT, "null");
+  h->use_graph = on != 0;
+  return SNU_OK;
+}
 extern "C" int snumat_set_profiling(snumat_t *h, int on) {
   if (!h) return set_error(SNU_ERR_BAD_INPUT, "null");
   h->profile = on != 0;
@@ -647,7 +652,7 @@ extern "C" int snumat_set_profiling(snumat_t *h, int on) {
 // time over reps launches (HIP events).
 namespace {
 template <typename T>
-int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int reps, double *ms_out) {
+int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int reps, double *ms_out) {
   T *dA, *dB, *dC;
   HIPCHK(hipMalloc((void **)&dA, sizeof(T) * (size_t)M * K));
   HIPCHK(hipMalloc((void **)&dB, sizeof(T) * (size_t)K * N));
@@ -673,7 +678,7 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int reps, double
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  auto run = [&]() { launch_gemm_variant<T>(dgp, 1, dgs, tiles, tile, kb, nullptr); };
+  auto run = [&]() { launch_gemm_variant<T>(dgp, 1, dgs, tiles, tile, kb, depth, nullptr); };
   run();
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipEventRecord(e0, nullptr));
@@ -695,13 +700,16 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int reps, double
 }  // namespace
 extern "C" int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                               double *ms_out) {
-  // tile: 64 or 128 (K chunk 16), or 6432 / 12832 (K chunk 32)
-  const int kb = tile > 1000 ? 32 : 16, t = tile > 1000 ? tile / 100 : tile;
+  // tile: 64 or 128 (K chunk 16), 6432 / 12832 (K chunk 32), 642 / 1282
+  // (K chunk 16, loads two chunks ahead)
+  int t = tile, kb = 16, depth = 1;
+  if (tile == 6432 || tile == 12832) t = tile / 100, kb = 32;
+  if (tile == 642 || tile == 1282) t = tile / 10, depth = 2;
   if (M < 1 || N < 1 || K < 1 || nseg < 1 || reps < 1 || (t != 64 && t != 128) ||
       (prec != SNU_FP64 && prec != SNU_FP32) || !ms_out)
     return set_error(SNU_ERR_BAD_INPUT, "bad gemm bench shape");
-  return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, t, kb, reps, ms_out)
-                          : bench_gemm<double>(M, N, K, nseg, t, kb, reps, ms_out);
+  return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, t, kb, depth, reps, ms_out)
+                          : bench_gemm<double>(M, N, K, nseg, t, kb, depth, reps, ms_out);
 }
 
 // Average time (ms) of a plain device copy of `bytes` (multiple of 16) over

@@ -55,6 +55,8 @@ struct snubatch {
   double *stage = nullptr;  // host-pointer solves: packed b then x
   size_t stage_cap = 0;
   bool factor_plan = false;  // merged dense plan built (on first factorize)
+  bool use_graph = false;
+  FactorGraph graph;
   std::vector<hipEvent_t> ev;
   double t[16] = {0};
 };
@@ -85,6 +87,7 @@ void bfree(snubatch *b, void *p) {
   (void)hipFree(p);
 }
 void free_batch(snubatch *b) {
+  free_graph(b->graph);
   free_dense_plan(b->f64);
   free_dense_plan(b->f32);
   free_gemv_plan(b->s64);
@@ -183,7 +186,8 @@ int batch_factorize(snubatch *b) {
   fs.eps = b->mats[0]->opts.eps;
   const DensePlan<T> &plan = std::is_same<T, double>::value ? (const DensePlan<T> &)b->f64
                                                             : (const DensePlan<T> &)b->f32;
-  int rc = run_factor<T>(fs, plan, b->stream, &b->ev, false, nullptr, nullptr);
+  int rc = run_factor<T>(fs, plan, b->stream, &b->ev, false, nullptr, nullptr,
+                         b->use_graph ? &b->graph : nullptr);
   if (rc) return rc;
   HIPCHK(hipEventSynchronize(b->ev[5]));
   for (int q = 0; q < 5; q++) b->t[q] = ev_ms(b->ev[q], b->ev[q + 1]);
@@ -324,10 +328,12 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   std::vector<LeafDev> leaves;
   WorkLists W;
   W.tptr.assign(1, 0);
+  W.tile_ptr.assign(1, 0);
   int wsz = 0, max_leaf = 0;
   for (snumat *h : b->mats) wsz = std::max(wsz, h->wd.wsz), max_leaf = std::max(max_leaf, h->wd.max_leaf);
   const int vsize = b->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
   const int waves = waves_for(wsz, vsize, max_leaf);
+  const int sep_wsz = wsz, sep_waves = waves_for(sep_wsz, vsize, 0);
   if (waves < 1) return fail(set_error(SNU_ERR_OOM, "leaf workspace exceeds the LDS of one workgroup"));
   int cb = 0;
   std::vector<PrepDev> prep;
@@ -342,7 +348,7 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
     }
     for (int r = 0; r < h->P.np; r++) {
       const int nbr = h->P.leaves[r].nbr;
-      for (int f = 0; f < nbr; f += waves) W.wg_leaf.push_back(lbase + r), W.wg_first.push_back(f);
+      for (int f = 0; f < nbr; f += sep_waves) W.wg_leaf.push_back(lbase + r), W.wg_first.push_back(f);
       for (int f = 0; f < nbr; f += 4) W.fwg_leaf.push_back(lbase + r), W.fwg_first.push_back(f);
     }
     const WorkLists &w = h->wl;
@@ -352,6 +358,12 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
     for (size_t t = 1; t < w.tptr.size(); t++) W.tptr.push_back(sbase + w.tptr[t]);
     for (int s : w.tsrc_leaf) W.tsrc_leaf.push_back(lbase + s);
     W.tsrc_ri.insert(W.tsrc_ri.end(), w.tsrc_ri.begin(), w.tsrc_ri.end());
+    const int gbase = (int)W.seg_leaf.size();
+    W.tile_blk.insert(W.tile_blk.end(), w.tile_blk.begin(), w.tile_blk.end());
+    W.tile_row0.insert(W.tile_row0.end(), w.tile_row0.begin(), w.tile_row0.end());
+    for (size_t t = 1; t < w.tile_ptr.size(); t++) W.tile_ptr.push_back(gbase + w.tile_ptr[t]);
+    for (int s : w.seg_leaf) W.seg_leaf.push_back(lbase + s);
+    W.seg_ri.insert(W.seg_ri.end(), w.seg_ri.begin(), w.seg_ri.end());
     cb += h->wd.cb_total;
     PrepDev pd;
     pd.arena = b->prec == SNU_FP32 ? (void *)h->v32.arena : (void *)h->v64.arena;
@@ -378,9 +390,12 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   wd.nwg = (int)W.wg_leaf.size();
   wd.nfwg = (int)W.fwg_leaf.size();
   wd.ntarget = (int)W.trow_glob.size();
+  wd.ntile = (int)W.tile_blk.size();
   wd.cb_total = cb;
   wd.wsz = wsz;
   wd.waves = waves;
+  wd.sep_wsz = sep_wsz;
+  wd.sep_waves = sep_waves;
   wd.max_leaf = max_leaf;
   int rc;
   if ((rc = bupload(b, &wd.leaves, leaves)) || (rc = bupload(b, &wd.wg_leaf, W.wg_leaf)) ||
@@ -389,6 +404,9 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
       (rc = bupload(b, &wd.trow_loc, W.trow_loc)) || (rc = bupload(b, &wd.trow_glob, W.trow_glob)) ||
       (rc = bupload(b, &wd.tptr, W.tptr)) || (rc = bupload(b, &wd.tsrc_leaf, W.tsrc_leaf)) ||
       (rc = bupload(b, &wd.tsrc_ri, W.tsrc_ri)) || (rc = bupload(b, &b->d_prep, prep)) ||
+      (rc = bupload(b, &wd.tile_blk, W.tile_blk)) || (rc = bupload(b, &wd.tile_row0, W.tile_row0)) ||
+      (rc = bupload(b, &wd.tile_ptr, W.tile_ptr)) || (rc = bupload(b, &wd.seg_leaf, W.seg_leaf)) ||
+      (rc = bupload(b, &wd.seg_ri, W.seg_ri)) ||
       (rc = bupload(b, &b->d_ir_a, ira)))
     return fail(rc);
   if ((rc = balloc(b, (void **)&wd.stall, sizeof(int))) ||
@@ -495,6 +513,12 @@ extern "C" int snubatch_launches(const snubatch_t *b, int *factor_launches, int 
   if (solve_launches)
     *solve_launches = (int)(b->prec == SNU_FP32 ? b->s32.fwd.size() + b->s32.bwd.size()
                                                 : b->s64.fwd.size() + b->s64.bwd.size());
+  return SNU_OK;
+}
+
+extern "C" int snubatch_use_graph(snubatch_t *b, int on) {
+  if (!b) return set_error(SNU_ERR_BAD_INPUT, "null");
+  b->use_graph = on != 0;
   return SNU_OK;
 }
 

@@ -74,6 +74,7 @@ struct WorkLists {
   std::vector<int> wg_leaf, wg_first;    // leaf separator-row workgroups
   std::vector<int> fwg_leaf, fwg_first;  // forward boundary-row workgroups
   std::vector<int> trow_blk, trow_loc, trow_glob, tptr, tsrc_leaf, tsrc_ri;  // targets
+  std::vector<int> tile_blk, tile_row0, tile_ptr, seg_leaf, seg_ri;          // assembly tiles
 };
 
 // Device copies of the work lists of a launch set (one matrix or a batch).
@@ -84,10 +85,19 @@ struct WorkDev {
   int *fwg_leaf = nullptr, *fwg_first = nullptr, nfwg = 0;
   int *trow_blk = nullptr, *trow_loc = nullptr, *trow_glob = nullptr, *tptr = nullptr;
   int *tsrc_leaf = nullptr, *tsrc_ri = nullptr, ntarget = 0;
+  int *tile_blk = nullptr, *tile_row0 = nullptr, *tile_ptr = nullptr;
+  int *seg_leaf = nullptr, *seg_ri = nullptr, ntile = 0;
   int cb_total = 0;
-  int wsz = 0, waves = 1;
+  int wsz = 0, waves = 1;          // leaf rows (workspace: leaf + boundary columns)
+  int sep_wsz = 0, sep_waves = 1;  // separator rows (same workspace, no flags)
   int max_leaf = 0;  // rows of the largest leaf (completion flags in LDS)
   int *stall = nullptr;  // leaf-row wait gave up (device flag)
+};
+
+// A factorization's launch sequence captured once into a HIP graph and
+// replayed (no memset nodes: buffers are cleared by kernels).
+struct FactorGraph {
+  hipGraphExec_t exec = nullptr;
 };
 
 }  // namespace snu
@@ -134,6 +144,8 @@ struct snumat {
   double ir_berr = 0;
   // timing
   bool profile = false;
+  bool use_graph = false;
+  snu::FactorGraph graph;
   std::vector<hipEvent_t> ev, pev;
   double t[16] = {0};
 };
@@ -208,10 +220,13 @@ struct FactorSet {
   const WorkDev *wd;
   double eps;
 };
+// graph (optional, not with profile): capture the launches on the first
+// call, replay them afterwards; the phase events then all mark the end.
 template <typename T>
 int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
                const std::vector<hipEvent_t> *ev, bool profile, std::vector<hipEvent_t> *pev,
-               double *gemm_ms);
+               double *gemm_ms, FactorGraph *graph = nullptr);
+void free_graph(FactorGraph &g);
 struct SolveSet {
   const PermDev *perm;
   const VecDev *vec;

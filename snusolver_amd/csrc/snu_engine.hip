@@ -807,6 +807,8 @@ int upload(snumat *h) {
   for (int r = 0; r < np; r++) h->wd.max_leaf = std::max(h->wd.max_leaf, P.leaves[r].leaf_size);
   const int vsize = h->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
   h->wd.waves = waves_for(h->wd.wsz, vsize, h->wd.max_leaf);
+  h->wd.sep_wsz = h->wd.wsz;
+  h->wd.sep_waves = waves_for(h->wd.sep_wsz, vsize, 0);
   if (h->wd.waves < 1)
     return set_error(SNU_ERR_OOM, "leaf workspace (" + std::to_string(maxw) +
                                       " columns) exceeds the LDS of one workgroup");
@@ -816,7 +818,7 @@ int upload(snumat *h) {
   WorkLists &W = h->wl;
   W = WorkLists();
   for (int r = 0; r < np; r++)
-    for (int f = 0; f < P.leaves[r].nbr; f += h->wd.waves) W.wg_leaf.push_back(r), W.wg_first.push_back(f);
+    for (int f = 0; f < P.leaves[r].nbr; f += h->wd.sep_waves) W.wg_leaf.push_back(r), W.wg_first.push_back(f);
   for (int r = 0; r < np; r++)
     for (int f = 0; f < P.leaves[r].nbr; f += 4) W.fwg_leaf.push_back(r), W.fwg_first.push_back(f);
   struct Src { int R, leaf, ri; };
@@ -847,6 +849,37 @@ int upload(snumat *h) {
     W.tsrc_ri.push_back(src[s].ri);
   }
   if (!src.empty()) W.tptr.push_back((int)src.size());
+  // assembly tiles: 64 consecutive rows of one separator block; per tile the
+  // contributing leaves in ascending rank, each with the boundary row (ri)
+  // feeding every row of the tile (-1: none)
+  W.tile_blk.clear();
+  W.tile_row0.clear();
+  W.tile_ptr.assign(1, 0);
+  W.seg_leaf.clear();
+  W.seg_ri.clear();
+  {
+    size_t s0 = 0;
+    while (s0 < src.size()) {
+      const int b = blk_of[src[s0].R];
+      const int row0 = ((src[s0].R - P.block_start[b]) / 64) * 64;
+      const int R0 = P.block_start[b] + row0, R1 = std::min(R0 + 64, P.block_start[b] + P.block_size[b]);
+      size_t s1 = s0;
+      while (s1 < src.size() && src[s1].R < R1) s1++;
+      std::vector<Src> part(src.begin() + s0, src.begin() + s1);
+      std::stable_sort(part.begin(), part.end(), [](const Src &a, const Src &c) { return a.leaf < c.leaf; });
+      for (size_t q = 0; q < part.size(); q++) {
+        if (q == 0 || part[q].leaf != part[q - 1].leaf) {
+          W.seg_leaf.push_back(part[q].leaf);
+          W.seg_ri.insert(W.seg_ri.end(), 64, -1);
+        }
+        W.seg_ri[W.seg_ri.size() - 64 + (part[q].R - R0)] = part[q].ri;
+      }
+      W.tile_blk.push_back(b);
+      W.tile_row0.push_back(row0);
+      W.tile_ptr.push_back((int)W.seg_leaf.size());
+      s0 = s1;
+    }
+  }
   // ---- dense tables ----
   std::vector<int> aoff((P.num_block + 1) * P.nk, 0);
   for (int p = 1; p <= P.num_block; p++)
@@ -877,11 +910,15 @@ int upload(snumat *h) {
   wd.nwg = (int)W.wg_leaf.size();
   wd.nfwg = (int)W.fwg_leaf.size();
   wd.ntarget = (int)W.trow_glob.size();
+  wd.ntile = (int)W.tile_blk.size();
   if ((rc = dupload(h, &wd.wg_leaf, W.wg_leaf)) || (rc = dupload(h, &wd.wg_first, W.wg_first)) ||
       (rc = dupload(h, &wd.fwg_leaf, W.fwg_leaf)) || (rc = dupload(h, &wd.fwg_first, W.fwg_first)) ||
       (rc = dupload(h, &wd.trow_blk, W.trow_blk)) || (rc = dupload(h, &wd.trow_loc, W.trow_loc)) ||
       (rc = dupload(h, &wd.trow_glob, W.trow_glob)) || (rc = dupload(h, &wd.tptr, W.tptr)) ||
-      (rc = dupload(h, &wd.tsrc_leaf, W.tsrc_leaf)) || (rc = dupload(h, &wd.tsrc_ri, W.tsrc_ri)))
+      (rc = dupload(h, &wd.tsrc_leaf, W.tsrc_leaf)) || (rc = dupload(h, &wd.tsrc_ri, W.tsrc_ri)) ||
+      (rc = dupload(h, &wd.tile_blk, W.tile_blk)) || (rc = dupload(h, &wd.tile_row0, W.tile_row0)) ||
+      (rc = dupload(h, &wd.tile_ptr, W.tile_ptr)) || (rc = dupload(h, &wd.seg_leaf, W.seg_leaf)) ||
+      (rc = dupload(h, &wd.seg_ri, W.seg_ri)))
     return rc;
   long long *d_offR, *d_offL;
   int *d_bsize, *d_anc_sum, *d_aoff;
@@ -965,7 +1002,13 @@ size_t estimate_device_bytes(const snumat *h) {
   return (size_t)bytes;
 }
 
+void free_graph(FactorGraph &g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g.exec = nullptr;
+}
+
 void free_device(snumat *h) {
+  free_graph(h->graph);
   free_gemv_plan(h->v64.sp);
   free_gemv_plan(h->v32.sp);
   free_dense_plan(h->v64.fact);
@@ -1003,10 +1046,10 @@ void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
 // ev (optional): 6 events marking prepare / leaf rows / leaf sep / assembly /
 // dense; pev (optional, with profile): a begin/end pair per GEMM launch,
 // their times summed into gemm_ms[0] (C -= AB) and gemm_ms[1] (C = +-AB).
+// The launches of one factorization on st (no host synchronisation).
 template <typename T>
-int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
-               const std::vector<hipEvent_t> *ev, bool profile, std::vector<hipEvent_t> *pev,
-               double *gemm_ms) {
+int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
+                   const std::vector<hipEvent_t> *ev, bool profile, std::vector<hipEvent_t> *pev) {
   const WorkDev &wd = *fs.wd;
   if (ev) HIPCHK(hipEventRecord((*ev)[0], st));
   launch_prepare<T>(fs.prep, fs.nmat, fs.max_len, fs.max_nnz, st);
@@ -1016,11 +1059,11 @@ int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
   launch_leaf_rows<T>(wd.leaves, wd.nleaves, wd.wsz, wd.waves, wd.max_leaf, fs.eps, wd.stall, st);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[2], st));
-  launch_leaf_sep<T>(wd.leaves, wd.wg_leaf, wd.wg_first, wd.nwg, wd.wsz, wd.waves, st);
+  launch_leaf_sep<T>(wd.leaves, wd.wg_leaf, wd.wg_first, wd.nwg, wd.sep_wsz, wd.sep_waves, st);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[3], st));
-  launch_assemble_schur<T>(wd.ntarget, wd.trow_blk, wd.trow_loc, wd.tptr, wd.tsrc_leaf,
-                           wd.tsrc_ri, wd.leaves, st);
+  launch_assemble_schur<T>(wd.ntile, wd.tile_blk, wd.tile_row0, wd.tile_ptr, wd.seg_leaf, wd.seg_ri,
+                           wd.leaves, st);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[4], st));
   int ngemm = 0;
@@ -1042,6 +1085,42 @@ int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
   }
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[5], st));
+  return SNU_OK;
+}
+
+template <typename T>
+int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
+               const std::vector<hipEvent_t> *ev, bool profile, std::vector<hipEvent_t> *pev,
+               double *gemm_ms, FactorGraph *graph) {
+  const WorkDev &wd = *fs.wd;
+  int rc;
+  if (graph && !profile) {
+    // replay the captured launch sequence (captured on first use)
+    if (!graph->exec) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_factor<T>(fs, plan, st, nullptr, false, nullptr);
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return set_error(SNU_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&graph->exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        graph->exec = nullptr;
+        return set_error(SNU_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+      }
+    }
+    if (ev) HIPCHK(hipEventRecord((*ev)[0], st));
+    HIPCHK(hipGraphLaunch(graph->exec, st));
+    if (ev) {
+      for (int q = 1; q <= 5; q++) HIPCHK(hipEventRecord((*ev)[q], st));
+    }
+  } else if ((rc = enqueue_factor<T>(fs, plan, st, ev, profile, pev))) {
+    return rc;
+  }
   int stalled = 0;
   HIPCHK(hipMemcpyAsync(&stalled, wd.stall, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -1049,7 +1128,7 @@ int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
   if (profile && pev && gemm_ms) {
     HIPCHK(hipStreamSynchronize(st));
     gemm_ms[0] = gemm_ms[1] = 0;
-    g = 0;
+    int g = 0;
     for (const Launch &L : plan.launches) {
       if (!is_gemm(L.kind)) continue;
       gemm_ms[(L.kind == 1 || L.kind == 6) ? 1 : 0] += ev_ms((*pev)[2 * g], (*pev)[2 * g + 1]);
@@ -1144,7 +1223,7 @@ int run_refined(const RefineSet &rs, float *bp, float *cbuf, const GemvPlan<floa
   template void merge_gemv<T>(const std::vector<const GemvPlan<T> *> &, GemvPlan<T> &);       \
   template int run_factor<T>(const FactorSet &, const DensePlan<T> &, hipStream_t,            \
                              const std::vector<hipEvent_t> *, bool, std::vector<hipEvent_t> *, \
-                             double *);                                                       \
+                             double *, FactorGraph *);                                        \
   template void run_sweep<T>(const SolveSet &, T *, T *, const GemvPlan<T> &, hipStream_t);
 SNU_ENGINE_INST(double)
 SNU_ENGINE_INST(float)

@@ -133,12 +133,15 @@ int leaf_flag_bytes(int max_leaf);
 void launch_zero_i32(int *p, int n, hipStream_t st);
 // dst = src, bytes a multiple of 16
 void launch_copy(const void *src, void *dst, long long bytes, hipStream_t st);
+// separator rows: L_SL row and Schur row per wave (workspace: leaf and
+// boundary columns)
 template <typename T>
 void launch_leaf_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
                      int wsz, int waves, hipStream_t st);
+// boundary Schur rows into the dense blocks, per tile of 64 separator rows
 template <typename T>
-void launch_assemble_schur(int ntarget, const int *trow_blk, const int *trow_loc,
-                           const int *tptr, const int *tsrc_leaf, const int *tsrc_ri,
+void launch_assemble_schur(int ntile, const int *tile_blk, const int *tile_row0,
+                           const int *tile_ptr, const int *seg_leaf, const int *seg_ri,
                            const LeafDev *leaves, hipStream_t st);
 // mode 0: C -= sum A B; 1: C = A B; 2: C = -A B
 template <typename T>
@@ -149,7 +152,7 @@ void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *seg
                     int total_tiles, hipStream_t st);
 template <typename T>
 void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
-                         int total_tiles, int tile, int kb, hipStream_t st);
+                         int total_tiles, int tile, int kb, int depth, hipStream_t st);
 // mode 0: y -= A x; 1: y = A x
 template <typename T>
 void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,

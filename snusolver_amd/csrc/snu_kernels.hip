@@ -76,7 +76,8 @@ void launch_fill(T *p, long long n, hipStream_t st) {
 // indexed by workspace column (leaf columns, then the leaf's boundary
 // columns), all zero on entry and on exit.
 //   LEAF_ROW: L part ends at diag[i]; the diagonal is clamped to +-eps
-//             (kernel_sparse.cu:742-749).
+//             (kernel_sparse.cu:742-749); pivots update all their U entries
+//             (leaf and boundary columns).
 //   otherwise a separator row: every entry is an L entry, and the updates
 //             that land on boundary columns form the row's Schur
 //             contribution -sum_k L(r,k) U(k,c) (kernel_sparse.cu:953-971).
@@ -239,7 +240,9 @@ void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, in
 }
 
 // Separator rows of the leaves: each wave factors one boundary row (its L_SL
-// row) and writes its Schur contribution row into S.
+// row) and writes its Schur contribution row into S.  (Splitting the two
+// halves into separate passes with smaller workspaces measured slower: the
+// cost is per pivot step, and the pass over the pivots is then made twice.)
 template <typename T>
 __global__ __launch_bounds__(512) void k_leaf_sep(const LeafDev *__restrict__ leaves,
                                                   const int *__restrict__ wg_leaf,
@@ -253,8 +256,8 @@ __global__ __launch_bounds__(512) void k_leaf_sep(const LeafDev *__restrict__ le
   const int ls = L.leaf_size;
   for (int c = lane; c < ls + L.nbc; c += 64) w[c] = T(0);
   wave_lds_fence();
-  const int r = L.brow[ri];
-  factor_row<T, false>(r, L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0), nullptr, nullptr);
+  factor_row<T, false>(L.brow[ri], L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0), nullptr,
+                       nullptr);
   T *Srow = (T *)L.S + (long long)ri * L.nbc;
   for (int j = lane; j < L.nbc; j += 64) {
     Srow[j] = w[ls + j];
@@ -286,42 +289,58 @@ __device__ __forceinline__ long long sep_index(const DenseDev &d, int bi,
   return d.offL[bj] + off + lr + (long long)lc * d.anc_sum[bj];
 }
 
-// One wave per target separator row; contributions are added leaf by leaf
-// in ascending rank order (deterministic; no atomics).
-// A target row and all its sources belong to one matrix, whose separator
-// tables and dense arena the sources' leaf descriptors carry.
+// One workgroup per assembly tile: 64 consecutive rows of one separator
+// block.  For each contributing leaf (ascending rank) and each chunk of 64 of
+// its boundary columns, the S block is staged through LDS (rows read along
+// S's rows) and added into the dense block down its columns, where the 64
+// rows of the tile are contiguous: both sides coalesced.  Every element
+// receives its leaves' contributions in ascending rank order from one thread
+// (deterministic; no atomics); zero contributions are skipped.
 template <typename T>
 __global__ __launch_bounds__(256) void k_assemble_schur(
-    int ntarget, const int *__restrict__ trow_blk,
-    const int *__restrict__ trow_loc, const int *__restrict__ tptr,
-    const int *__restrict__ tsrc_leaf, const int *__restrict__ tsrc_ri,
-    const LeafDev *__restrict__ leaves) {
-  const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (t >= ntarget) return;
-  const int bi = trow_blk[t], lr = trow_loc[t];
-  for (int s = tptr[t]; s < tptr[t + 1]; s++) {
-    const LeafDev L = leaves[tsrc_leaf[s]];
+    int ntile, const int *__restrict__ tile_blk, const int *__restrict__ tile_row0,
+    const int *__restrict__ tile_ptr, const int *__restrict__ seg_leaf,
+    const int *__restrict__ seg_ri, const LeafDev *__restrict__ leaves) {
+  __shared__ T st[64][65];
+  const int tile = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int bi = tile_blk[tile], row0 = tile_row0[tile];
+  for (int sg = tile_ptr[tile]; sg < tile_ptr[tile + 1]; sg++) {
+    const LeafDev L = leaves[seg_leaf[sg]];
     const DenseDev dd = *L.dd;
     T *dense = (T *)L.dense;
-    const T *Srow = (const T *)L.S + (long long)tsrc_ri[s] * L.nbc;
-    for (int j = lane; j < L.nbc; j += 64) {
-      T v = Srow[j];
-      if (v != T(0)) {
-        long long idx = sep_index(dd, bi, lr, L.bcol_blk[j], L.bcol_loc[j]);
-        dense[idx] += v;
+    const T *S = (const T *)L.S;
+    const int *ri = seg_ri + (long long)sg * 64;
+    for (int c0 = 0; c0 < L.nbc; c0 += 64) {
+      // stage: wave w reads tile rows w, w + 4, ... (64 columns each)
+      for (int rr = wave; rr < 64; rr += 4) {
+        const int r = ri[rr];
+        T v = T(0);
+        if (r >= 0 && c0 + lane < L.nbc) v = S[(long long)r * L.nbc + c0 + lane];
+        st[rr][lane] = v;
       }
+      __syncthreads();
+      // add: wave w takes columns w, w + 4, ...; lane = tile row
+      const int nc = min(64, L.nbc - c0);
+      for (int cc = wave; cc < nc; cc += 4) {
+        const T v = st[lane][cc];
+        if (v != T(0)) {
+          const long long idx =
+              sep_index(dd, bi, row0 + lane, L.bcol_blk[c0 + cc], L.bcol_loc[c0 + cc]);
+          dense[idx] += v;
+        }
+      }
+      __syncthreads();
     }
-    __threadfence();
   }
 }
 template <typename T>
-void launch_assemble_schur(int ntarget, const int *trow_blk, const int *trow_loc,
-                           const int *tptr, const int *tsrc_leaf, const int *tsrc_ri,
+void launch_assemble_schur(int ntile, const int *tile_blk, const int *tile_row0,
+                           const int *tile_ptr, const int *seg_leaf, const int *seg_ri,
                            const LeafDev *leaves, hipStream_t st) {
-  if (ntarget <= 0) return;
-  hipLaunchKernelGGL(k_assemble_schur<T>, dim3((ntarget + 3) / 4), dim3(256), 0, st, ntarget,
-                     trow_blk, trow_loc, tptr, tsrc_leaf, tsrc_ri, leaves);
+  if (ntile <= 0) return;
+  hipLaunchKernelGGL(k_assemble_schur<T>, dim3(ntile), dim3(256), 0, st, ntile, tile_blk,
+                     tile_row0, tile_ptr, seg_leaf, seg_ri, leaves);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,7 +383,10 @@ __device__ __forceinline__ double dpp_d(double v) {
   return __hiloint2double(hi, lo);
 }
 typedef float f4 __attribute__((ext_vector_type(4)));
-template <typename T, int MODE, int BMB, int BNB, int KB>
+// DEPTH: chunks of A and B in flight in registers ahead of the chunk being
+// multiplied (1: the next chunk's loads overlap one MFMA block; 2: loads are
+// issued two chunks ahead, alternating two register stages).
+template <typename T, int MODE, int BMB, int BNB, int KB, int DEPTH = 1>
 __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ probs,
                                               int nprob,
                                               const GemmSegT<T> *__restrict__ segs) {
@@ -405,24 +427,44 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
       for (int r = 0; r < 4; r++) acc[a][b][r] = 0.0;
       accf[a][b] = f4{0.f, 0.f, 0.f, 0.f};
     }
-  T ra[NA], rb[NB];
+  T ra[NA], rb[NB], ra2[NA], rb2[NB];
   // Loads are unconditional and unmasked with addresses clamped into the
   // operands, so they stay branch-free and in flight across the MFMA block:
   // a clamped row of A (column of B) only feeds a row (column) of the tile
   // that is never stored; the K tail is zeroed in LDS after the store.
-  auto load = [&](int sgi, int kk0) {
+  auto load_into = [&](int sgi, int kk0, T (&xa)[NA], T (&xb)[NB]) {
     const GemmSegT<T> S = segs[P.seg0 + sgi];
 #pragma unroll
     for (int i = 0; i < NA; i++) {
       int e = tid + 256 * i;
       int gm = min(m0 + e % BM, P.M - 1), gk = min(kk0 + e / BM, S.K - 1);
-      ra[i] = gptr(S.A)[gm + (long long)gk * S.lda];
+      xa[i] = gptr(S.A)[gm + (long long)gk * S.lda];
     }
 #pragma unroll
     for (int i = 0; i < NB; i++) {
       int e = tid + 256 * i;
       int gk = min(kk0 + e % KB, S.K - 1), gn = min(n0 + e / KB, P.N - 1);
-      rb[i] = gptr(S.B)[gk + (long long)gn * S.ldb];
+      xb[i] = gptr(S.B)[gk + (long long)gn * S.ldb];
+    }
+  };
+  auto load = [&](int sgi, int kk0) { load_into(sgi, kk0, ra, rb); };
+  auto store_from = [&](int buf, int kvalid, const T (&xa)[NA], const T (&xb)[NB]) {
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+      int e = tid + 256 * i;
+      As[buf][e / BM][e % BM] = xa[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      int e = tid + 256 * i;
+      Bs[buf][e % KB][e / KB] = xb[i];
+    }
+    if (kvalid < KB) {  // uniform: last chunk of a segment whose K % KB != 0
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        int e = tid + 256 * i;
+        if (e % KB >= kvalid) Bs[buf][e % KB][e / KB] = T(0);
+      }
     }
   };
   auto store = [&](int buf, int kvalid) {
@@ -451,59 +493,107 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
       kk0 = 0;
     }
   };
+  // one K chunk from LDS buffer buf: all LDS operands first (one wait),
+  // then the MFMA block
+  auto compute = [&](int buf) {
+      T bvk[KB / 4][BNB], avk[KB / 4][BMB];
+#pragma unroll
+      for (int kq = 0; kq < KB / 4; kq++) {
+#pragma unroll
+        for (int b = 0; b < BNB; b++) bvk[kq][b] = Bs[buf][4 * kq + lk][wn + b * 16 + l16];
+#pragma unroll
+        for (int a = 0; a < BMB; a++) avk[kq][a] = As[buf][4 * kq + lk][wm + a * 16 + l16];
+      }
+#pragma unroll
+      for (int kq = 0; kq < KB / 4; kq++) {
+        const T *bv = bvk[kq], *av = avk[kq];
+        if constexpr (F64) {
+          double avr[BMB][4];
+#pragma unroll
+          for (int a = 0; a < BMB; a++) {
+            avr[a][0] = av[a];
+            avr[a][1] = dpp_d<0x12C>(av[a]);  // lane i <- lane i + 4 (row of 16)
+            avr[a][2] = dpp_d<0x128>(av[a]);  // lane i <- lane i + 8
+            avr[a][3] = dpp_d<0x124>(av[a]);  // lane i <- lane i + 12
+          }
+#pragma unroll
+          for (int a = 0; a < BMB; a++)
+#pragma unroll
+            for (int b = 0; b < BNB; b++)
+#pragma unroll
+              for (int r = 0; r < 4; r++)
+                acc[a][b][r] = __builtin_amdgcn_mfma_f64_4x4x4f64(bv[b], avr[a][r], acc[a][b][r], 0, 0, 0);
+        } else {
+#pragma unroll
+          for (int a = 0; a < BMB; a++)
+#pragma unroll
+            for (int b = 0; b < BNB; b++)
+              accf[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[b], av[a], accf[a][b], 0, 0, 0);
+        }
+      }
+  };
   int sg = 0, k0 = 0;
   if (P.nseg == 0) return;
-  load(sg, k0);
-  store(0, segs[P.seg0].K - k0);
-  __syncthreads();
-  int buf = 0;
-  for (;;) {
-    int nsg = sg, nk0 = k0;
-    advance(nsg, nk0);
-    const bool more = nsg < P.nseg;
-    if (more) load(nsg, nk0);
-    // all LDS operands of the chunk first (one wait), then the MFMA block
-    T bvk[KB / 4][BNB], avk[KB / 4][BMB];
-#pragma unroll
-    for (int kq = 0; kq < KB / 4; kq++) {
-#pragma unroll
-      for (int b = 0; b < BNB; b++) bvk[kq][b] = Bs[buf][4 * kq + lk][wn + b * 16 + l16];
-#pragma unroll
-      for (int a = 0; a < BMB; a++) avk[kq][a] = As[buf][4 * kq + lk][wm + a * 16 + l16];
+  if constexpr (DEPTH == 1) {
+    load(sg, k0);
+    store(0, segs[P.seg0].K - k0);
+    __syncthreads();
+    int buf = 0;
+    for (;;) {
+      int nsg = sg, nk0 = k0;
+      advance(nsg, nk0);
+      const bool more = nsg < P.nseg;
+      if (more) load(nsg, nk0);
+      compute(buf);
+      if (!more) break;
+      store(buf ^ 1, segs[P.seg0 + nsg].K - nk0);
+      __syncthreads();
+      buf ^= 1;
+      sg = nsg;
+      k0 = nk0;
     }
-#pragma unroll
-    for (int kq = 0; kq < KB / 4; kq++) {
-      const T *bv = bvk[kq], *av = avk[kq];
-      if constexpr (F64) {
-        double avr[BMB][4];
-#pragma unroll
-        for (int a = 0; a < BMB; a++) {
-          avr[a][0] = av[a];
-          avr[a][1] = dpp_d<0x12C>(av[a]);  // lane i <- lane i + 4 (row of 16)
-          avr[a][2] = dpp_d<0x128>(av[a]);  // lane i <- lane i + 8
-          avr[a][3] = dpp_d<0x124>(av[a]);  // lane i <- lane i + 12
-        }
-#pragma unroll
-        for (int a = 0; a < BMB; a++)
-#pragma unroll
-          for (int b = 0; b < BNB; b++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-              acc[a][b][r] = __builtin_amdgcn_mfma_f64_4x4x4f64(bv[b], avr[a][r], acc[a][b][r], 0, 0, 0);
-      } else {
-#pragma unroll
-        for (int a = 0; a < BMB; a++)
-#pragma unroll
-          for (int b = 0; b < BNB; b++)
-            accf[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[b], av[a], accf[a][b], 0, 0, 0);
+  } else {
+    // chunk c is multiplied from LDS while chunk c+1 sits in one register
+    // stage (stored after the MFMA block) and chunk c+2 is in flight into
+    // the other; the stages alternate, so no register copies wait on loads
+    load(sg, k0);
+    store(0, segs[P.seg0].K - k0);
+    __syncthreads();
+    int s1 = sg, k1 = k0;
+    advance(s1, k1);
+    int s2 = s1, k2 = k1;
+    if (s1 < P.nseg) {
+      load_into(s1, k1, ra, rb);
+      advance(s2, k2);
+      if (s2 < P.nseg) load_into(s2, k2, ra2, rb2);
+    }
+    int buf = 0;
+    for (;;) {
+      // stage a holds (s1, k1), stage b holds (s2, k2)
+      compute(buf);
+      if (s1 >= P.nseg) break;
+      store_from(buf ^ 1, segs[P.seg0 + s1].K - k1, ra, rb);
+      __syncthreads();
+      buf ^= 1;
+      int s3 = s2, k3 = k2;
+      if (s3 < P.nseg) {
+        advance(s3, k3);
+        if (s3 < P.nseg) load_into(s3, k3, ra, rb);
+      }
+      compute(buf);
+      if (s2 >= P.nseg) break;
+      store_from(buf ^ 1, segs[P.seg0 + s2].K - k2, ra2, rb2);
+      __syncthreads();
+      buf ^= 1;
+      s1 = s3;
+      k1 = k3;
+      s2 = s3;
+      k2 = k3;
+      if (s2 < P.nseg) {
+        advance(s2, k2);
+        if (s2 < P.nseg) load_into(s2, k2, ra2, rb2);
       }
     }
-    if (!more) break;
-    store(buf ^ 1, segs[P.seg0 + nsg].K - nk0);
-    __syncthreads();
-    buf ^= 1;
-    sg = nsg;
-    k0 = nk0;
   }
   const int blk = (lane >> 2) & 3;
 #pragma unroll
@@ -549,20 +639,22 @@ void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *seg
   if (total_tiles <= 0 || nprob <= 0) return;
   hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
 }
-// C -= AB variants for tuning: tile 64 or 128, K chunk 16 or 32 (the tile
-// count is that of the tile size)
+// C -= AB variants for tuning: tile 64 or 128, K chunk 16 or 32, register
+// prefetch depth 1 or 2 (the tile count is that of the tile size)
 template <typename T>
 void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
-                         int total_tiles, int tile, int kb, hipStream_t st) {
+                         int total_tiles, int tile, int kb, int depth, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
-  if (tile == 128 && kb == 32)
-    hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 32>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
-  else if (tile == 128)
-    hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
-  else if (kb == 32)
-    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 32>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
-  else
-    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  const dim3 g(total_tiles), b(256);
+  if (tile == 128) {
+    if (depth == 2) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 2>), g, b, 0, st, probs, nprob, segs);
+    else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 32>), g, b, 0, st, probs, nprob, segs);
+    else hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16>), g, b, 0, st, probs, nprob, segs);
+  } else {
+    if (depth == 2) hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 2>), g, b, 0, st, probs, nprob, segs);
+    else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 32>), g, b, 0, st, probs, nprob, segs);
+    else hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16>), g, b, 0, st, probs, nprob, segs);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -651,17 +743,16 @@ void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
 // ---------------------------------------------------------------------------
 // The block is padded to 64 x 64 with a unit diagonal, so the inverses of
 // the padded factors carry the nb x nb inverses in their leading corner.
-// Inverses by recursive doubling over 256 threads:
-//   inv([[A, B], [0, C]]) = [[A^-1, -A^-1 B C^-1], [0, C^-1]]   (upper)
-//   inv([[A, 0], [B, C]]) = [[A^-1, 0], [-C^-1 B A^-1, C^-1]]   (unit lower)
-// with block size s = 1, 2, 4, ..., 32; each level is two small matrix
-// products over all pairs of diagonal blocks at once.
+// Inverses by substitution, one thread per column (wave 0: L^-1, wave 1:
+// U^-1), each thread's column kept in a padded LDS array (ld 65: the 64
+// columns fall in different banks):
+//   L^-1 (unit lower), column c: x_c = 1, x_r = -sum_{c<=k<r} L_rk x_k
+//   U^-1, column c: x_c = 1/U_cc, x_r = -(sum_{r<k<=c} U_rk x_k) / U_rr
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag_lu(const DiagProbT<T> *__restrict__ probs) {
   __shared__ T a[64 * 64];   // LU factors, col-major, ld 64
-  __shared__ T xu[64 * 64];  // U^-1
-  __shared__ T xl[64 * 64];  // L^-1
-  __shared__ T t[64 * 64];   // products: upper-right blocks for U, lower-left for L
+  __shared__ T xu[65 * 64];  // U^-1, col-major, ld 65
+  __shared__ T xl[65 * 64];  // L^-1, col-major, ld 65
   __shared__ T ub[2][64], lb[2][64];  // pivot row / pivot column, double-buffered
   const DiagProbT<T> P = probs[blockIdx.x];
   const int nb = P.nb, tid = threadIdx.x;
@@ -707,51 +798,32 @@ __global__ __launch_bounds__(256) void k_diag_lu(const DiagProbT<T> *__restrict_
     if (r < nb && c < nb) P.D[r + (long long)c * P.ld] = x[q];
   }
   __syncthreads();
-  for (int e = tid; e < 64 * 64; e += 256) {
-    int r = e & 63, c = e >> 6;
-    xu[e] = r == c ? T(1) / a[e] : T(0);
-    xl[e] = r == c ? T(1) : T(0);
+  // every lane of a wave walks the same (row, k) sequence, so the factor
+  // entries are LDS broadcasts; entries of x outside the column's triangle
+  // are zero and contribute nothing
+  if (tid < 64) {
+    const int c = tid;
+    T *x = xl + c * 65;
+    for (int rr = 0; rr < 64; rr++) {
+      T acc = T(0);
+      for (int k = 0; k < rr; k++) acc += a[rr + k * 64] * x[k];
+      x[rr] = rr < c ? T(0) : (rr == c ? T(1) : -acc);
+    }
+  } else if (tid < 128) {
+    const int c = tid - 64;
+    T *x = xu + c * 65;
+    for (int rr = 63; rr >= 0; rr--) {
+      T acc = T(0);
+      for (int k = rr + 1; k < 64; k++) acc += a[rr + k * 64] * x[k];
+      x[rr] = rr > c ? T(0) : (rr == c ? T(1) / a[c + c * 64] : -acc / a[rr + rr * 64]);
+    }
   }
   __syncthreads();
-  for (int lg = 0; lg < 6; lg++) {
-    const int sz = 1 << lg, tot = 32 * sz;  // 64 / (2 sz) pairs of sz x sz blocks
-    // t_U = B_U C^-1 (upper-right block), t_L = B_L A^-1 (lower-left block)
-    for (int e = tid; e < 2 * tot; e += 256) {
-      const bool up = e < tot;
-      const int f = up ? e : e - tot;
-      const int p = f >> (2 * lg), i = f & (sz - 1), j = (f >> lg) & (sz - 1);
-      const int r0 = p * 2 * sz, c0 = r0 + sz;
-      T acc = T(0);
-      if (up) {  // U[r0+i][c0+k] * xu[c0+k][c0+j], k <= j
-        for (int k = 0; k <= j; k++) acc += a[(r0 + i) + (c0 + k) * 64] * xu[(c0 + k) + (c0 + j) * 64];
-        t[(r0 + i) + (c0 + j) * 64] = acc;
-      } else {   // L[c0+i][r0+k] * xl[r0+k][r0+j], k >= j
-        for (int k = j; k < sz; k++) acc += a[(c0 + i) + (r0 + k) * 64] * xl[(r0 + k) + (r0 + j) * 64];
-        t[(c0 + i) + (r0 + j) * 64] = acc;
-      }
-    }
-    __syncthreads();
-    for (int e = tid; e < 2 * tot; e += 256) {
-      const bool up = e < tot;
-      const int f = up ? e : e - tot;
-      const int p = f >> (2 * lg), i = f & (sz - 1), j = (f >> lg) & (sz - 1);
-      const int r0 = p * 2 * sz, c0 = r0 + sz;
-      T acc = T(0);
-      if (up) {  // -A^-1 t_U: xu[r0+i][r0+k] * t[r0+k][c0+j], k >= i
-        for (int k = i; k < sz; k++) acc += xu[(r0 + i) + (r0 + k) * 64] * t[(r0 + k) + (c0 + j) * 64];
-        xu[(r0 + i) + (c0 + j) * 64] = -acc;
-      } else {   // -C^-1 t_L: xl[c0+i][c0+k] * t[c0+k][r0+j], k <= i
-        for (int k = 0; k <= i; k++) acc += xl[(c0 + i) + (c0 + k) * 64] * t[(c0 + k) + (r0 + j) * 64];
-        xl[(c0 + i) + (r0 + j) * 64] = -acc;
-      }
-    }
-    __syncthreads();
-  }
   for (int e = tid; e < 64 * 64; e += 256) {
     const int rr = e & 63, cc = e >> 6;
     if (rr < nb && cc < nb) {
-      P.Uinv[rr + cc * P.ldi] = xu[e];
-      P.Linv[rr + cc * P.ldi] = xl[e];
+      P.Uinv[rr + cc * P.ldi] = xu[rr + cc * 65];
+      P.Linv[rr + cc * P.ldi] = xl[rr + cc * 65];
     }
   }
 }
@@ -1127,7 +1199,7 @@ int device_lds_limit() {
   template void launch_gemm128<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,       \
                                   hipStream_t);                                               \
   template void launch_gemm_variant<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,  \
-                                       int, int, hipStream_t);                                \
+                                       int, int, int, hipStream_t);                           \
   template void launch_gemv<T>(const GemvProbT<T> *, int, const GemvSegT<T> *, int, int,     \
                                hipStream_t);                                                  \
   template void launch_diag_lu<T>(const DiagProbT<T> *, int, hipStream_t);                   \

@@ -335,3 +335,33 @@ def C_size():
 def C_ref(v):
     import ctypes
     return ctypes.byref(v)
+
+
+def test_graph_replays_equal_eager(S):
+    """Factorizations replayed from a captured HIP graph give the eager
+    results on every replay, with eager work and synchronisations between
+    the replays (single matrix and batch)."""
+    A = S.gen_grid(16, 16, 12, 7, 0, 6.1, 0)
+    order, sizes = S.order_grid(16, 16, 12, 16)
+    _, b = S.gen_rhs(A, 2, 1)
+    with S.SnuMat(A, 16, order=order, sizes=sizes, device=0) as M:
+        x0 = M.factorize().solve(b)
+        M.use_graph(True)
+        for rep in range(3):
+            x = M.factorize().solve(b)
+            assert np.array_equal(x, x0), rep
+            with S.SnuMat(A, 16, order=order, sizes=sizes, device=0) as E:  # eager work in between
+                assert np.array_equal(E.factorize().solve(b), x0)
+    mem = _batch_members(S)[:6]
+    bs = [S.gen_rhs(A, 1, 3)[1][:, 0] for A, _, _ in mem]
+    for precision in ("fp64", "fp32"):
+        mats = [S.SnuMat(A, np_, order=o, sizes=z, device=0, precision=precision) for A, np_, (o, z) in mem]
+        with S.Batch(mats) as B:
+            ref = B.factorize().solve(bs)
+            B.use_graph(True)
+            for rep in range(3):
+                xs = B.factorize().solve(bs)
+                for x, r in zip(xs, ref):
+                    assert np.array_equal(x, r), (precision, rep)
+        for M in mats:
+            M.close()
