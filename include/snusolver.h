@@ -163,7 +163,8 @@ int snubatch_destroy(snubatch_t *b);
  * matrix on the host, evicts idle least-recently-used matrices until it
  * fits the budget, then uploads and factorizes it (one precision per
  * server).  Each answer equals the matrix's standalone solve of the same
- * columns up to the refinement tolerance (bitwise for SNU_FP64). */
+ * columns to rounding (the sums of the sparse solves are grouped by the
+ * column count packed per matrix) and the refinement tolerance. */
 typedef struct snuserve snuserve_t;
 int snuserve_create(int device, size_t budget_bytes, int slots, int max_cols,
                     snuserve_t **out);

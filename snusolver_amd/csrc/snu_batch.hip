@@ -410,6 +410,7 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
       (rc = bupload(b, &b->d_ir_a, ira)))
     return fail(rc);
   if ((rc = balloc(b, (void **)&wd.stall, sizeof(int))) ||
+      (rc = balloc(b, (void **)&wd.half_done, sizeof(int) * (leaves.size() + 1))) ||
       (rc = balloc(b, (void **)&b->d_anorm, sizeof(unsigned long long) * count)) ||
       (rc = balloc(b, (void **)&b->d_active, sizeof(int) * count)) ||
       (rc = balloc(b, (void **)&b->d_vec, sizeof(VecDev) * count)) ||

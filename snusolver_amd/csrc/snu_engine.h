@@ -92,6 +92,7 @@ struct WorkDev {
   int sep_wsz = 0, sep_waves = 1;  // separator rows (same workspace, no flags)
   int max_leaf = 0;  // rows of the largest leaf (completion flags in LDS)
   int *stall = nullptr;  // leaf-row wait gave up (device flag)
+  int *half_done = nullptr;  // per leaf: its helper workgroup finished
 };
 
 // A factorization's launch sequence captured once into a HIP graph and

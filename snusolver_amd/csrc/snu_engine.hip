@@ -794,6 +794,8 @@ int upload(snumat *h) {
     D.nbc = L.nbc;
     D.nlev = (int)L.lev_ptr.size() - 1;
     D.nulev = (int)L.ulev_ptr.size() - 1;
+    D.split_a = L.split_a;
+    D.split_b = L.split_b;
     D.gstart = P.block_start[L.leaf];
     D.cb_off = cb;
     D.vec = 0;
@@ -951,7 +953,8 @@ int upload(snumat *h) {
     D.bcol_loc = d_bcol_loc + o.bcol;
     D.dd = h->d_dd;
   }
-  if ((rc = dalloc(h, &wd.stall, 1)) || (rc = dalloc(h, &h->d_vec, 1)) || (rc = dalloc(h, &h->d_perm, 2)) ||
+  if ((rc = dalloc(h, &wd.stall, 1)) || (rc = dalloc(h, &wd.half_done, np)) ||
+      (rc = dalloc(h, &h->d_vec, 1)) || (rc = dalloc(h, &h->d_perm, 2)) ||
       (rc = dalloc(h, &h->d_prep, 1)) || (rc = dalloc(h, &h->d_ir, 1)) ||
       (rc = dalloc(h, &h->d_active, 1)) || (rc = dalloc(h, &h->d_norms, 1)))
     return rc;
@@ -1056,7 +1059,8 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
   if (fs.ir) launch_rowabs_max(fs.ir, fs.nmat, fs.max_n, fs.anorm_out, st);
   if (ev) HIPCHK(hipEventRecord((*ev)[1], st));
   launch_zero_i32(wd.stall, 1, st);
-  launch_leaf_rows<T>(wd.leaves, wd.nleaves, wd.wsz, wd.waves, wd.max_leaf, fs.eps, wd.stall, st);
+  launch_leaf_rows<T>(wd.leaves, wd.nleaves, wd.wsz, wd.waves, wd.max_leaf, fs.eps, wd.stall,
+                      wd.half_done, st);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[2], st));
   launch_leaf_sep<T>(wd.leaves, wd.wg_leaf, wd.wg_first, wd.nwg, wd.sep_wsz, wd.sep_waves, st);

@@ -30,6 +30,7 @@ struct LeafPlan {
   std::vector<int> bcol;    // core cols of the boundary cols
   std::vector<int> lev_ptr, lev_rows;    // factor / forward level sets
   std::vector<int> ulev_ptr, ulev_rows;  // backward level sets
+  int split_a = 0, split_b = 0;          // rows [a, b) factor beside [0, a)
   double flops = 0;         // structural leaf flops (LU + Schur)
 };
 

@@ -33,6 +33,7 @@ struct LeafDev {
   void *dense;                           // the matrix's dense separator arena
   const DenseDev *dd;                    // the matrix's separator tables
   int core_n, leaf_size, nbr, nbc, nlev, nulev;
+  int split_a, split_b;                  // rows [split_a, split_b) depend on none below split_a
   int gstart;                            // permuted index of the first leaf row
   int cb_off;                            // rows of the rhs contribution buffer
   int vec;                               // the matrix's slot in the launch's VecDev table
@@ -126,9 +127,10 @@ void launch_prepare(const PrepDev *tab, int nmat, long long max_len, long long m
 // share the LDS with the waves' workspaces)
 // stall: set to 1 if a row gave up waiting for a pivot row (never expected;
 // the factors are then invalid and the caller reports an error)
+// half_done: nleaves ints (cleared here), the helper workgroups' completion
 template <typename T>
 void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, int max_leaf,
-                      double eps, int *stall, hipStream_t st);
+                      double eps, int *stall, int *half_done, hipStream_t st);
 int leaf_flag_bytes(int max_leaf);
 void launch_zero_i32(int *p, int n, hipStream_t st);
 // dst = src, bytes a multiple of 16

@@ -200,43 +200,84 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
   }
 }
 
-// One workgroup per leaf.  Wave w factors leaf rows w, w + W, w + 2W, ...
-// in ascending order; a row waits only for the pivot rows it reaches (LDS
-// completion flags), so the rows of a dense chain overlap instead of running
-// one level at a time.  Every wait ends: the lowest unfinished row depends
-// only on finished rows and its wave is working on it.
+// Two workgroups per leaf, 2L (helper) and 2L + 1 (main).  A leaf's rows
+// split as [0, sa) [sa, sb) [sb, ls): the middle range depends on no row
+// below sa (the two halves of the leaf's own dissection), so the helper
+// factors it while the main workgroup factors [0, sa); the main one then
+// waits for the helper's completion flag and factors [sb, ls), which may use
+// every earlier row.  The waiting workgroup has the higher index, so the
+// one it waits for was dispatched before it; the wait is bounded.  Leaves
+// without a split (sa == sb) run on the main workgroup alone.
+// Inside a workgroup, wave w takes rows w, w + W, ... of its ranges in
+// ascending order and waits (LDS completion flags) only for the pivot rows
+// it reaches, so the rows of a dense chain overlap; the lowest unfinished
+// row depends only on finished rows and its wave is working on it.
 // LDS: completion flags (fl bytes), then one workspace row (wsz values) per
 // wave.
 extern __shared__ __align__(16) char lds_raw[];
 template <typename T>
 __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ leaves,
-                                                   int wsz, int fl, T eps, int *stall) {
-  const LeafDev L = leaves[blockIdx.x];
+                                                   int wsz, int fl, T eps, int *stall,
+                                                   int *half_done) {
+  const int leaf = blockIdx.x >> 1, role = blockIdx.x & 1;
+  const LeafDev L = leaves[leaf];
   volatile unsigned char *done = (volatile unsigned char *)lds_raw;
   T *lds = (T *)(lds_raw + fl);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
+  if (role == 0 && L.split_a == L.split_b) return;  // no helper work
   for (int r = threadIdx.x; r < L.leaf_size; r += blockDim.x) done[r] = 0;
   T *w = lds + (size_t)wave * wsz;
   for (int c = lane; c < L.leaf_size + L.nbc; c += 64) w[c] = T(0);
   __syncthreads();
   const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol;
   T *v = (T *)L.vals;
-  for (int r = wave; r < L.leaf_size; r += W)
+  if (role == 0) {
+    for (int r = L.split_a + wave; r < L.split_b; r += W)
+      factor_row<T, true>(r, rp, dg, wc, v, w, lane, eps, done, stall);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      __hip_atomic_store(half_done + leaf, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return;
+  }
+  for (int r = wave; r < L.split_a; r += W)
+    factor_row<T, true>(r, rp, dg, wc, v, w, lane, eps, done, stall);
+  if (L.split_a < L.split_b) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      constexpr int kSpinLimit = 1 << 26;
+      for (int spin = 0;; spin++) {
+        if (__hip_atomic_load(half_done + leaf, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (spin == kSpinLimit) {
+          *stall = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int r = L.split_a + threadIdx.x; r < L.split_b; r += blockDim.x) done[r] = 1;
+    __syncthreads();
+  }
+  for (int r = L.split_b + wave; r < L.leaf_size; r += W)
     factor_row<T, true>(r, rp, dg, wc, v, w, lane, eps, done, stall);
 }
 // LDS bytes of the completion flags for leaves of up to max_leaf rows
 int leaf_flag_bytes(int max_leaf) { return (max_leaf + 15) & ~15; }
 template <typename T>
 void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, int max_leaf,
-                      double eps, int *stall, hipStream_t st) {
+                      double eps, int *stall, int *half_done, hipStream_t st) {
   if (nleaves <= 0) return;
   const int fl = leaf_flag_bytes(max_leaf);
   size_t lds = (size_t)fl + (size_t)wsz * waves * sizeof(T);
   (void)hipFuncSetAttribute((const void *)k_leaf_rows<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_leaf_rows<T>, dim3(nleaves), dim3(64 * waves), lds, st, leaves, wsz, fl,
-                     (T)eps, stall);
+  launch_zero_i32(half_done, nleaves, st);
+  hipLaunchKernelGGL(k_leaf_rows<T>, dim3(2 * nleaves), dim3(64 * waves), lds, st, leaves, wsz, fl,
+                     (T)eps, stall, half_done);
 }
 
 // Separator rows of the leaves: each wave factors one boundary row (its L_SL
@@ -880,8 +921,23 @@ void launch_permute_out(const PermDev *pd, int nmat, const VecDev *vd, const T *
                      pd, vd, bp);
 }
 
-// y_L = L_LL^-1 b_L (unit lower), level by level; the (row, rhs column)
-// pairs of a level are spread over the waves, one wave per pair.
+// Sparse dot products of the leaf solves with several right-hand sides: a
+// wave serves CW = the largest power of two <= min(nrhs, 16) columns at once,
+// KL = 64 / CW lanes per column striding over the row's entries, partial
+// sums combined within the KL lanes.  CW = 1 (one rhs) is the plain wave
+// reduction.  The rounding of a column's sum depends on CW, i.e. on nrhs.
+__device__ __forceinline__ int col_width(int nrhs) {
+  const int m = min(nrhs, 16);
+  return 1 << (31 - __clz(max(m, 1)));
+}
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int kl) {
+  for (int o = kl >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// y_L = L_LL^-1 b_L (unit lower), level by level; the (row, column group)
+// pairs of a level are spread over the waves.
 template <typename T>
 __global__ __launch_bounds__(256) void k_leaf_fwd(const LeafDev *__restrict__ leaves,
                                                   const VecDev *__restrict__ vd, T *bp) {
@@ -892,15 +948,18 @@ __global__ __launch_bounds__(256) void k_leaf_fwd(const LeafDev *__restrict__ le
   const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol;
   const T *v = (const T *)L.vals;
   const int *lp = L.levptr, *lr = L.levrows;
+  const int cw = col_width(V.nrhs), kl = 64 / cw, ng = (V.nrhs + cw - 1) / cw;
+  const int sub = lane % kl, cl = lane / kl;
   for (int lev = 0; lev < L.nlev; lev++) {
     const int r0 = lp[lev], nr = lp[lev + 1] - r0;
-    for (int t = wave; t < nr * V.nrhs; t += W) {
-      const int i = lr[r0 + t % nr], col = t / nr;
-      T *b = bp + V.base + (long long)col * V.ld + L.gstart;
+    for (int t = wave; t < nr * ng; t += W) {
+      const int i = lr[r0 + t % nr], col = (t / nr) * cw + cl;
+      const bool on = col < V.nrhs;
+      T *b = bp + V.base + (long long)min(col, V.nrhs - 1) * V.ld + L.gstart;
       T s = T(0);
-      for (int p = rp[i] + lane; p < dg[i]; p += 64) s += v[p] * b[wc[p]];
-      s = wave_sum(s);
-      if (lane == 0) b[i] -= s;
+      for (int p = rp[i] + sub; p < dg[i]; p += kl) s += v[p] * b[wc[p]];
+      s = group_sum(s, kl);
+      if (on && sub == 0) b[i] -= s;
     }
     __syncthreads();
   }
@@ -927,12 +986,16 @@ __global__ __launch_bounds__(256) void k_leaf_fwd_sep(const LeafDev *__restrict_
   const int r = L.brow[ri];
   const int *rp = L.rowptr, *wc = L.wcol;
   const T *v = (const T *)L.vals;
-  for (int col = 0; col < V.nrhs; col++) {
-    const T *b = bp + V.base + (long long)col * V.ld + L.gstart;
+  const int cw = col_width(V.nrhs), kl = 64 / cw;
+  const int sub = lane % kl, cl = lane / kl;
+  for (int c0 = 0; c0 < V.nrhs; c0 += cw) {
+    const int col = c0 + cl;
+    const bool on = col < V.nrhs;
+    const T *b = bp + V.base + (long long)min(col, V.nrhs - 1) * V.ld + L.gstart;
     T s = T(0);
-    for (int p = rp[r] + lane; p < rp[r + 1]; p += 64) s += v[p] * b[wc[p]];
-    s = wave_sum(s);
-    if (lane == 0) cbuf[L.cb_off + ri + (long long)col * cb_ld] = s;
+    for (int p = rp[r] + sub; p < rp[r + 1]; p += kl) s += v[p] * b[wc[p]];
+    s = group_sum(s, kl);
+    if (on && sub == 0) cbuf[L.cb_off + ri + (long long)col * cb_ld] = s;
   }
 }
 template <typename T>
@@ -991,19 +1054,22 @@ __global__ __launch_bounds__(256) void k_leaf_bwd(const LeafDev *__restrict__ le
   const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol, *bg = L.bcol_glob;
   const T *v = (const T *)L.vals;
   const int *lp = L.ulevptr, *lr = L.ulevrows;
+  const int cw = col_width(V.nrhs), kl = 64 / cw, ng = (V.nrhs + cw - 1) / cw;
+  const int sub = lane % kl, cl = lane / kl;
   for (int lev = 0; lev < L.nulev; lev++) {
     const int r0 = lp[lev], nr = lp[lev + 1] - r0;
-    for (int t = wave; t < nr * V.nrhs; t += W) {
-      const int i = lr[r0 + t % nr], col = t / nr;
-      T *b = bp + V.base + (long long)col * V.ld;
+    for (int t = wave; t < nr * ng; t += W) {
+      const int i = lr[r0 + t % nr], col = (t / nr) * cw + cl;
+      const bool on = col < V.nrhs;
+      T *b = bp + V.base + (long long)min(col, V.nrhs - 1) * V.ld;
       T s = T(0);
-      for (int p = dg[i] + 1 + lane; p < rp[i + 1]; p += 64) {
+      for (int p = dg[i] + 1 + sub; p < rp[i + 1]; p += kl) {
         int c = wc[p];
         long long g = c < ls ? (long long)L.gstart + c : (long long)bg[c - ls];
         s += v[p] * b[g];
       }
-      s = wave_sum(s);
-      if (lane == 0) b[L.gstart + i] = (b[L.gstart + i] - s) / v[dg[i]];
+      s = group_sum(s, kl);
+      if (on && sub == 0) b[L.gstart + i] = (b[L.gstart + i] - s) / v[dg[i]];
     }
     __syncthreads();
   }
@@ -1187,7 +1253,7 @@ int device_lds_limit() {
 #define SNU_INSTANTIATE(T)                                                                    \
   template void launch_fill<T>(T *, long long, hipStream_t);                                  \
   template void launch_prepare<T>(const PrepDev *, int, long long, long long, hipStream_t);   \
-  template void launch_leaf_rows<T>(const LeafDev *, int, int, int, int, double, int *,       \
+  template void launch_leaf_rows<T>(const LeafDev *, int, int, int, int, double, int *, int *,\
                                     hipStream_t);                                              \
   template void launch_leaf_sep<T>(const LeafDev *, const int *, const int *, int, int, int,  \
                                    hipStream_t);                                              \

@@ -267,9 +267,10 @@ def test_batch_equals_standalone(S, precision):
 def test_serve_alone_vs_mix(S, precision):
     """Requests against several resident matrices, with uneven column counts,
     submitted together and packed by two slots: every answer equals the
-    matrix's standalone solve of the same columns (bitwise for FP64; within
-    1e-10 relative for FP32 + refinement, whose step count depends on the
-    columns packed together), backward error <= 1e-12."""
+    matrix's standalone solve of the same columns within 1e-12 relative for
+    FP64 (the leaf solves' partial sums are grouped by the matrix's column
+    count in the pack) and 1e-10 for FP32 + refinement (whose step count also
+    depends on the columns packed together); backward error <= 1e-12."""
     mem = _batch_members(S)[2:7]
     rng = np.random.default_rng(11)
     reqs = []
@@ -289,10 +290,8 @@ def test_serve_alone_vs_mix(S, precision):
             x, lat = srv.wait(t)
             A = mem[m][0]
             assert S.backward_error(A, x, b) <= BERR_TOL
-            if precision == "fp64":
-                assert np.array_equal(x, xa)
-            else:
-                assert np.max(np.abs(x - xa)) <= 1e-10 * np.max(np.abs(xa))
+            tol = 1e-12 if precision == "fp64" else 1e-10
+            assert np.max(np.abs(x - xa)) <= tol * np.max(np.abs(xa))
         st = srv.stats()
         print(f"served {st['requests']:.0f} requests in {st['packed_solves']:.0f} packed solves, "
               f"p50 {st['p50_ms']:.2f} ms p99 {st['p99_ms']:.2f} ms")
