@@ -27,7 +27,8 @@ namespace snu {
 // into one launch.
 struct Launch {
   int kind;  // 0 diag LU; GEMM (64 tile) 1: C = AB, 2: C -= AB, 6: C = -AB;
-             // 5: GEMM C -= AB (128 tile); GEMV 7: y = Ax, 8: y -= Ax
+             // 5: GEMM C -= AB (128 tile); 3: copy C = A (GEMM descriptors,
+             // 64 x 64 tiles); GEMV 7: y = Ax, 8: y -= Ax
   int p0, np, tiles;
   double flops;
   long long key;

@@ -281,13 +281,16 @@ inline long long fkey(int height, long long step) { return (long long)height * 1
 
 // Host-built launch descriptors of the dense factorization of one matrix.
 // Per tree level, bottom-up:
-//   for each 64-column step k: k_diag_lu (LU of the diagonal block and the
-//   inverses of its triangles, written into the separator's full L^-1 / U^-1),
-//   L21 <- L21 U11^-1 and U12 <- L11^-1 U12 (GEMM by the block inverses),
-//   trailing update of R and Lp (GEMM);
+//   for each 64-column step k of the diagonal blocks D: k_diag_lu (LU of the
+//   diagonal block and the inverses of its triangles, written into the
+//   separator's full L^-1 / U^-1), L21 <- L21 U11^-1 and U12 <- L11^-1 U12
+//   inside D (GEMM by the block inverses), trailing update of D (GEMM);
 //   then the full inverses L^-1, U^-1 of every separator of the level by
-//   recursive doubling (two grouped GEMMs per doubling step; used by the
-//   solves);
+//   recursive doubling (two grouped GEMMs per doubling step);
+//   then the panels in one pass each, Lp <- Lp U^-1 and Up <- L^-1 Up (GEMM
+//   into scratch over the structurally nonzero rows / columns, K limited by
+//   the triangle, then copied back): the panels are read and written once
+//   instead of once per 64-column step;
 //   then one grouped GEMM for all Schur updates of the ancestors,
 //   K-concatenated over the level's separators below each of them.
 template <typename T>
@@ -333,57 +336,40 @@ void build_dense_plan(snumat *h) {
         d.np++;
       }
       if (d.np) out.push_back(d);
-      // L21 <- L21 U11^-1 (rows of D below the block, all rows of Lp)
+      // L21 <- L21 U11^-1 (rows of D below the block)
       Launch tr{1, (int)h_gp.size(), 0, 0, 0, fkey(height, ks + 1)};
       for (int i = lo; i <= hi; i++) {
-        int s = P.block_size[i], a = P.anc_sum[i];
+        int s = P.block_size[i];
         if (s <= k) continue;
         int nb = std::min(64, s - k);
         const T *U11i = Ui(i) + k + (long long)k * s;
         T *X1 = R(i) + (k + nb) + (long long)k * s;
         tr.np += add_gemm(h_gp, h_gs, tr.tiles, X1, s, s - k - nb, nb,
                           {GemmSeg{X1, U11i, s, s, nb}}, tr.flops);
-        T *X2 = Lp(i) + (long long)k * a;  // only the structurally nonzero rows
-        for (auto rr : nz_runs(a, [&](int x0, int x1) { return P.anyL(i, x0, x1); }))
-          tr.np += add_gemm(h_gp, h_gs, tr.tiles, X2 + rr.first, a, rr.second - rr.first, nb,
-                            {GemmSeg{X2 + rr.first, U11i, a, s, nb}}, tr.flops);
       }
       if (tr.np) out.push_back(tr);
-      // U12 <- L11^-1 U12 (row block of R right of the diagonal block)
+      // U12 <- L11^-1 U12 (row block of D right of the diagonal block)
       Launch tl{1, (int)h_gp.size(), 0, 0, 0, fkey(height, ks + 2)};
       for (int i = lo; i <= hi; i++) {
-        int s = P.block_size[i], a = P.anc_sum[i];
+        int s = P.block_size[i];
         if (s <= k) continue;
         int nb = std::min(64, s - k);
         const T *L11i = Li(i) + k + (long long)k * s;
-        T *X = R(i) + k + (long long)(k + nb) * s;  // D part, dense
+        T *X = R(i) + k + (long long)(k + nb) * s;
         tl.np += add_gemm(h_gp, h_gs, tl.tiles, X, s, nb, s - k - nb,
                           {GemmSeg{L11i, X, s, s, nb}}, tl.flops);
-        for (auto cr : nz_runs(a, [&](int x0, int x1) { return P.anyU(i, x0, x1); })) {
-          T *Xu = R(i) + k + (long long)(s + cr.first) * s;  // nonzero U-panel columns
-          tl.np += add_gemm(h_gp, h_gs, tl.tiles, Xu, s, nb, cr.second - cr.first,
-                            {GemmSeg{L11i, Xu, s, s, nb}}, tl.flops);
-        }
       }
       if (tl.np) out.push_back(tl);
-      // trailing updates inside the separator's own R and Lp
+      // trailing update of D
       GemmBatch<T> tu;
       for (int i = lo; i <= hi; i++) {
-        int s = P.block_size[i], a = P.anc_sum[i];
+        int s = P.block_size[i];
         if (s <= k) continue;
         int nb = std::min(64, s - k);
         T *Ccol = R(i) + k + (long long)(k + nb) * s;  // U12 rows
         const T *L21 = R(i) + (k + nb) + (long long)k * s;
         tu.add(h_gs, R(i) + (k + nb) + (long long)(k + nb) * s, s, s - k - nb, s - k - nb,
                {GemmSeg{L21, Ccol, s, s, nb}});
-        for (auto cr : nz_runs(a, [&](int x0, int x1) { return P.anyU(i, x0, x1); })) {
-          long long co = (long long)(s + cr.first) * s;
-          tu.add(h_gs, R(i) + (k + nb) + co, s, s - k - nb, cr.second - cr.first,
-                 {GemmSeg{L21, R(i) + k + co, s, s, nb}});
-        }
-        for (auto rr : nz_runs(a, [&](int x0, int x1) { return P.anyL(i, x0, x1); }))
-          tu.add(h_gs, Lp(i) + rr.first + (long long)(k + nb) * a, a, rr.second - rr.first,
-                 s - k - nb, {GemmSeg{Lp(i) + rr.first + (long long)k * a, Ccol, a, s, nb}});
       }
       tu.flush(h_gp, out, fkey(height, ks + 3));
     }
@@ -422,6 +408,50 @@ void build_dense_plan(snumat *h) {
         t2.p0 = (int)h_gp.size();
         h_gp.insert(h_gp.end(), fin.begin(), fin.end());
         out.push_back(t2);
+      }
+    }
+    // panels: Lp <- Lp U^-1 per 64-column strip of the result (U^-1 is upper
+    // triangular: strip [n0, n0 + 64) needs rows [0, n0 + 64) of it), Up <-
+    // L^-1 Up per 64-row strip (rows [0, m0 + 64) of Up), over the
+    // structurally nonzero row / column runs, into scratch; then copied back
+    {
+      Launch pm{1, (int)h_gp.size(), 0, 0, 0, fkey(height, 60000000LL)};
+      std::vector<GemmProbT<T>> cp;
+      Launch cl{3, 0, 0, 0, 0, fkey(height, 60000001LL)};
+      double cfl = 0;
+      long long toff = 0;
+      for (int i = lo; i <= hi; i++) {
+        const int s = P.block_size[i], a = P.anc_sum[i];
+        for (auto rr : nz_runs(a, [&](int x0, int x1) { return P.anyL(i, x0, x1); })) {
+          const int m = rr.second - rr.first;
+          T *dst = VV.tmp + toff;
+          toff += (long long)m * s;
+          T *src = Lp(i) + rr.first;
+          for (int n0 = 0; n0 < s; n0 += 64) {
+            const int nt = std::min(64, s - n0), kk = n0 + nt;
+            pm.np += add_gemm(h_gp, h_gs, pm.tiles, dst + (long long)n0 * m, m, m, nt,
+                              {GemmSeg{src, Ui(i) + (long long)n0 * s, a, s, kk}}, pm.flops);
+          }
+          cl.np += add_gemm(cp, h_gs, cl.tiles, src, a, m, s, {GemmSeg{dst, dst, m, m, 1}}, cfl);
+        }
+        for (auto cr : nz_runs(a, [&](int x0, int x1) { return P.anyU(i, x0, x1); })) {
+          const int nc = cr.second - cr.first;
+          T *dst = VV.tmp + toff;
+          toff += (long long)s * nc;
+          T *src = R(i) + (long long)(s + cr.first) * s;
+          for (int m0 = 0; m0 < s; m0 += 64) {
+            const int mt = std::min(64, s - m0), kk = m0 + mt;
+            pm.np += add_gemm(h_gp, h_gs, pm.tiles, dst + m0, s, mt, nc,
+                              {GemmSeg{Li(i) + m0, src, s, s, kk}}, pm.flops);
+          }
+          cl.np += add_gemm(cp, h_gs, cl.tiles, src, s, s, nc, {GemmSeg{dst, dst, s, s, 1}}, cfl);
+        }
+      }
+      if (pm.np) out.push_back(pm);
+      if (cl.np) {
+        cl.p0 = (int)h_gp.size();
+        h_gp.insert(h_gp.end(), cp.begin(), cp.end());
+        out.push_back(cl);
       }
     }
     // Schur updates of every ancestor, K-concatenated over the level's
@@ -484,13 +514,22 @@ void build_dense_plan(snumat *h) {
 }
 
 // Scratch needed by the products of the triangular inversion (max over the
-// doubling steps of every level).
+// doubling steps of every level) and by the panel products of a level.
 long long inversion_tmp_size(const Plan &P) {
   long long best = 0;
   for (int l = P.max_level - 1; l >= 0; l--) {
     int lo = 1 << l, hi = std::min((2 << l) - 1, P.np - 1);
     int smax = 0;
     for (int i = lo; i <= hi; i++) smax = std::max(smax, P.block_size[i]);
+    long long panels = 0;
+    for (int i = lo; i <= hi; i++) {
+      const int s = P.block_size[i], a = P.anc_sum[i];
+      for (auto rr : nz_runs(a, [&](int x0, int x1) { return P.anyL(i, x0, x1); }))
+        panels += (long long)(rr.second - rr.first) * s;
+      for (auto cr : nz_runs(a, [&](int x0, int x1) { return P.anyU(i, x0, x1); }))
+        panels += (long long)(cr.second - cr.first) * s;
+    }
+    best = std::max(best, panels);
     for (int hh = 64; hh < smax; hh *= 2) {
       long long t = 0;
       for (int i = lo; i <= hi; i++) {
@@ -1038,6 +1077,7 @@ void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
     case 0: launch_diag_lu(dp + L.p0, L.np, st); break;
     case 1: launch_gemm(gp + L.p0, L.np, gs, L.tiles, 1, st); break;
     case 2: launch_gemm(gp + L.p0, L.np, gs, L.tiles, 0, st); break;
+    case 3: launch_copy2d(gp + L.p0, L.np, gs, L.tiles, st); break;
     case 5: launch_gemm128(gp + L.p0, L.np, gs, L.tiles, st); break;
     case 6: launch_gemm(gp + L.p0, L.np, gs, L.tiles, 2, st); break;
     case 7: launch_gemv(vp + L.p0, L.np, vs, L.tiles, 1, st); break;

@@ -146,6 +146,10 @@ void launch_assemble_schur(int ntile, const int *tile_blk, const int *tile_row0,
                            const int *tile_ptr, const int *seg_leaf, const int *seg_ri,
                            const LeafDev *leaves, hipStream_t st);
 // mode 0: C -= sum A B; 1: C = A B; 2: C = -A B
+// C = A (first segment) per problem, 64 x 64 tiles numbered as the GEMM's
+template <typename T>
+void launch_copy2d(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs, int total_tiles,
+                   hipStream_t st);
 template <typename T>
 void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                  int total_tiles, int mode, hipStream_t st);

@@ -14,6 +14,8 @@
 
 #include "snu_kernels.h"
 
+#include <utility>
+
 namespace snu {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -95,16 +97,51 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
                            T eps, volatile unsigned char *done, int *stall) {
 #pragma clang fp contract(off)
   // Pivot rows are streamed one pivot ahead: while pivot t is applied, the
-  // first NCH chunks (64 entries each) of pivot t+1's U row are already in
-  // flight (unconditional loads with clamped addresses; the update itself is
-  // masked).  Longer U rows fall back to loads inside the update loop.
+  // first NCH chunks (64 entries each) of pivot t+1's U row are in flight
+  // into the other of two register sets (chunks past the row's end are not
+  // loaded; lanes past it read a clamped entry that is never used).  Longer
+  // U rows fall back to loads inside the update.  A pivot's U entries name
+  // distinct workspace columns, so its update gathers every workspace value
+  // it touches (one LDS round trip, overlapping the division) before it
+  // writes any of them back.
   constexpr int NCH = 16;
   const int p0 = rowptr[i], p1 = rowptr[i + 1];
   const int pd = LEAF_ROW ? diag[i] : p1;
   for (int p = p0 + lane; p < p1; p += 64) w[wcol[p]] = val[p];
   wave_lds_fence();
-  int cc[NCH], cn[NCH];
-  T uc[NCH], un[NCH];
+  int ca[NCH] = {}, cb[NCH] = {};
+  T ua[NCH] = {}, ub[NCH] = {};
+  auto fetch = [&](int s, int e, int(&c)[NCH], T(&u)[NCH]) {
+    const int n = (e - s + 63) >> 6;
+#pragma unroll
+    for (int j = 0; j < NCH; j++)
+      if (j < n) {
+        const int q = min(s + lane + 64 * j, e - 1);
+        c[j] = wcol[q];
+        u[j] = val[q];
+      }
+  };
+  auto apply = [&](int kt, int s, int e, T pt, const int(&c)[NCH], const T(&u)[NCH]) {
+    const int n = (e - s + 63) >> 6;
+    const T wk = w[kt];
+    T wv[NCH];
+#pragma unroll
+    for (int j = 0; j < NCH; j++) wv[j] = (j < n && s + lane + 64 * j < e) ? w[c[j]] : T(0);
+    const T l = wk / pt;
+#pragma unroll
+    for (int j = 0; j < NCH; j++)
+      if (j < n && s + lane + 64 * j < e) {
+        T prod = l * u[j];
+        w[c[j]] = wv[j] - prod;
+      }
+    for (int q = s + lane + 64 * NCH; q < e; q += 64) {
+      int c2 = wcol[q];
+      T prod = l * val[q];
+      w[c2] = w[c2] - prod;
+    }
+    if (lane == 0) w[kt] = l;
+    wave_lds_fence();
+  };
   for (int pb = p0; pb < pd;) {
     int cnt = min(64, pd - pb);
     int k = 0, ks = 1, ke = 1;
@@ -139,44 +176,14 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
       ke = rowptr[k + 1];
       piv = val[kd];
     }
-    {
-      const int s0 = rl(ks, 0), e0 = rl(ke, 0);
-#pragma unroll
-      for (int j = 0; j < NCH; j++) {
-        int q = min(s0 + lane + 64 * j, e0 - 1);
-        cn[j] = wcol[q];
-        un[j] = val[q];
-      }
-    }
-    for (int t = 0; t < cnt; t++) {
-      const int kt = rl(k, t), st = rl(ks, t), et = rl(ke, t);
-      const T pt = rld(piv, t);
-#pragma unroll
-      for (int j = 0; j < NCH; j++) cc[j] = cn[j], uc[j] = un[j];
-      if (t + 1 < cnt) {
-        const int s1 = rl(ks, t + 1), e1 = rl(ke, t + 1);
-#pragma unroll
-        for (int j = 0; j < NCH; j++) {
-          int q = min(s1 + lane + 64 * j, e1 - 1);
-          cn[j] = wcol[q];
-          un[j] = val[q];
-        }
-      }
-      const T l = w[kt] / pt;
-#pragma unroll
-      for (int j = 0; j < NCH; j++) {
-        if (st + lane + 64 * j < et) {
-          T prod = l * uc[j];
-          w[cc[j]] = w[cc[j]] - prod;
-        }
-      }
-      for (int q = st + lane + 64 * NCH; q < et; q += 64) {
-        int c2 = wcol[q];
-        T prod = l * val[q];
-        w[c2] = w[c2] - prod;
-      }
-      if (lane == 0) w[kt] = l;
-      wave_lds_fence();
+    // pivot t of the batch is held in (ca, ua) for even t, (cb, ub) for odd t
+    fetch(rl(ks, 0), rl(ke, 0), ca, ua);
+    for (int t = 0; t < cnt; t += 2) {
+      if (t + 1 < cnt) fetch(rl(ks, t + 1), rl(ke, t + 1), cb, ub);
+      apply(rl(k, t), rl(ks, t), rl(ke, t), rld(piv, t), ca, ua);
+      if (t + 1 >= cnt) break;
+      if (t + 2 < cnt) fetch(rl(ks, t + 2), rl(ke, t + 2), ca, ua);
+      apply(rl(k, t + 1), rl(ks, t + 1), rl(ke, t + 1), rld(piv, t + 1), cb, ub);
     }
     pb += cnt;
   }
@@ -200,7 +207,10 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
   }
 }
 
-// Two workgroups per leaf, 2L (helper) and 2L + 1 (main).  A leaf's rows
+// With split set, two workgroups per leaf, 2L (helper) and 2L + 1 (main);
+// otherwise one, which factors every row (used when the leaves alone fill
+// the GPU: a main workgroup waiting for its helper would hold a CU).  A
+// leaf's rows
 // split as [0, sa) [sa, sb) [sb, ls): the middle range depends on no row
 // below sa (the two halves of the leaf's own dissection), so the helper
 // factors it while the main workgroup factors [0, sa); the main one then
@@ -218,14 +228,15 @@ extern __shared__ __align__(16) char lds_raw[];
 template <typename T>
 __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ leaves,
                                                    int wsz, int fl, T eps, int *stall,
-                                                   int *half_done) {
-  const int leaf = blockIdx.x >> 1, role = blockIdx.x & 1;
+                                                   int *half_done, int split) {
+  const int leaf = split ? blockIdx.x >> 1 : blockIdx.x, role = split ? blockIdx.x & 1 : 1;
   const LeafDev L = leaves[leaf];
+  const int sa = split ? L.split_a : 0, sb = split ? L.split_b : 0;
   volatile unsigned char *done = (volatile unsigned char *)lds_raw;
   T *lds = (T *)(lds_raw + fl);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = blockDim.x >> 6;
-  if (role == 0 && L.split_a == L.split_b) return;  // no helper work
+  if (role == 0 && sa == sb) return;  // no helper work
   for (int r = threadIdx.x; r < L.leaf_size; r += blockDim.x) done[r] = 0;
   T *w = lds + (size_t)wave * wsz;
   for (int c = lane; c < L.leaf_size + L.nbc; c += 64) w[c] = T(0);
@@ -233,7 +244,7 @@ __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ l
   const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol;
   T *v = (T *)L.vals;
   if (role == 0) {
-    for (int r = L.split_a + wave; r < L.split_b; r += W)
+    for (int r = sa + wave; r < sb; r += W)
       factor_row<T, true>(r, rp, dg, wc, v, w, lane, eps, done, stall);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -242,9 +253,9 @@ __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ l
     }
     return;
   }
-  for (int r = wave; r < L.split_a; r += W)
+  for (int r = wave; r < sa; r += W)
     factor_row<T, true>(r, rp, dg, wc, v, w, lane, eps, done, stall);
-  if (L.split_a < L.split_b) {
+  if (sa < sb) {
     __syncthreads();
     if (threadIdx.x == 0) {
       constexpr int kSpinLimit = 1 << 26;
@@ -259,10 +270,10 @@ __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ l
     }
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int r = L.split_a + threadIdx.x; r < L.split_b; r += blockDim.x) done[r] = 1;
+    for (int r = sa + threadIdx.x; r < sb; r += blockDim.x) done[r] = 1;
     __syncthreads();
   }
-  for (int r = L.split_b + wave; r < L.leaf_size; r += W)
+  for (int r = sb + wave; r < L.leaf_size; r += W)
     factor_row<T, true>(r, rp, dg, wc, v, w, lane, eps, done, stall);
 }
 // LDS bytes of the completion flags for leaves of up to max_leaf rows
@@ -275,9 +286,16 @@ void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, in
   size_t lds = (size_t)fl + (size_t)wsz * waves * sizeof(T);
   (void)hipFuncSetAttribute((const void *)k_leaf_rows<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  launch_zero_i32(half_done, nleaves, st);
-  hipLaunchKernelGGL(k_leaf_rows<T>, dim3(2 * nleaves), dim3(64 * waves), lds, st, leaves, wsz, fl,
-                     (T)eps, stall, half_done);
+  // split when both workgroups of every leaf are resident together
+  int dev = 0, cus = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_leaf_rows<T>,
+                                                     64 * waves, lds);
+  const int split = 2LL * nleaves <= (long long)cus * per_cu ? 1 : 0;
+  if (split) launch_zero_i32(half_done, nleaves, st);
+  hipLaunchKernelGGL(k_leaf_rows<T>, dim3(split ? 2 * nleaves : nleaves), dim3(64 * waves), lds,
+                     st, leaves, wsz, fl, (T)eps, stall, half_done, split);
 }
 
 // Separator rows of the leaves: each wave factors one boundary row (its L_SL
@@ -663,6 +681,35 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
         }
       }
 }
+// Grouped 2-D copy over GEMM descriptors: C (ldc) = A of the problem's first
+// segment (lda), one workgroup per 64 x 64 tile, rows contiguous per wave.
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy2d(const GemmProbT<T> *__restrict__ probs, int nprob,
+                                                const GemmSegT<T> *__restrict__ segs) {
+  const int tile = blockIdx.x;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmProbT<T> P = probs[lo];
+  const GemmSegT<T> S = segs[P.seg0];
+  const int lt = tile - P.tile0;
+  const int m0 = (lt / P.tiles_n) * 64, n0 = (lt % P.tiles_n) * 64;
+  const int r = threadIdx.x & 63;
+  if (m0 + r >= P.M) return;
+  const T *src = S.A + (m0 + r) + (long long)n0 * S.lda;
+  T *dst = P.C + (m0 + r) + (long long)n0 * P.ldc;
+  const int nc = min(64, P.N - n0);
+  for (int c = threadIdx.x >> 6; c < nc; c += 4) dst[(long long)c * P.ldc] = src[(long long)c * S.lda];
+}
+template <typename T>
+void launch_copy2d(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs, int total_tiles,
+                   hipStream_t st) {
+  if (total_tiles <= 0 || nprob <= 0) return;
+  hipLaunchKernelGGL(k_copy2d<T>, dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+}
 template <typename T>
 void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                  int total_tiles, int mode, hipStream_t st) {
@@ -782,96 +829,115 @@ void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
 // dense: diagonal block LU (no pivoting, as cusolverDnDgetrf with a null
 // ipiv / LAPACKE_mkl_dgetrfnp) and the inverses of its triangular factors.
 // ---------------------------------------------------------------------------
-// The block is padded to 64 x 64 with a unit diagonal, so the inverses of
-// the padded factors carry the nb x nb inverses in their leading corner.
-// Inverses by substitution, one thread per column (wave 0: L^-1, wave 1:
-// U^-1), each thread's column kept in a padded LDS array (ld 65: the 64
-// columns fall in different banks):
-//   L^-1 (unit lower), column c: x_c = 1, x_r = -sum_{c<=k<r} L_rk x_k
-//   U^-1, column c: x_c = 1/U_cc, x_r = -(sum_{r<k<=c} U_rk x_k) / U_rr
+// One workgroup of 8 waves per block, thread (r, g) holding row r, columns
+// g, g + 8, ..., g + 56 of three 64 x 64 matrices in registers (the block is
+// padded with a unit diagonal): the factors being formed, X = L^-1 and
+// Y = U^-1.
+// Pivot step j (one barrier): row j of the factors (final: U row j) and of
+// X, column j of the factors and of Y are published in LDS; then with one
+// division m = (r > j ? A_rj : Y_rj) / U_jj per thread,
+//   rows r > j:  A_rc -= m U_jc (c > j), L_rj = m,  X_rc -= m X_jc (c <= j)
+//   rows r <= j: Y_rj = m,  Y_rc -= m U_jc (c > j)
+// which are right-looking LU, forward elimination L X = I (X_j is final at
+// step j) and Y U = I column by column (Y_rj final once divided by U_jj).
+// Pivot step J (a template parameter, so that whether column g + 8q is left
+// of, at or right of J is a compile-time fact for most (J, q) and a uniform
+// select for the rest; runtime steps make the compiler shuffle the register
+// arrays between branches).
+template <typename T, int J>
+__device__ __forceinline__ void diag_step(T (&x)[8], T (&xl)[8], T (&xu)[8], int r, int g,
+                                          T (*ub)[64], T (*xr)[64], T (*lb)[64], T (*yb)[64]) {
+  constexpr int b = J & 1;
+  if (r == J) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      ub[b][g + 8 * q] = x[q];
+      xr[b][g + 8 * q] = xl[q];
+    }
+  }
+  if (g == (J & 7)) {
+    lb[b][r] = x[J >> 3];
+    yb[b][r] = xu[J >> 3];
+  }
+  __syncthreads();
+  // keep the scheduler from interleaving steps (register pressure)
+  __builtin_amdgcn_sched_barrier(0);
+  T uq[8], xq[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    if (8 * q + 7 > J) uq[q] = ub[b][g + 8 * q];
+    if (8 * q <= J) xq[q] = xr[b][g + 8 * q];
+  }
+  const bool below = r > J;
+  const T m = (below ? lb[b][r] : yb[b][r]) / ub[b][J];
+  // multipliers zeroed on the lanes an update does not apply to (x - 0 u is
+  // x exactly)
+  const T mb = below ? m : T(0), ma = below ? T(0) : m;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    if (8 * q + 7 < J) {  // left of J for every wave
+      xl[q] = xl[q] - mb * xq[q];
+    } else if (8 * q > J) {  // right of J for every wave
+      x[q] = x[q] - mb * uq[q];
+      xu[q] = xu[q] - ma * uq[q];
+    } else {  // column g + 8q is left of, at or right of J depending on g
+      const int c = g + 8 * q;
+      const bool right = c > J, at = c == J;
+      const T nx = x[q] - mb * uq[q];
+      const T ny = xu[q] - ma * uq[q];
+      const T nl = xl[q] - mb * xq[q];
+      x[q] = right ? nx : (at && below ? m : x[q]);
+      xu[q] = right ? ny : (at && !below ? m : xu[q]);
+      xl[q] = right ? xl[q] : nl;
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  // opaque to the optimizer: without this it carries facts about earlier
+  // steps' values forward and the register pressure grows with every step
+#pragma unroll
+  for (int q = 0; q < 8; q++) asm volatile("" : "+v"(x[q]), "+v"(xl[q]), "+v"(xu[q]));
+}
+template <typename T, int... J>
+__device__ __forceinline__ void diag_steps(std::integer_sequence<int, J...>, T (&x)[8],
+                                           T (&xl)[8], T (&xu)[8], int r, int g, T (*ub)[64],
+                                           T (*xr)[64], T (*lb)[64], T (*yb)[64]) {
+  (diag_step<T, J>(x, xl, xu, r, g, ub, xr, lb, yb), ...);
+}
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag_lu(const DiagProbT<T> *__restrict__ probs) {
-  __shared__ T a[64 * 64];   // LU factors, col-major, ld 64
-  __shared__ T xu[65 * 64];  // U^-1, col-major, ld 65
-  __shared__ T xl[65 * 64];  // L^-1, col-major, ld 65
-  __shared__ T ub[2][64], lb[2][64];  // pivot row / pivot column, double-buffered
+__global__ __launch_bounds__(512) void k_diag_lu(const DiagProbT<T> *__restrict__ probs) {
+  __shared__ T ub[2][64], xr[2][64], lb[2][64], yb[2][64];
   const DiagProbT<T> P = probs[blockIdx.x];
   const int nb = P.nb, tid = threadIdx.x;
-  // LU in registers: thread (r, g) holds row r, columns g, g + 4, ..., g + 60
-  // (padding: identity beyond nb).  Per pivot j one barrier: the pivot row
-  // and column are published in LDS, every thread forms its row's
-  // multiplier and updates its 16 entries.
-  const int r = tid & 63, g = tid >> 6;
-  T x[16];
+  const int r = tid & 63, g = __builtin_amdgcn_readfirstlane(tid >> 6);
+  T x[8], xl[8], xu[8];
 #pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const int c = g + 4 * q;
+  for (int q = 0; q < 8; q++) {
+    const int c = g + 8 * q;
     x[q] = (r < nb && c < nb) ? P.D[r + (long long)c * P.ld] : (r == c ? T(1) : T(0));
+    xl[q] = xu[q] = r == c ? T(1) : T(0);
   }
-  for (int j = 0; j < nb; j++) {
-    const int b = j & 1;
-    if (r == j) {
+  // all 64 steps run: past nb the padding is an identity block, whose steps
+  // change nothing (zero multipliers)
+  diag_steps<T>(std::make_integer_sequence<int, 64>{}, x, xl, xu, r, g, ub, xr, lb, yb);
+  // the descriptor is read again (through a pointer the compiler cannot
+  // equate with the first), so no store address stays live across the steps
+  const DiagProbT<T> *pq = probs;
+  asm volatile("" : "+s"(pq));
+  const DiagProbT<T> Q = pq[blockIdx.x];
 #pragma unroll
-      for (int q = 0; q < 16; q++) ub[b][g + 4 * q] = x[q];
-    }
-    if (g == (j & 3)) {
-      T v = x[0];
-#pragma unroll
-      for (int q = 1; q < 16; q++)
-        if (4 * q + g == j) v = x[q];
-      lb[b][r] = v;
-    }
-    __syncthreads();
-    if (r > j) {
-      const T l = lb[b][r] / ub[b][j];
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int c = g + 4 * q;
-        if (c > j) x[q] -= l * ub[b][c];
-        else if (c == j) x[q] = l;
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 16; q++) {
-    const int c = g + 4 * q;
-    a[r + c * 64] = x[q];
-    if (r < nb && c < nb) P.D[r + (long long)c * P.ld] = x[q];
-  }
-  __syncthreads();
-  // every lane of a wave walks the same (row, k) sequence, so the factor
-  // entries are LDS broadcasts; entries of x outside the column's triangle
-  // are zero and contribute nothing
-  if (tid < 64) {
-    const int c = tid;
-    T *x = xl + c * 65;
-    for (int rr = 0; rr < 64; rr++) {
-      T acc = T(0);
-      for (int k = 0; k < rr; k++) acc += a[rr + k * 64] * x[k];
-      x[rr] = rr < c ? T(0) : (rr == c ? T(1) : -acc);
-    }
-  } else if (tid < 128) {
-    const int c = tid - 64;
-    T *x = xu + c * 65;
-    for (int rr = 63; rr >= 0; rr--) {
-      T acc = T(0);
-      for (int k = rr + 1; k < 64; k++) acc += a[rr + k * 64] * x[k];
-      x[rr] = rr > c ? T(0) : (rr == c ? T(1) / a[c + c * 64] : -acc / a[rr + rr * 64]);
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int rr = e & 63, cc = e >> 6;
-    if (rr < nb && cc < nb) {
-      P.Uinv[rr + cc * P.ldi] = xu[rr + cc * 65];
-      P.Linv[rr + cc * P.ldi] = xl[rr + cc * 65];
+  for (int q = 0; q < 8; q++) {
+    const int c = g + 8 * q;
+    if (r < Q.nb && c < Q.nb) {
+      Q.D[r + (long long)c * Q.ld] = x[q];
+      Q.Linv[r + (long long)c * Q.ldi] = xl[q];
+      Q.Uinv[r + (long long)c * Q.ldi] = xu[q];
     }
   }
 }
 template <typename T>
 void launch_diag_lu(const DiagProbT<T> *probs, int nprob, hipStream_t st) {
   if (nprob <= 0) return;
-  hipLaunchKernelGGL(k_diag_lu<T>, dim3(nprob), dim3(256), 0, st, probs);
+  hipLaunchKernelGGL(k_diag_lu<T>, dim3(nprob), dim3(512), 0, st, probs);
 }
 
 // ---------------------------------------------------------------------------
@@ -1260,6 +1326,7 @@ int device_lds_limit() {
   template void launch_assemble_schur<T>(int, const int *, const int *, const int *,          \
                                          const int *, const int *, const LeafDev *,           \
                                          hipStream_t);                                        \
+  template void launch_copy2d<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int, hipStream_t); \
   template void launch_gemm<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int, int,     \
                                hipStream_t);                                                  \
   template void launch_gemm128<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,       \
