@@ -141,7 +141,7 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
         "traffic": profiled_traffic(),
-        "kernel": "snu::k_gemm<double, 0, 2, 2> (grouped FP64 MFMA 4x4x4, C -= sum A B)",
+        "kernel": "snu::k_gemm<double, 0, 2, 2, 16, 1, true> (grouped FP64 MFMA 16x16x4, C -= sum A B)",
         "launches": int(gemm_launches),
         "avg_launch_ms": round(gemm_ms / max(gemm_launches, 1), 4),
         "gflop_per_launch": round(gemm_gflop / max(gemm_launches, 1), 4),

@@ -652,7 +652,8 @@ extern "C" int snumat_set_profiling(snumat_t *h, int on) {
 // time over reps launches (HIP events).
 namespace {
 template <typename T>
-int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int reps, double *ms_out) {
+int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int m16, int reps,
+               double *ms_out) {
   T *dA, *dB, *dC;
   HIPCHK(hipMalloc((void **)&dA, sizeof(T) * (size_t)M * K));
   HIPCHK(hipMalloc((void **)&dB, sizeof(T) * (size_t)K * N));
@@ -678,7 +679,7 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int r
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  auto run = [&]() { launch_gemm_variant<T>(dgp, 1, dgs, tiles, tile, kb, depth, nullptr); };
+  auto run = [&]() { launch_gemm_variant<T>(dgp, 1, dgs, tiles, tile, kb, depth, m16, nullptr); };
   run();
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipEventRecord(e0, nullptr));
@@ -701,15 +702,17 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int r
 extern "C" int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                               double *ms_out) {
   // tile: 64 or 128 (K chunk 16), 6432 / 12832 (K chunk 32), 642 / 1282
-  // (K chunk 16, loads two chunks ahead)
-  int t = tile, kb = 16, depth = 1;
+  // (K chunk 16, loads two chunks ahead), 6416 / 12816 (FP64 on the
+  // 16x16x4 instruction)
+  int t = tile, kb = 16, depth = 1, m16 = 0;
   if (tile == 6432 || tile == 12832) t = tile / 100, kb = 32;
   if (tile == 642 || tile == 1282) t = tile / 10, depth = 2;
+  if (tile == 6416 || tile == 12816) t = tile / 100, m16 = 1;
   if (M < 1 || N < 1 || K < 1 || nseg < 1 || reps < 1 || (t != 64 && t != 128) ||
       (prec != SNU_FP64 && prec != SNU_FP32) || !ms_out)
     return set_error(SNU_ERR_BAD_INPUT, "bad gemm bench shape");
-  return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, t, kb, depth, reps, ms_out)
-                          : bench_gemm<double>(M, N, K, nseg, t, kb, depth, reps, ms_out);
+  return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, t, kb, depth, m16, reps, ms_out)
+                          : bench_gemm<double>(M, N, K, nseg, t, kb, depth, m16, reps, ms_out);
 }
 
 // Average time (ms) of a plain device copy of `bytes` (multiple of 16) over

@@ -397,6 +397,8 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   wd.sep_wsz = sep_wsz;
   wd.sep_waves = sep_waves;
   wd.max_leaf = max_leaf;
+  wd.max_lev = 0;
+  for (snumat *h : b->mats) wd.max_lev = std::max(wd.max_lev, h->wd.max_lev);
   int rc;
   if ((rc = bupload(b, &wd.leaves, leaves)) || (rc = bupload(b, &wd.wg_leaf, W.wg_leaf)) ||
       (rc = bupload(b, &wd.wg_first, W.wg_first)) || (rc = bupload(b, &wd.fwg_leaf, W.fwg_leaf)) ||

@@ -92,6 +92,7 @@ struct WorkDev {
   int wsz = 0, waves = 1;          // leaf rows (workspace: leaf + boundary columns)
   int sep_wsz = 0, sep_waves = 1;  // separator rows (same workspace, no flags)
   int max_leaf = 0;  // rows of the largest leaf (completion flags in LDS)
+  int max_lev = 0;   // most level sets of a leaf (forward or backward)
   int *stall = nullptr;  // leaf-row wait gave up (device flag)
   int *half_done = nullptr;  // per leaf: its helper workgroup finished
 };

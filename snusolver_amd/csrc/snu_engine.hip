@@ -478,35 +478,28 @@ void build_dense_plan(snumat *h) {
               if (P.anyL(i, r0, r0 + mt) && P.anyU(i, c0, c0 + nt)) who[(size_t)a * tn + b].push_back(i);
             }
           }
-        // rectangles of tiles with one contributor set: runs along a tile
-        // row, then equal runs of consecutive tile rows stacked
-        std::vector<char> used((size_t)tm * tn, 0);
-        for (int a = 0; a < tm; a++)
-          for (int b = 0; b < tn; b++) {
-            const std::vector<int> &w = who[(size_t)a * tn + b];
-            if (used[(size_t)a * tn + b] || w.empty()) continue;
-            int b1 = b + 1;
-            while (b1 < tn && !used[(size_t)a * tn + b1] && who[(size_t)a * tn + b1] == w) b1++;
-            int a1 = a + 1;
-            for (; a1 < tm; a1++) {
-              bool same = true;
-              for (int q = b; q < b1 && same; q++)
-                same = !used[(size_t)a1 * tn + q] && who[(size_t)a1 * tn + q] == w;
-              if (!same) break;
+        // one problem per 64 x 64 target tile with its contributor set,
+        // emitted in strips of 8 tile rows, column by column inside a
+        // strip: consecutive tiles (which one XCD runs together) share A
+        // rows (same tile row and contributor) and B columns (same tile
+        // column), so their operands meet in that XCD's L2
+        constexpr int kStrip = 8;
+        for (int a0 = 0; a0 < tm; a0 += kStrip)
+          for (int b = 0; b < tn; b++)
+            for (int a = a0; a < std::min(tm, a0 + kStrip); a++) {
+              const std::vector<int> &w = who[(size_t)a * tn + b];
+              if (w.empty()) continue;
+              const int m0 = 64 * a, n0 = 64 * b;
+              const int mt = std::min(64, M - m0), nt = std::min(64, N - n0);
+              std::vector<GemmSeg> segs;
+              for (int i : w) {
+                const int si = P.block_size[i], ai = P.anc_sum[i], off = P.anc_off(i, p);
+                const int r0 = off + (pass == 0 ? 0 : sp) + m0, c0 = off + n0;
+                segs.push_back(GemmSeg{Lp(i) + r0, R(i) + (long long)(si + c0) * si, ai, si, si});
+              }
+              T *C = pass == 0 ? R(p) + m0 + (long long)n0 * sp : Lp(p) + m0 + (long long)n0 * ap;
+              sc.add(h_gs, C, pass == 0 ? sp : ap, mt, nt, segs);
             }
-            for (int x = a; x < a1; x++)
-              for (int q = b; q < b1; q++) used[(size_t)x * tn + q] = 1;
-            const int m0 = 64 * a, n0 = 64 * b;
-            const int mt = std::min(M, 64 * a1) - m0, nt = std::min(N, 64 * b1) - n0;
-            std::vector<GemmSeg> segs;
-            for (int i : w) {
-              const int si = P.block_size[i], ai = P.anc_sum[i], off = P.anc_off(i, p);
-              const int r0 = off + (pass == 0 ? 0 : sp) + m0, c0 = off + n0;
-              segs.push_back(GemmSeg{Lp(i) + r0, R(i) + (long long)(si + c0) * si, ai, si, si});
-            }
-            T *C = pass == 0 ? R(p) + m0 + (long long)n0 * sp : Lp(p) + m0 + (long long)n0 * ap;
-            sc.add(h_gs, C, pass == 0 ? sp : ap, mt, nt, segs);
-          }
       }
     }
     sc.flush(h_gp, out, fkey(height, 90000000LL));
@@ -790,8 +783,8 @@ int upload(snumat *h) {
   const int np = P.np;
   // ---- leaf arrays (concatenated; descriptors hold pointers into them) ----
   std::vector<int> rowptr, diag, levptr, levrows, ulevptr, ulevrows;
-  std::vector<int> brow, bcol_glob, bcol_blk, bcol_loc, wcol;
-  struct Off { size_t rp, idx, lev, levr, ulev, ulevr, brow, bcol; long long S; };
+  std::vector<int> brow, bcol_glob, bcol_blk, bcol_loc, wcol, ftask, btask, ubnd;
+  struct Off { size_t rp, idx, lev, levr, ulev, ulevr, brow, bcol, task, row; long long S; };
   std::vector<Off> off(np);
   long long S_total = 0;
   h->h_leaves.assign(np, LeafDev());
@@ -808,11 +801,32 @@ int upload(snumat *h) {
     o.ulevr = ulevrows.size();
     o.brow = brow.size();
     o.bcol = bcol_glob.size();
+    o.task = ftask.size();
+    o.row = ubnd.size();
     o.S = S_total;
     S_total += (long long)L.nbr * L.nbc;
     rowptr.insert(rowptr.end(), L.lu_rowptr.begin(), L.lu_rowptr.end());
     diag.insert(diag.end(), L.lu_diag.begin(), L.lu_diag.end());
     diag.push_back(0);  // diag shares rowptr's offsets (core_n + 1 entries)
+    // level-ordered solve tasks {row, first entry, end, diagonal}: forward
+    // over the L part, backward over the U entries on leaf columns (they
+    // precede the boundary ones: columns ascend), then the boundary start
+    {
+      const int ls = L.leaf_size;
+      std::vector<int> pint(ls);
+      for (int i = 0; i < ls; i++) {
+        int q = L.lu_diag[i] + 1;
+        while (q < L.lu_rowptr[i + 1] && L.lu_colidx[q] < ls) q++;
+        pint[i] = q;
+        ubnd.push_back(q);
+      }
+      for (int k = 0; k < ls; k++) {
+        const int i = L.lev_rows[k];
+        ftask.insert(ftask.end(), {i, L.lu_rowptr[i], L.lu_diag[i], L.lu_diag[i]});
+        const int j = L.ulev_rows[k];
+        btask.insert(btask.end(), {j, L.lu_diag[j] + 1, pint[j], L.lu_diag[j]});
+      }
+    }
     wcol.insert(wcol.end(), L.wcol.begin(), L.wcol.end());
     levptr.insert(levptr.end(), L.lev_ptr.begin(), L.lev_ptr.end());
     levrows.insert(levrows.end(), L.lev_rows.begin(), L.lev_rows.end());
@@ -846,6 +860,9 @@ int upload(snumat *h) {
   h->wd.wsz = (maxw + 1) & ~1;
   h->wd.max_leaf = 0;
   for (int r = 0; r < np; r++) h->wd.max_leaf = std::max(h->wd.max_leaf, P.leaves[r].leaf_size);
+  h->wd.max_lev = 0;
+  for (int r = 0; r < np; r++)
+    h->wd.max_lev = std::max(h->wd.max_lev, (int)std::max(P.leaves[r].lev_ptr.size(), P.leaves[r].ulev_ptr.size()) - 1);
   const int vsize = h->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
   h->wd.waves = waves_for(h->wd.wsz, vsize, h->wd.max_leaf);
   h->wd.sep_wsz = h->wd.wsz;
@@ -936,7 +953,7 @@ int upload(snumat *h) {
   h->arena_len = P.leaf_total + P.dense_total;
   // ---- structure uploads ----
   int *d_rowptr, *d_diag, *d_wcol, *d_levptr, *d_levrows, *d_ulevptr, *d_ulevrows;
-  int *d_brow, *d_bcol_glob, *d_bcol_blk, *d_bcol_loc;
+  int *d_brow, *d_bcol_glob, *d_bcol_blk, *d_bcol_loc, *d_ftask, *d_btask, *d_ubnd;
   if ((rc = dalloc(h, &h->d_aval, (size_t)P.nnz))) return rc;
   if ((rc = dupload(h, &h->d_adest, std::vector<long long>(P.a_dest.begin(), P.a_dest.end())))) return rc;
   if ((rc = dupload(h, &d_rowptr, rowptr)) || (rc = dupload(h, &d_diag, diag)) ||
@@ -944,7 +961,9 @@ int upload(snumat *h) {
       (rc = dupload(h, &d_levrows, levrows)) || (rc = dupload(h, &d_ulevptr, ulevptr)) ||
       (rc = dupload(h, &d_ulevrows, ulevrows)) || (rc = dupload(h, &d_brow, brow)) ||
       (rc = dupload(h, &d_bcol_glob, bcol_glob)) || (rc = dupload(h, &d_bcol_blk, bcol_blk)) ||
-      (rc = dupload(h, &d_bcol_loc, bcol_loc)) || (rc = dupload(h, &h->d_order, P.order)))
+      (rc = dupload(h, &d_bcol_loc, bcol_loc)) || (rc = dupload(h, &h->d_order, P.order)) ||
+      (rc = dupload(h, &d_ftask, ftask)) || (rc = dupload(h, &d_btask, btask)) ||
+      (rc = dupload(h, &d_ubnd, ubnd)))
     return rc;
   WorkDev &wd = h->wd;
   wd.nleaves = np;
@@ -990,6 +1009,9 @@ int upload(snumat *h) {
     D.bcol_glob = d_bcol_glob + o.bcol;
     D.bcol_blk = d_bcol_blk + o.bcol;
     D.bcol_loc = d_bcol_loc + o.bcol;
+    D.ftask = d_ftask + o.task;
+    D.btask = d_btask + o.task;
+    D.ubnd = d_ubnd + o.row;
     D.dd = h->d_dd;
   }
   if ((rc = dalloc(h, &wd.stall, 1)) || (rc = dalloc(h, &wd.half_done, np)) ||
@@ -1186,14 +1208,14 @@ template <typename T>
 void run_sweep(const SolveSet &ss, T *bp, T *cbuf, const GemvPlan<T> &plan, hipStream_t st) {
   const WorkDev &wd = *ss.wd;
   launch_permute_in(ss.perm, ss.nmat, ss.vec, bp, ss.max_elems, st);
-  launch_leaf_fwd(wd.leaves, wd.nleaves, ss.vec, bp, st);
+  launch_leaf_solve(wd.leaves, wd.nleaves, ss.vec, bp, wd.max_leaf, wd.max_lev, ss.max_rhs, false, st);
   launch_leaf_fwd_sep(wd.leaves, wd.fwg_leaf, wd.fwg_first, wd.nfwg, 4, ss.vec, bp, cbuf, ss.cb_ld,
                       st);
   launch_assemble_rhs(wd.ntarget, wd.trow_glob, wd.tptr, wd.tsrc_leaf, wd.tsrc_ri, wd.leaves,
                       ss.vec, cbuf, ss.cb_ld, bp, ss.max_rhs, st);
   for (const Launch &L : plan.fwd) run_launch<T>(L, nullptr, nullptr, nullptr, plan.d_vp, plan.d_vs, st);
   for (const Launch &L : plan.bwd) run_launch<T>(L, nullptr, nullptr, nullptr, plan.d_vp, plan.d_vs, st);
-  launch_leaf_bwd(wd.leaves, wd.nleaves, ss.vec, bp, st);
+  launch_leaf_solve(wd.leaves, wd.nleaves, ss.vec, bp, wd.max_leaf, wd.max_lev, ss.max_rhs, true, st);
   launch_permute_out(ss.perm, ss.nmat, ss.vec, bp, ss.max_elems, st);
 }
 

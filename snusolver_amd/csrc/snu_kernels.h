@@ -26,6 +26,8 @@ struct LeafDev {
   const int *rowptr, *diag, *wcol;       // LU rowptr (core_n + 1), diag, workspace column per slot
   const int *levptr, *levrows;           // factor / forward level sets
   const int *ulevptr, *ulevrows;         // backward level sets
+  const int *ftask, *btask;              // level-ordered row tasks {row, p0, p1, diag}, fwd / bwd
+  const int *ubnd;                       // per leaf row: first U entry on a boundary column
   const int *brow;                       // boundary rows (core-local)
   const int *bcol_glob, *bcol_blk, *bcol_loc;  // boundary columns: permuted index, block, offset
   void *vals;                            // leaf LU values (value type of the launch)
@@ -158,7 +160,7 @@ void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *seg
                     int total_tiles, hipStream_t st);
 template <typename T>
 void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
-                         int total_tiles, int tile, int kb, int depth, hipStream_t st);
+                         int total_tiles, int tile, int kb, int depth, int m16, hipStream_t st);
 // mode 0: y -= A x; 1: y = A x
 template <typename T>
 void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
@@ -185,6 +187,11 @@ void launch_assemble_rhs(int ntarget, const int *trow_glob, const int *tptr,
                          hipStream_t st);
 template <typename T>
 void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, hipStream_t st);
+// Leaf forward (bwd false) or backward solve with the solution in LDS;
+// leaves beyond the LDS fall back to launch_leaf_fwd / launch_leaf_bwd.
+template <typename T>
+void launch_leaf_solve(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
+                       int max_lev, int max_rhs, bool bwd, hipStream_t st);
 
 // ---- FP64 iterative refinement (one matrix per table entry) ----
 void launch_residual(const IrDev *tab, int nmat, int max_n, hipStream_t st);

@@ -47,6 +47,12 @@ __device__ __forceinline__ float rld(float v, int l) {
 __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
+// Workgroup barrier for LDS-only communication: waits for this wave's LDS
+// (and scalar) operations, not for its global loads, which stay in flight
+// across the barrier (a __syncthreads would drain them).
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -90,7 +96,7 @@ void launch_fill(T *p, long long n, hipStream_t st) {
 // pivots it has reached; the row's own flag is raised after its values are
 // stored.
 template <typename T, bool LEAF_ROW>
-__device__ void factor_row(int i, const int *__restrict__ rowptr,
+__device__ __forceinline__ void factor_row(int i, const int *__restrict__ rowptr,
                            const int *__restrict__ diag,
                            const int *__restrict__ wcol,
                            T *__restrict__ val, T *w, int lane,
@@ -111,7 +117,8 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
   wave_lds_fence();
   int ca[NCH] = {}, cb[NCH] = {};
   T ua[NCH] = {}, ub[NCH] = {};
-  auto fetch = [&](int s, int e, int(&c)[NCH], T(&u)[NCH]) {
+  // always inlined: a call would pass the register arrays through memory
+  auto fetch = [&](int s, int e, int(&c)[NCH], T(&u)[NCH]) __attribute__((always_inline)) {
     const int n = (e - s + 63) >> 6;
 #pragma unroll
     for (int j = 0; j < NCH; j++)
@@ -121,7 +128,8 @@ __device__ void factor_row(int i, const int *__restrict__ rowptr,
         u[j] = val[q];
       }
   };
-  auto apply = [&](int kt, int s, int e, T pt, const int(&c)[NCH], const T(&u)[NCH]) {
+  auto apply = [&](int kt, int s, int e, T pt, const int(&c)[NCH], const T(&u)[NCH])
+                   __attribute__((always_inline)) {
     const int n = (e - s + 63) >> 6;
     const T wk = w[kt];
     T wv[NCH];
@@ -442,10 +450,15 @@ __device__ __forceinline__ double dpp_d(double v) {
   return __hiloint2double(hi, lo);
 }
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
 // DEPTH: chunks of A and B in flight in registers ahead of the chunk being
 // multiplied (1: the next chunk's loads overlap one MFMA block; 2: loads are
 // issued two chunks ahead, alternating two register stages).
-template <typename T, int MODE, int BMB, int BNB, int KB, int DEPTH = 1>
+// M16 (FP64 only): v_mfma_f64_16x16x4f64 per 16 x 16 block, operands
+// swapped as in the FP32 path (no DPP rotations); its D[i][j] is in lane
+// j + 16 (i mod 4), register i / 4 (the FP32 instruction: lane
+// j + 16 (i / 4), register i mod 4).
+template <typename T, int MODE, int BMB, int BNB, int KB, int DEPTH = 1, bool M16 = false>
 __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ probs,
                                               int nprob,
                                               const GemmSegT<T> *__restrict__ segs) {
@@ -456,6 +469,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
   __shared__ T Bs[2][KB][BN + 1];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   int lo = 0, hi = nprob - 1;
+  if (nprob == (int)gridDim.x) lo = hi = tile;  // one tile per problem
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
     if (probs[mid].tile0 <= tile) lo = mid;
@@ -478,6 +492,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
   // FP32: one 16x16x4 product per block, 4 output registers
   double acc[BMB][BNB][4];
   f4 accf[BMB][BNB];
+  d4 accd[BMB][BNB];
 #pragma unroll
   for (int a = 0; a < BMB; a++)
 #pragma unroll
@@ -485,6 +500,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
 #pragma unroll
       for (int r = 0; r < 4; r++) acc[a][b][r] = 0.0;
       accf[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+      accd[a][b] = d4{0.0, 0.0, 0.0, 0.0};
     }
   T ra[NA], rb[NB], ra2[NA], rb2[NB];
   // Loads are unconditional and unmasked with addresses clamped into the
@@ -566,7 +582,13 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
 #pragma unroll
       for (int kq = 0; kq < KB / 4; kq++) {
         const T *bv = bvk[kq], *av = avk[kq];
-        if constexpr (F64) {
+        if constexpr (F64 && M16) {
+#pragma unroll
+          for (int a = 0; a < BMB; a++)
+#pragma unroll
+            for (int b = 0; b < BNB; b++)
+              accd[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b], av[a], accd[a][b], 0, 0, 0);
+        } else if constexpr (F64) {
           double avr[BMB][4];
 #pragma unroll
           for (int a = 0; a < BMB; a++) {
@@ -663,7 +685,12 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
       for (int r = 0; r < 4; r++) {
         int m, n;
         T v;
-        if constexpr (F64) {
+        if constexpr (F64 && M16) {
+          // D = B^T A^T: register r of lane l holds C[l % 16][4 r + l / 16]
+          m = m0 + wm + a * 16 + l16;
+          n = n0 + wn + b * 16 + 4 * r + lk;
+          v = accd[a][b][r];
+        } else if constexpr (F64) {
           m = m0 + wm + a * 16 + 4 * ((blk + r) & 3) + (lane & 3);
           n = n0 + wn + b * 16 + 4 * blk + lk;
           v = acc[a][b][r];
@@ -714,26 +741,39 @@ template <typename T>
 void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                  int total_tiles, int mode, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
+  // FP64 on the 16x16x4 instruction: measured faster than four rotated
+  // 4x4x4 products per block on large products, equal on small ones
+  constexpr bool m16 = std::is_same<T, double>::value;
   if (mode == 1)
-    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
+                       nprob, segs);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
+                       nprob, segs);
   else
-    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
+                       nprob, segs);
 }
 template <typename T>
 void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                     int total_tiles, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
-  hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  constexpr bool m16 = std::is_same<T, double>::value;
+  hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
+                     nprob, segs);
 }
 // C -= AB variants for tuning: tile 64 or 128, K chunk 16 or 32, register
 // prefetch depth 1 or 2 (the tile count is that of the tile size)
 template <typename T>
 void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
-                         int total_tiles, int tile, int kb, int depth, hipStream_t st) {
+                         int total_tiles, int tile, int kb, int depth, int m16, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
   const dim3 g(total_tiles), b(256);
+  if (m16 && std::is_same<T, double>::value) {
+    if (tile == 128) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1, true>), g, b, 0, st, probs, nprob, segs);
+    else hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 1, true>), g, b, 0, st, probs, nprob, segs);
+    return;
+  }
   if (tile == 128) {
     if (depth == 2) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 2>), g, b, 0, st, probs, nprob, segs);
     else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 32>), g, b, 0, st, probs, nprob, segs);
@@ -996,9 +1036,31 @@ __device__ __forceinline__ int col_width(int nrhs) {
   const int m = min(nrhs, 16);
   return 1 << (31 - __clz(max(m, 1)));
 }
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_v(T v) {
+  if constexpr (sizeof(T) == 8) return dpp_d<CTRL>(v);
+  else return dpp_f<CTRL>(v);
+}
+// Sum over each group of kl consecutive lanes (kl a power of two <= 64),
+// valid in the group's first lane: DPP within rows of 16 lanes (quad
+// butterflies, then rotations that bring lanes +4 and +8 of a row to its
+// front), read-lane sums across rows.
 template <typename T>
 __device__ __forceinline__ T group_sum(T v, int kl) {
-  for (int o = kl >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if (kl >= 2) v += dpp_v<0xB1>(v);   // quad_perm 1,0,3,2
+  if (kl >= 4) v += dpp_v<0x4E>(v);   // quad_perm 2,3,0,1
+  if (kl >= 8) v += dpp_v<0x12C>(v);  // row_ror:12: lane i <- lane i + 4
+  if (kl >= 16) v += dpp_v<0x128>(v); // row_ror:8: lane i <- lane i + 8
+  if (kl == 32) {
+    const T lo = rld(v, 0) + rld(v, 16), hi = rld(v, 32) + rld(v, 48);
+    v = (threadIdx.x & 63) < 32 ? lo : hi;
+  } else if (kl == 64) {
+    v = (rld(v, 0) + rld(v, 16)) + (rld(v, 32) + rld(v, 48));
+  }
   return v;
 }
 
@@ -1144,6 +1206,158 @@ template <typename T>
 void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, hipStream_t st) {
   if (nleaves <= 0) return;
   hipLaunchKernelGGL(k_leaf_bwd<T>, dim3(nleaves), dim3(256), 0, st, leaves, vd, bp);
+}
+
+// Leaf triangular solves with the leaf's solution segment in LDS: the
+// level offsets and the level-ordered row tasks {row, first entry, end,
+// diagonal} are staged in LDS once, then per level every (row, column) task
+// gathers from LDS, so a level costs one round of entry loads instead of a
+// dependent chain of index, entry and vector loads.  Backward: the boundary
+// part of every U row (entries on separator columns, whose x is final) is
+// subtracted for all rows at once before the sweep, which then reads only
+// leaf columns.  Columns go in batches of cmax.
+// LDS: [level offsets | tasks at task_off | cmax x ls values at xs_off].
+template <typename T, bool BWD>
+__global__ __launch_bounds__(256) void k_leaf_solve(const LeafDev *__restrict__ leaves,
+                                                    const VecDev *__restrict__ vd, T *bp,
+                                                    int cmax, int task_off, int xs_off) {
+  const LeafDev L = leaves[blockIdx.x];
+  const VecDev V = vd[L.vec];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, W = blockDim.x >> 6;
+  const int ls = L.leaf_size, nlev = BWD ? L.nulev : L.nlev;
+  int *lps = (int *)lds_raw;
+  int4 *tq = (int4 *)(lds_raw + task_off);
+  T *xs = (T *)(lds_raw + xs_off);
+  const int *lp = BWD ? L.ulevptr : L.levptr;
+  const int4 *tk = (const int4 *)(BWD ? L.btask : L.ftask);
+  for (int e = tid; e <= nlev; e += blockDim.x) lps[e] = lp[e];
+  for (int e = tid; e < ls; e += blockDim.x) tq[e] = tk[e];
+  const T *v = (const T *)L.vals;
+  const int *wc = L.wcol;
+  for (int c0 = 0; c0 < V.nrhs; c0 += cmax) {
+    const int nc = min(cmax, V.nrhs - c0);
+    __syncthreads();
+    for (int e = tid; e < nc * ls; e += blockDim.x) {
+      const int c = e / ls, i = e - c * ls;
+      xs[e] = bp[V.base + (long long)(c0 + c) * V.ld + L.gstart + i];
+    }
+    __syncthreads();
+    const int cw = col_width(nc), kl = 64 / cw, ng = (nc + cw - 1) / cw;
+    const int sub = lane % kl, cl = lane / kl;
+    if (BWD) {
+      for (int t = wave; t < ls * ng; t += W) {
+        const int i = t % ls, col = (t / ls) * cw + cl, cc = min(col, nc - 1);
+        const T *b = bp + V.base + (long long)(c0 + cc) * V.ld;
+        T s = T(0);
+        const int pe = L.rowptr[i + 1];
+        for (int p = L.ubnd[i] + sub; p < pe; p += 8 * kl) {
+          // eight chunks of loads in flight (clamped addresses, masked use)
+          T vv[8];
+          int gg[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const int pc = min(p + j * kl, pe - 1);
+            vv[j] = v[pc];
+            gg[j] = L.bcol_glob[wc[pc] - ls];
+          }
+#pragma unroll
+          for (int j = 0; j < 8; j++)
+            if (p + j * kl < pe) s += vv[j] * b[gg[j]];
+        }
+        s = group_sum(s, kl);
+        if (col < nc && sub == 0) xs[cc * ls + i] -= s;
+      }
+      __syncthreads();
+    }
+    // the first chunk of each wave's first task of the next level is loaded
+    // while the current level is solved
+    auto first_chunk = [&](int lev, T &pv, int &pw) {
+      pv = T(0);
+      pw = 0;
+      if (lev >= nlev) return;
+      const int r0 = lps[lev], nr = lps[lev + 1] - r0;
+      if (wave >= nr * ng) return;
+      const int4 q = tq[r0 + wave % nr];
+      const int p = q.y + sub;
+      if (p < q.z) pv = v[p], pw = wc[p];
+    };
+    // one level; (cv, cwi) hold this level's first chunk, (nv, nw) receive
+    // the next level's (two register sets alternate: no copy waits on loads)
+    auto level = [&](int lev, T cv, int cwi, T &nv, int &nw) {
+      const int r0 = lps[lev], nr = lps[lev + 1] - r0;
+      first_chunk(lev + 1, nv, nw);
+      for (int t = wave; t < nr * ng; t += W) {
+        const int4 q = tq[r0 + t % nr];
+        const int col = (t / nr) * cw + cl, cc = min(col, nc - 1);
+        const T *x = xs + cc * ls;
+        T s = T(0);
+        int p = q.y + sub;
+        if (t == wave) {  // first chunk already in registers
+          if (p < q.z) s = cv * x[cwi];
+          p += kl;
+        }
+        for (; p < q.z; p += 8 * kl) {
+          // eight chunks of loads in flight (clamped addresses, masked use)
+          T vv[8];
+          int ww[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const int pc = min(p + j * kl, q.z - 1);
+            vv[j] = v[pc];
+            ww[j] = wc[pc];
+          }
+#pragma unroll
+          for (int j = 0; j < 8; j++)
+            if (p + j * kl < q.z) s += vv[j] * x[ww[j]];
+        }
+        s = group_sum(s, kl);
+        if (col < nc && sub == 0) {
+          T *xi = xs + cc * ls + q.x;
+          if (BWD) *xi = (*xi - s) / v[q.w];
+          else *xi -= s;
+        }
+      }
+      lds_barrier();
+    };
+    T va, vb;
+    int wa, wb;
+    first_chunk(0, va, wa);
+    for (int lev = 0; lev < nlev; lev += 2) {
+      level(lev, va, wa, vb, wb);
+      if (lev + 1 < nlev) level(lev + 1, vb, wb, va, wa);
+    }
+    for (int e = tid; e < nc * ls; e += blockDim.x) {
+      const int c = e / ls, i = e - c * ls;
+      bp[V.base + (long long)(c0 + c) * V.ld + L.gstart + i] = xs[e];
+    }
+  }
+}
+template <typename T>
+void launch_leaf_solve(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
+                       int max_lev, int max_rhs, bool bwd, hipStream_t st) {
+  if (nleaves <= 0) return;
+  constexpr int kLds = 160 * 1024;
+  const int task_off = ((max_lev + 1) * 4 + 15) & ~15;
+  const int xs_off = task_off + max_leaf * 16;
+  const int col_bytes = max_leaf * (int)sizeof(T);
+  if (xs_off + col_bytes > kLds || max_rhs < 1) {  // leaves too large for LDS
+    if (bwd) launch_leaf_bwd(leaves, nleaves, vd, bp, st);
+    else launch_leaf_fwd(leaves, nleaves, vd, bp, st);
+    return;
+  }
+  const int cmax = std::max(1, std::min(std::min(max_rhs, 16), (kLds - xs_off) / std::max(col_bytes, 1)));
+  const size_t lds = (size_t)xs_off + (size_t)cmax * col_bytes;
+  if (bwd) {
+    (void)hipFuncSetAttribute((const void *)k_leaf_solve<T, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_leaf_solve<T, true>), dim3(nleaves), dim3(256), lds, st, leaves, vd, bp,
+                       cmax, task_off, xs_off);
+  } else {
+    (void)hipFuncSetAttribute((const void *)k_leaf_solve<T, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_leaf_solve<T, false>), dim3(nleaves), dim3(256), lds, st, leaves, vd, bp,
+                       cmax, task_off, xs_off);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1332,7 +1546,7 @@ int device_lds_limit() {
   template void launch_gemm128<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,       \
                                   hipStream_t);                                               \
   template void launch_gemm_variant<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,  \
-                                       int, int, int, hipStream_t);                           \
+                                       int, int, int, int, hipStream_t);                      \
   template void launch_gemv<T>(const GemvProbT<T> *, int, const GemvSegT<T> *, int, int,     \
                                hipStream_t);                                                  \
   template void launch_diag_lu<T>(const DiagProbT<T> *, int, hipStream_t);                   \
@@ -1346,7 +1560,9 @@ int device_lds_limit() {
   template void launch_assemble_rhs<T>(int, const int *, const int *, const int *,            \
                                        const int *, const LeafDev *, const VecDev *,          \
                                        const T *, int, T *, int, hipStream_t);                \
-  template void launch_leaf_bwd<T>(const LeafDev *, int, const VecDev *, T *, hipStream_t);
+  template void launch_leaf_bwd<T>(const LeafDev *, int, const VecDev *, T *, hipStream_t);   \
+  template void launch_leaf_solve<T>(const LeafDev *, int, const VecDev *, T *, int, int, int,  \
+                                     bool, hipStream_t);
 SNU_INSTANTIATE(double)
 SNU_INSTANTIATE(float)
 #undef SNU_INSTANTIATE
