@@ -102,21 +102,21 @@ __device__ __forceinline__ void factor_row(int i, const int *__restrict__ rowptr
                            T *__restrict__ val, T *w, int lane,
                            T eps, volatile unsigned char *done, int *stall) {
 #pragma clang fp contract(off)
-  // Pivot rows are streamed one pivot ahead: while pivot t is applied, the
-  // first NCH chunks (64 entries each) of pivot t+1's U row are in flight
-  // into the other of two register sets (chunks past the row's end are not
-  // loaded; lanes past it read a clamped entry that is never used).  Longer
-  // U rows fall back to loads inside the update.  A pivot's U entries name
-  // distinct workspace columns, so its update gathers every workspace value
-  // it touches (one LDS round trip, overlapping the division) before it
-  // writes any of them back.
-  constexpr int NCH = 16;
+  // Pivot rows are streamed three pivots ahead: while pivot t is applied,
+  // the first NCH chunks (64 entries each) of the U rows of pivots t+1..t+3
+  // are in flight into a ring of four register sets (chunks past a row's
+  // end are not loaded; lanes past it read a clamped entry that is never
+  // used).  Longer U rows fall back to loads inside the update.  A pivot's U
+  // entries name distinct workspace columns, so its update gathers every
+  // workspace value it touches (one LDS round trip, overlapping the
+  // division) before it writes any of them back.
+  constexpr int NCH = 4;
   const int p0 = rowptr[i], p1 = rowptr[i + 1];
   const int pd = LEAF_ROW ? diag[i] : p1;
   for (int p = p0 + lane; p < p1; p += 64) w[wcol[p]] = val[p];
   wave_lds_fence();
-  int ca[NCH] = {}, cb[NCH] = {};
-  T ua[NCH] = {}, ub[NCH] = {};
+  int ca[NCH] = {}, cb[NCH] = {}, cc[NCH] = {}, cd[NCH] = {};
+  T ua[NCH] = {}, ub[NCH] = {}, uc[NCH] = {}, ud[NCH] = {};
   // always inlined: a call would pass the register arrays through memory
   auto fetch = [&](int s, int e, int(&c)[NCH], T(&u)[NCH]) __attribute__((always_inline)) {
     const int n = (e - s + 63) >> 6;
@@ -184,14 +184,22 @@ __device__ __forceinline__ void factor_row(int i, const int *__restrict__ rowptr
       ke = rowptr[k + 1];
       piv = val[kd];
     }
-    // pivot t of the batch is held in (ca, ua) for even t, (cb, ub) for odd t
+    // pivot t of the batch is held in set t mod 4 of (a, b, c, d)
     fetch(rl(ks, 0), rl(ke, 0), ca, ua);
-    for (int t = 0; t < cnt; t += 2) {
-      if (t + 1 < cnt) fetch(rl(ks, t + 1), rl(ke, t + 1), cb, ub);
+    if (1 < cnt) fetch(rl(ks, 1), rl(ke, 1), cb, ub);
+    if (2 < cnt) fetch(rl(ks, 2), rl(ke, 2), cc, uc);
+    for (int t = 0; t < cnt; t += 4) {
+      if (t + 3 < cnt) fetch(rl(ks, t + 3), rl(ke, t + 3), cd, ud);
       apply(rl(k, t), rl(ks, t), rl(ke, t), rld(piv, t), ca, ua);
       if (t + 1 >= cnt) break;
-      if (t + 2 < cnt) fetch(rl(ks, t + 2), rl(ke, t + 2), ca, ua);
+      if (t + 4 < cnt) fetch(rl(ks, t + 4), rl(ke, t + 4), ca, ua);
       apply(rl(k, t + 1), rl(ks, t + 1), rl(ke, t + 1), rld(piv, t + 1), cb, ub);
+      if (t + 2 >= cnt) break;
+      if (t + 5 < cnt) fetch(rl(ks, t + 5), rl(ke, t + 5), cb, ub);
+      apply(rl(k, t + 2), rl(ks, t + 2), rl(ke, t + 2), rld(piv, t + 2), cc, uc);
+      if (t + 3 >= cnt) break;
+      if (t + 6 < cnt) fetch(rl(ks, t + 6), rl(ke, t + 6), cc, uc);
+      apply(rl(k, t + 3), rl(ks, t + 3), rl(ke, t + 3), rld(piv, t + 3), cd, ud);
     }
     pb += cnt;
   }
@@ -434,11 +442,19 @@ void launch_assemble_schur(int ntile, const int *tile_blk, const int *tile_row0,
 // width) or B (M <= tile height): a workgroup reads its whole A row panel /
 // B column panel before its epilogue writes.
 // Workgroup ids are remapped so that each XCD (ids go round-robin over the 8
-// XCDs) works on a contiguous run of tiles, which share operand panels in
+// XCDs) works on runs of consecutive tiles, which share operand panels in
 // that XCD's L2.
+// Chunks of kXcdChunk consecutive tiles go to the XCDs in turn (XCD x runs
+// chunks x, x + 8, x + 16, ...): neighbouring tiles still share an L2, and
+// tiles of unequal K (problems of one launch differ) spread evenly over the
+// XCDs instead of one XCD receiving a whole run of long tiles.  The last
+// partial round of 8 chunks keeps the identity order.
+constexpr int kXcdChunk = 32;
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
-  const int q = nb >> 3, r = nb & 7, x = bid & 7, i = bid >> 3;
-  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+  const int full = nb / (8 * kXcdChunk) * (8 * kXcdChunk);
+  if (bid >= full) return bid;
+  const int x = bid & 7, i = bid >> 3;
+  return ((i / kXcdChunk) * 8 + x) * kXcdChunk + (i % kXcdChunk);
 }
 // row_ror within rows of 16 lanes reads a valid lane for every lane, so the
 // destination's old value is never kept (no zeroing needed)
@@ -613,6 +629,43 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
         }
       }
   };
+  const int blk = (lane >> 2) & 3;
+  // element (a, b, r) of this lane's accumulators: its place in C and value
+  auto elem = [&](int a, int b, int r, int &m, int &n, T &v) __attribute__((always_inline)) {
+    if constexpr (F64 && M16) {
+      // D = B^T A^T: register r of lane l holds C[l % 16][4 r + l / 16]
+      m = m0 + wm + a * 16 + l16;
+      n = n0 + wn + b * 16 + 4 * r + lk;
+      v = accd[a][b][r];
+    } else if constexpr (F64) {
+      m = m0 + wm + a * 16 + 4 * ((blk + r) & 3) + (lane & 3);
+      n = n0 + wn + b * 16 + 4 * blk + lk;
+      v = acc[a][b][r];
+    } else {
+      // D = B^T A^T: register r of lane l holds C[l % 16][4 (l / 16) + r]
+      m = m0 + wm + a * 16 + l16;
+      n = n0 + wn + b * 16 + 4 * lk + r;
+      v = accf[a][b][r];
+    }
+  };  // C -= AB: after the K loop every C value of the tile is loaded before any
+  // is stored, so the loads are in flight together (one round trip, not one
+  // per element)
+  T cold[BMB][BNB][4];
+  auto load_c = [&]() __attribute__((always_inline)) {
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int a = 0; a < BMB; a++)
+#pragma unroll
+        for (int b = 0; b < BNB; b++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            int m, n;
+            T v;
+            elem(a, b, r, m, n, v);
+            cold[a][b][r] = (m < P.M && n < P.N) ? gptr(P.C)[m + (long long)n * P.ldc] : T(0);
+          }
+    }
+  };
   int sg = 0, k0 = 0;
   if (P.nseg == 0) return;
   if constexpr (DEPTH == 1) {
@@ -676,7 +729,8 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
       }
     }
   }
-  const int blk = (lane >> 2) & 3;
+  load_c();
+
 #pragma unroll
   for (int a = 0; a < BMB; a++)
 #pragma unroll
@@ -685,26 +739,12 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
       for (int r = 0; r < 4; r++) {
         int m, n;
         T v;
-        if constexpr (F64 && M16) {
-          // D = B^T A^T: register r of lane l holds C[l % 16][4 r + l / 16]
-          m = m0 + wm + a * 16 + l16;
-          n = n0 + wn + b * 16 + 4 * r + lk;
-          v = accd[a][b][r];
-        } else if constexpr (F64) {
-          m = m0 + wm + a * 16 + 4 * ((blk + r) & 3) + (lane & 3);
-          n = n0 + wn + b * 16 + 4 * blk + lk;
-          v = acc[a][b][r];
-        } else {
-          // D = B^T A^T: register r of lane l holds C[l % 16][4 (l / 16) + r]
-          m = m0 + wm + a * 16 + l16;
-          n = n0 + wn + b * 16 + 4 * lk + r;
-          v = accf[a][b][r];
-        }
+        elem(a, b, r, m, n, v);
         if (m < P.M && n < P.N) {
           auto c = gptr(P.C) + m + (long long)n * P.ldc;
           if (MODE == 1) *c = v;
           else if (MODE == 2) *c = -v;
-          else *c = *c - v;
+          else *c = cold[a][b][r] - v;
         }
       }
 }
@@ -1208,7 +1248,8 @@ void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp
   hipLaunchKernelGGL(k_leaf_bwd<T>, dim3(nleaves), dim3(256), 0, st, leaves, vd, bp);
 }
 
-// Leaf triangular solves with the leaf's solution segment in LDS: the
+// Leaf triangular solves with the leaf's solution segment in LDS (used when
+// every leaf of the launch is resident at once): the
 // level offsets and the level-ordered row tasks {row, first entry, end,
 // diagonal} are staged in LDS once, then per level every (row, column) task
 // gathers from LDS, so a level costs one round of entry loads instead of a
@@ -1347,17 +1388,25 @@ void launch_leaf_solve(const LeafDev *leaves, int nleaves, const VecDev *vd, T *
   }
   const int cmax = std::max(1, std::min(std::min(max_rhs, 16), (kLds - xs_off) / std::max(col_bytes, 1)));
   const size_t lds = (size_t)xs_off + (size_t)cmax * col_bytes;
-  if (bwd) {
-    (void)hipFuncSetAttribute((const void *)k_leaf_solve<T, true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  // many leaves (batches): the LDS sized for the largest would leave most
+  // of them waiting for a CU; the global-memory kernels keep them resident
+  const void *kern = bwd ? (const void *)k_leaf_solve<T, true> : (const void *)k_leaf_solve<T, false>;
+  (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int dev = 0, cus = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds);
+  if ((long long)nleaves > (long long)cus * std::max(per_cu, 1)) {
+    if (bwd) launch_leaf_bwd(leaves, nleaves, vd, bp, st);
+    else launch_leaf_fwd(leaves, nleaves, vd, bp, st);
+    return;
+  }
+  if (bwd)
     hipLaunchKernelGGL((k_leaf_solve<T, true>), dim3(nleaves), dim3(256), lds, st, leaves, vd, bp,
                        cmax, task_off, xs_off);
-  } else {
-    (void)hipFuncSetAttribute((const void *)k_leaf_solve<T, false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  else
     hipLaunchKernelGGL((k_leaf_solve<T, false>), dim3(nleaves), dim3(256), lds, st, leaves, vd, bp,
                        cmax, task_off, xs_off);
-  }
 }
 
 // ---------------------------------------------------------------------------
