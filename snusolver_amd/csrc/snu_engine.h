@@ -32,6 +32,8 @@ struct Launch {
   int p0, np, tiles;
   double flops;
   long long key;
+  int wgs = 0;          // GEMV: workgroups of the split-K partial pass (0: unsplit)
+  long long part = 0;   // GEMV: partial-sum values of the launch
 };
 inline bool is_gemm(int kind) { return kind == 1 || kind == 2 || kind == 5 || kind == 6; }
 
@@ -53,7 +55,8 @@ struct GemvPlan {
   std::vector<Launch> fwd, bwd;
   GemvProbT<T> *d_vp = nullptr;
   GemvSegT<T> *d_vs = nullptr;
-  size_t cap_vp = 0, cap_vs = 0;  // device capacities (grow-only)
+  T *d_part = nullptr;            // split-K partial sums (launches run in turn)
+  size_t cap_vp = 0, cap_vs = 0, cap_part = 0;  // device capacities (grow-only)
 };
 
 // Value-typed device state of one matrix (FP64, or FP32 refined in FP64).

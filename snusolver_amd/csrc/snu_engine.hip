@@ -620,8 +620,34 @@ int upload_gemv_plan(GemvPlan<T> &g, hipStream_t st) {
     cap = c;
     return SNU_OK;
   };
+  // split-K layout: pieces per problem from its longest segment (the
+  // problem alone decides), workgroup and partial-sum offsets per launch;
+  // a launch whose problems are all single-piece runs unsplit
+  size_t need_part = 0;
+  for (std::vector<Launch> *ls : {&g.fwd, &g.bwd})
+    for (Launch &L : *ls) {
+      int wgs = 0, split = 0;
+      long long off = 0;
+      for (int q = 0; q < L.np; q++) {
+        GemvProbT<T> &p = g.vp[L.p0 + q];
+        int kmax = 1;
+        for (int j = 0; j < p.nseg; j++) kmax = std::max(kmax, g.vs[p.seg0 + j].K);
+        p.parts = (kmax + kGemvPiece - 1) / kGemvPiece;
+        p.wg0 = wgs;
+        p.poff = off;
+        const int tiles = (p.M + 63) / 64;
+        wgs += tiles * p.parts;
+        off += (long long)tiles * p.parts * p.nrhs * 64;
+        split |= p.parts > 1;
+      }
+      L.wgs = split ? wgs : 0;
+      L.part = split ? off : 0;
+      need_part = std::max(need_part, (size_t)L.part);
+    }
   int rc;
-  if ((rc = grow(&g.d_vp, g.cap_vp, g.vp.size())) || (rc = grow(&g.d_vs, g.cap_vs, g.vs.size()))) return rc;
+  if ((rc = grow(&g.d_vp, g.cap_vp, g.vp.size())) || (rc = grow(&g.d_vs, g.cap_vs, g.vs.size())) ||
+      (need_part && (rc = grow(&g.d_part, g.cap_part, need_part))))
+    return rc;
   if (!g.vp.empty())
     HIPCHK(hipMemcpyAsync(g.d_vp, g.vp.data(), g.vp.size() * sizeof(g.vp[0]), hipMemcpyHostToDevice, st));
   if (!g.vs.empty())
@@ -632,6 +658,7 @@ template <typename T>
 void free_gemv_plan(GemvPlan<T> &g) {
   if (g.d_vp) (void)hipFree(g.d_vp);
   if (g.d_vs) (void)hipFree(g.d_vs);
+  if (g.d_part) (void)hipFree(g.d_part);
   g = GemvPlan<T>();
 }
 template <typename T>
@@ -1094,7 +1121,7 @@ namespace {
 template <typename T>
 void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
                 const DiagProbT<T> *dp, const GemvProbT<T> *vp, const GemvSegT<T> *vs,
-                hipStream_t st) {
+                T *part, hipStream_t st) {
   switch (L.kind) {
     case 0: launch_diag_lu(dp + L.p0, L.np, st); break;
     case 1: launch_gemm(gp + L.p0, L.np, gs, L.tiles, 1, st); break;
@@ -1102,8 +1129,11 @@ void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
     case 3: launch_copy2d(gp + L.p0, L.np, gs, L.tiles, st); break;
     case 5: launch_gemm128(gp + L.p0, L.np, gs, L.tiles, st); break;
     case 6: launch_gemm(gp + L.p0, L.np, gs, L.tiles, 2, st); break;
-    case 7: launch_gemv(vp + L.p0, L.np, vs, L.tiles, 1, st); break;
-    case 8: launch_gemv(vp + L.p0, L.np, vs, L.tiles, 0, st); break;
+    case 7:
+    case 8:
+      if (L.wgs > 0) launch_gemv_split(vp + L.p0, L.np, vs, L.tiles, L.wgs, L.kind == 7 ? 1 : 0, part, st);
+      else launch_gemv(vp + L.p0, L.np, vs, L.tiles, L.kind == 7 ? 1 : 0, st);
+      break;
   }
 }
 }  // namespace
@@ -1143,7 +1173,7 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
   for (const Launch &L : plan.launches) {
     const bool gm = is_gemm(L.kind);
     if (profile && pev && gm) HIPCHK(hipEventRecord((*pev)[2 * g], st));
-    run_launch<T>(L, plan.d_gp, plan.d_gs, plan.d_dp, nullptr, nullptr, st);
+    run_launch<T>(L, plan.d_gp, plan.d_gs, plan.d_dp, nullptr, nullptr, nullptr, st);
     if (profile && pev && gm) {
       HIPCHK(hipEventRecord((*pev)[2 * g + 1], st));
       g++;
@@ -1213,8 +1243,10 @@ void run_sweep(const SolveSet &ss, T *bp, T *cbuf, const GemvPlan<T> &plan, hipS
                       st);
   launch_assemble_rhs(wd.ntarget, wd.trow_glob, wd.tptr, wd.tsrc_leaf, wd.tsrc_ri, wd.leaves,
                       ss.vec, cbuf, ss.cb_ld, bp, ss.max_rhs, st);
-  for (const Launch &L : plan.fwd) run_launch<T>(L, nullptr, nullptr, nullptr, plan.d_vp, plan.d_vs, st);
-  for (const Launch &L : plan.bwd) run_launch<T>(L, nullptr, nullptr, nullptr, plan.d_vp, plan.d_vs, st);
+  for (const Launch &L : plan.fwd)
+    run_launch<T>(L, nullptr, nullptr, nullptr, plan.d_vp, plan.d_vs, plan.d_part, st);
+  for (const Launch &L : plan.bwd)
+    run_launch<T>(L, nullptr, nullptr, nullptr, plan.d_vp, plan.d_vs, plan.d_part, st);
   launch_leaf_solve(wd.leaves, wd.nleaves, ss.vec, bp, wd.max_leaf, wd.max_lev, ss.max_rhs, true, st);
   launch_permute_out(ss.perm, ss.nmat, ss.vec, bp, ss.max_elems, st);
 }

@@ -110,7 +110,13 @@ struct GemvProbT {
   T *y;
   long long ldy;
   int M, seg0, nseg, tri, tile0, nrhs;
+  // split K: each segment in pieces of kGemvPiece columns, one workgroup per
+  // (64-row tile, piece) writing partial sums at poff; wg0 numbers the
+  // problem's first workgroup in its launch
+  int parts, wg0;
+  long long poff;
 };
+constexpr int kGemvPiece = 512;
 using GemmSeg = GemmSegT<double>;
 using GemmProb = GemmProbT<double>;
 using DiagProb = DiagProbT<double>;
@@ -162,6 +168,11 @@ template <typename T>
 void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                          int total_tiles, int tile, int kb, int depth, int m16, hipStream_t st);
 // mode 0: y -= A x; 1: y = A x
+// Split-K GEMV: partial sums per (tile, piece) into part, then a fixed-order
+// sum per row applied to y (mode 1: y = sum, 0: y -= sum).
+template <typename T>
+void launch_gemv_split(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
+                       int total_tiles, int total_wgs, int mode, T *part, hipStream_t st);
 template <typename T>
 void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
                  int total_tiles, int mode, hipStream_t st);

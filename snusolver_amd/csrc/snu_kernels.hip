@@ -95,22 +95,21 @@ void launch_fill(T *p, long long n, hipStream_t st) {
 // concurrently on the waves of the workgroup, each waiting only for the
 // pivots it has reached; the row's own flag is raised after its values are
 // stored.
-template <typename T, bool LEAF_ROW>
+template <typename T, bool LEAF_ROW, int NCH, int NST>
 __device__ __forceinline__ void factor_row(int i, const int *__restrict__ rowptr,
                            const int *__restrict__ diag,
                            const int *__restrict__ wcol,
                            T *__restrict__ val, T *w, int lane,
                            T eps, volatile unsigned char *done, int *stall) {
 #pragma clang fp contract(off)
-  // Pivot rows are streamed three pivots ahead: while pivot t is applied,
-  // the first NCH chunks (64 entries each) of the U rows of pivots t+1..t+3
-  // are in flight into a ring of four register sets (chunks past a row's
-  // end are not loaded; lanes past it read a clamped entry that is never
-  // used).  Longer U rows fall back to loads inside the update.  A pivot's U
+  // Pivot rows are streamed NST - 1 pivots ahead: while pivot t is applied,
+  // the first NCH chunks (64 entries each) of the U rows of the next pivots
+  // are in flight into a ring of NST (2 or 4) register sets (chunks past a
+  // row's end are not loaded; lanes past it read a clamped entry that is
+  // never used).  Longer U rows are loaded inside the update.  A pivot's U
   // entries name distinct workspace columns, so its update gathers every
   // workspace value it touches (one LDS round trip, overlapping the
   // division) before it writes any of them back.
-  constexpr int NCH = 4;
   const int p0 = rowptr[i], p1 = rowptr[i + 1];
   const int pd = LEAF_ROW ? diag[i] : p1;
   for (int p = p0 + lane; p < p1; p += 64) w[wcol[p]] = val[p];
@@ -184,8 +183,19 @@ __device__ __forceinline__ void factor_row(int i, const int *__restrict__ rowptr
       ke = rowptr[k + 1];
       piv = val[kd];
     }
-    // pivot t of the batch is held in set t mod 4 of (a, b, c, d)
+    // pivot t of the batch is held in set t mod NST of (a, b, c, d)
     fetch(rl(ks, 0), rl(ke, 0), ca, ua);
+    if constexpr (NST == 2) {
+      for (int t = 0; t < cnt; t += 2) {
+        if (t + 1 < cnt) fetch(rl(ks, t + 1), rl(ke, t + 1), cb, ub);
+        apply(rl(k, t), rl(ks, t), rl(ke, t), rld(piv, t), ca, ua);
+        if (t + 1 >= cnt) break;
+        if (t + 2 < cnt) fetch(rl(ks, t + 2), rl(ke, t + 2), ca, ua);
+        apply(rl(k, t + 1), rl(ks, t + 1), rl(ke, t + 1), rld(piv, t + 1), cb, ub);
+      }
+      pb += cnt;
+      continue;
+    }
     if (1 < cnt) fetch(rl(ks, 1), rl(ke, 1), cb, ub);
     if (2 < cnt) fetch(rl(ks, 2), rl(ke, 2), cc, uc);
     for (int t = 0; t < cnt; t += 4) {
@@ -261,7 +271,7 @@ __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ l
   T *v = (T *)L.vals;
   if (role == 0) {
     for (int r = sa + wave; r < sb; r += W)
-      factor_row<T, true>(r, rp, dg, wc, v, w, lane, eps, done, stall);
+      factor_row<T, true, 4, 4>(r, rp, dg, wc, v, w, lane, eps, done, stall);
     __syncthreads();
     if (threadIdx.x == 0) {
       __threadfence();
@@ -270,7 +280,7 @@ __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ l
     return;
   }
   for (int r = wave; r < sa; r += W)
-    factor_row<T, true>(r, rp, dg, wc, v, w, lane, eps, done, stall);
+    factor_row<T, true, 4, 4>(r, rp, dg, wc, v, w, lane, eps, done, stall);
   if (sa < sb) {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -290,7 +300,7 @@ __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ l
     __syncthreads();
   }
   for (int r = sb + wave; r < L.leaf_size; r += W)
-    factor_row<T, true>(r, rp, dg, wc, v, w, lane, eps, done, stall);
+    factor_row<T, true, 4, 4>(r, rp, dg, wc, v, w, lane, eps, done, stall);
 }
 // LDS bytes of the completion flags for leaves of up to max_leaf rows
 int leaf_flag_bytes(int max_leaf) { return (max_leaf + 15) & ~15; }
@@ -331,7 +341,7 @@ __global__ __launch_bounds__(512) void k_leaf_sep(const LeafDev *__restrict__ le
   const int ls = L.leaf_size;
   for (int c = lane; c < ls + L.nbc; c += 64) w[c] = T(0);
   wave_lds_fence();
-  factor_row<T, false>(L.brow[ri], L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0), nullptr,
+  factor_row<T, false, 16, 2>(L.brow[ri], L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0), nullptr,
                        nullptr);
   T *Srow = (T *)L.S + (long long)ri * L.nbc;
   for (int j = lane; j < L.nbc; j += 64) {
@@ -894,6 +904,107 @@ __global__ __launch_bounds__(256) void k_gemv(const GemvProbT<T> *__restrict__ p
     }
     __syncthreads();
   }
+}
+// Split-K GEMV, pass 1: workgroup (problem, tile lt, piece) sums
+// A[rows of lt][k] x[k] over k in [piece * kGemvPiece, (piece + 1) *
+// kGemvPiece) of every segment (within the triangle for tri problems), in
+// the order of the unsplit kernel, and stores the 64 x nrhs partial block.
+// Pass 2 adds the pieces in ascending order.  The decomposition depends on
+// the problem alone, so a problem gives the same bits in any launch.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_part(const GemvProbT<T> *__restrict__ probs,
+                                                   int nprob,
+                                                   const GemvSegT<T> *__restrict__ segs,
+                                                   T *__restrict__ part) {
+  constexpr int CG = 16;
+  __shared__ T red[4][CG][64];
+  const int b = blockIdx.x;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].wg0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemvProbT<T> P = probs[lo];
+  const int local = b - P.wg0, lt = local / P.parts, pc = local % P.parts;
+  const int nrhs = P.nrhs;
+  const int m0 = lt * 64;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int m = m0 + lane, mc = min(m, P.M - 1);
+  T *out = part + P.poff + (long long)(lt * P.parts + pc) * nrhs * 64;
+  for (int c0 = 0; c0 < nrhs; c0 += CG) {
+    const int nc = min(CG, nrhs - c0);
+    T acc[CG];
+#pragma unroll
+    for (int c = 0; c < CG; c++) acc[c] = T(0);
+    for (int sg = 0; sg < P.nseg; sg++) {
+      const GemvSegT<T> S = segs[P.seg0 + sg];
+      int kb = 0, ke = S.K;
+      if (P.tri == 1) ke = min(S.K, m0 + 64);
+      if (P.tri == 2) kb = m0;
+      kb = max(kb, pc * kGemvPiece);
+      ke = min(ke, (pc + 1) * kGemvPiece);
+      auto A = gptr(S.A) + mc;
+      auto X = gptr(S.x) + (long long)c0 * S.ldx;
+      int k = kb + w;
+      for (; k + 12 < ke; k += 16) {
+        T a0 = A[(long long)k * S.lda], a1 = A[(long long)(k + 4) * S.lda];
+        T a2 = A[(long long)(k + 8) * S.lda], a3 = A[(long long)(k + 12) * S.lda];
+#pragma unroll
+        for (int c = 0; c < CG; c++)
+          if (c < nc) {
+            auto xc = X + (long long)c * S.ldx;
+            acc[c] += a0 * xc[k] + a1 * xc[k + 4] + a2 * xc[k + 8] + a3 * xc[k + 12];
+          }
+      }
+      for (; k < ke; k += 4) {
+        T a0 = A[(long long)k * S.lda];
+#pragma unroll
+        for (int c = 0; c < CG; c++)
+          if (c < nc) acc[c] += a0 * X[(long long)c * S.ldx + k];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CG; c++) red[w][c][lane] = acc[c];
+    __syncthreads();
+    if (w == 0)
+      for (int c = 0; c < nc; c++)
+        out[(c0 + c) * 64 + lane] = red[0][c][lane] + red[1][c][lane] + red[2][c][lane] + red[3][c][lane];
+    __syncthreads();
+  }
+}
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_gemv_sum(const GemvProbT<T> *__restrict__ probs, int nprob,
+                                                  const T *__restrict__ part) {
+  const int tile = blockIdx.x;
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemvProbT<T> P = probs[lo];
+  const int lt = tile - P.tile0, m = lt * 64 + (threadIdx.x & 63);
+  if (m >= P.M) return;
+  const T *in = part + P.poff + (long long)lt * P.parts * P.nrhs * 64 + (threadIdx.x & 63);
+  for (int c = threadIdx.x >> 6; c < P.nrhs; c += 4) {
+    T v = in[c * 64];
+    for (int pc = 1; pc < P.parts; pc++) v += in[((long long)pc * P.nrhs + c) * 64];
+    auto y = gptr(P.y) + m + (long long)c * P.ldy;
+    if (MODE == 1) *y = v;
+    else *y = *y - v;
+  }
+}
+template <typename T>
+void launch_gemv_split(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
+                       int total_tiles, int total_wgs, int mode, T *part, hipStream_t st) {
+  if (total_tiles <= 0 || nprob <= 0) return;
+  hipLaunchKernelGGL(k_gemv_part<T>, dim3(total_wgs), dim3(256), 0, st, probs, nprob, segs, part);
+  if (mode == 1)
+    hipLaunchKernelGGL((k_gemv_sum<T, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, part);
+  else
+    hipLaunchKernelGGL((k_gemv_sum<T, 0>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, part);
 }
 template <typename T>
 void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
@@ -1596,6 +1707,8 @@ int device_lds_limit() {
                                   hipStream_t);                                               \
   template void launch_gemm_variant<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,  \
                                        int, int, int, int, hipStream_t);                      \
+  template void launch_gemv_split<T>(const GemvProbT<T> *, int, const GemvSegT<T> *, int, int,  \
+                                     int, T *, hipStream_t);                                    \
   template void launch_gemv<T>(const GemvProbT<T> *, int, const GemvSegT<T> *, int, int,     \
                                hipStream_t);                                                  \
   template void launch_diag_lu<T>(const DiagProbT<T> *, int, hipStream_t);                   \
