@@ -391,6 +391,8 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   wd.nfwg = (int)W.fwg_leaf.size();
   wd.ntarget = (int)W.trow_glob.size();
   wd.ntile = (int)W.tile_blk.size();
+  wd.max_nt = 0;
+  for (snumat *h : b->mats) wd.max_nt = std::max(wd.max_nt, h->wd.max_nt);
   wd.cb_total = cb;
   wd.wsz = wsz;
   wd.waves = waves;
@@ -411,6 +413,25 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
       (rc = bupload(b, &wd.seg_ri, W.seg_ri)) ||
       (rc = bupload(b, &b->d_ir_a, ira)))
     return fail(rc);
+  // leaf-tail GEMM problems over the concatenated leaves
+  {
+    auto tails = [&](auto tag) -> int {
+      using T = decltype(tag);
+      std::vector<GemmProbT<T>> gp;
+      std::vector<GemmSegT<T>> gs;
+      int tiles = 0, r;
+      tail_gemm_list<T>(leaves, gp, gs, tiles);
+      GemmProbT<T> *dgp = nullptr;
+      GemmSegT<T> *dgs = nullptr;
+      if (!gp.empty() && ((r = bupload(b, &dgp, gp)) || (r = bupload(b, &dgs, gs)))) return r;
+      wd.tail_gp = dgp;
+      wd.tail_gs = dgs;
+      wd.tail_np = (int)gp.size();
+      wd.tail_tiles = tiles;
+      return SNU_OK;
+    };
+    if ((rc = b->prec == SNU_FP32 ? tails(float()) : tails(double()))) return fail(rc);
+  }
   if ((rc = balloc(b, (void **)&wd.stall, sizeof(int))) ||
       (rc = balloc(b, (void **)&wd.half_done, sizeof(int) * (leaves.size() + 1))) ||
       (rc = balloc(b, (void **)&b->d_anorm, sizeof(unsigned long long) * count)) ||

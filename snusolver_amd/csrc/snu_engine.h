@@ -91,6 +91,9 @@ struct WorkDev {
   int *tsrc_leaf = nullptr, *tsrc_ri = nullptr, ntarget = 0;
   int *tile_blk = nullptr, *tile_row0 = nullptr, *tile_ptr = nullptr;
   int *seg_leaf = nullptr, *seg_ri = nullptr, ntile = 0;
+  void *tail_gp = nullptr, *tail_gs = nullptr;  // leaf-tail GEMM problems (value type of the set)
+  int tail_np = 0, tail_tiles = 0;
+  int max_nt = 0;  // largest leaf tail
   int cb_total = 0;
   int wsz = 0, waves = 1;          // leaf rows (workspace: leaf + boundary columns)
   int sep_wsz = 0, sep_waves = 1;  // separator rows (same workspace, no flags)
@@ -191,6 +194,11 @@ template <typename T>
 int add_gemm(std::vector<GemmProbT<T>> &gp, std::vector<GemmSegT<T>> &gs, int &tiles,
              T *C, long long ldc, int M, int N, const std::vector<GemmSegT<T>> &segs,
              double &flops, int tb = 64);
+// One C -= A B problem per leaf with a dense tail: S^T -= tail_u^T tail_l^T
+// (column-major view of the row-major blocks).
+template <typename T>
+void tail_gemm_list(const std::vector<LeafDev> &leaves, std::vector<GemmProbT<T>> &gp,
+                    std::vector<GemmSegT<T>> &gs, int &tiles);
 
 // GEMV plan of one matrix's dense solves into the packed vectors bp / ys at
 // this matrix's segment (ld = n, nrhs columns); keys order fwd / bwd.

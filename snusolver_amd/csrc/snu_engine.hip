@@ -108,6 +108,19 @@ int add_gemm(std::vector<GemmProbT<T>> &gp, std::vector<GemmSegT<T>> &gs, int &t
   gp.push_back(p);
   return 1;
 }
+template <typename T>
+void tail_gemm_list(const std::vector<LeafDev> &leaves, std::vector<GemmProbT<T>> &gp,
+                    std::vector<GemmSegT<T>> &gs, int &tiles) {
+  double flops = 0;
+  for (const LeafDev &L : leaves)
+    if (L.nt > 0 && L.nbr > 0 && L.nbc > 0)
+      add_gemm(gp, gs, tiles, (T *)L.S, L.nbc, L.nbc, L.nbr,
+               {GemmSegT<T>{(const T *)L.tail_u, (const T *)L.tail_l, L.nbc, L.nt, L.nt}}, flops, 64);
+}
+template void tail_gemm_list<double>(const std::vector<LeafDev> &, std::vector<GemmProbT<double>> &,
+                                     std::vector<GemmSegT<double>> &, int &);
+template void tail_gemm_list<float>(const std::vector<LeafDev> &, std::vector<GemmProbT<float>> &,
+                                    std::vector<GemmSegT<float>> &, int &);
 template int add_gemm<double>(std::vector<GemmProbT<double>> &, std::vector<GemmSegT<double>> &,
                               int &, double *, long long, int, int,
                               const std::vector<GemmSegT<double>> &, double &, int);
@@ -781,6 +794,23 @@ void merge_gemv(const std::vector<const GemvPlan<T> *> &in, GemvPlan<T> &out) {
 // device layout of one matrix
 // ---------------------------------------------------------------------------
 namespace {
+// The leaf-tail GEMM problems of one matrix (leaf descriptors complete).
+template <typename T>
+int upload_tail_gemm(snumat *h) {
+  std::vector<GemmProbT<T>> gp;
+  std::vector<GemmSegT<T>> gs;
+  int tiles = 0, rc;
+  tail_gemm_list<T>(h->h_leaves, gp, gs, tiles);
+  WorkDev &wd = h->wd;
+  GemmProbT<T> *dgp = nullptr;
+  GemmSegT<T> *dgs = nullptr;
+  if (!gp.empty() && ((rc = dupload(h, &dgp, gp)) || (rc = dupload(h, &dgs, gs)))) return rc;
+  wd.tail_gp = dgp;
+  wd.tail_gs = dgs;
+  wd.tail_np = (int)gp.size();
+  wd.tail_tiles = tiles;
+  return SNU_OK;
+}
 template <typename T>
 int alloc_values(snumat *h, long long S_total) {
   Values<T> &VV = vals<T>(h);
@@ -810,8 +840,10 @@ int upload(snumat *h) {
   const int np = P.np;
   // ---- leaf arrays (concatenated; descriptors hold pointers into them) ----
   std::vector<int> rowptr, diag, levptr, levrows, ulevptr, ulevrows;
-  std::vector<int> brow, bcol_glob, bcol_blk, bcol_loc, wcol, ftask, btask, ubnd;
-  struct Off { size_t rp, idx, lev, levr, ulev, ulevr, brow, bcol, task, row; long long S; };
+  std::vector<int> brow, bcol_glob, bcol_blk, bcol_loc, wcol, ftask, btask, ubnd, sep_tail;
+  const int vsz = h->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
+  int max_nt = 0;
+  struct Off { size_t rp, idx, lev, levr, ulev, ulevr, brow, bcol, task, row; long long S, tu, tl, tp; };
   std::vector<Off> off(np);
   long long S_total = 0;
   h->h_leaves.assign(np, LeafDev());
@@ -832,6 +864,27 @@ int upload(snumat *h) {
     o.row = ubnd.size();
     o.S = S_total;
     S_total += (long long)L.nbr * L.nbc;
+    // dense tail: the rows after the leaf's split (its own top separator),
+    // when they fit the LDS of the tail kernels
+    int nt = L.split_b > 0 && L.nbr > 0 ? L.leaf_size - L.split_b : 0;
+    if (nt > kMaxLeafTail || (long long)128 * nt * vsz > device_lds_limit() ||
+        (long long)nt * (nt + 1) / 2 * vsz > device_lds_limit())
+      nt = 0;
+    D.t0 = L.leaf_size - nt;
+    D.nt = nt;
+    max_nt = std::max(max_nt, nt);
+    o.tu = S_total;
+    S_total += (long long)nt * L.nbc;
+    o.tl = S_total;
+    S_total += (long long)L.nbr * nt;
+    o.tp = S_total;
+    S_total += (long long)nt * (nt + 1) / 2;
+    for (int ri = 0; ri < L.nbr; ri++) {
+      const int i = L.brow[ri];
+      int q = L.lu_rowptr[i];
+      while (q < L.lu_rowptr[i + 1] && L.lu_colidx[q] < D.t0) q++;
+      sep_tail.push_back(q);
+    }
     rowptr.insert(rowptr.end(), L.lu_rowptr.begin(), L.lu_rowptr.end());
     diag.insert(diag.end(), L.lu_diag.begin(), L.lu_diag.end());
     diag.push_back(0);  // diag shares rowptr's offsets (core_n + 1 entries)
@@ -904,6 +957,7 @@ int upload(snumat *h) {
   W = WorkLists();
   for (int r = 0; r < np; r++)
     for (int f = 0; f < P.leaves[r].nbr; f += h->wd.sep_waves) W.wg_leaf.push_back(r), W.wg_first.push_back(f);
+  h->wd.max_nt = max_nt;
   for (int r = 0; r < np; r++)
     for (int f = 0; f < P.leaves[r].nbr; f += 4) W.fwg_leaf.push_back(r), W.fwg_first.push_back(f);
   struct Src { int R, leaf, ri; };
@@ -980,7 +1034,7 @@ int upload(snumat *h) {
   h->arena_len = P.leaf_total + P.dense_total;
   // ---- structure uploads ----
   int *d_rowptr, *d_diag, *d_wcol, *d_levptr, *d_levrows, *d_ulevptr, *d_ulevrows;
-  int *d_brow, *d_bcol_glob, *d_bcol_blk, *d_bcol_loc, *d_ftask, *d_btask, *d_ubnd;
+  int *d_brow, *d_bcol_glob, *d_bcol_blk, *d_bcol_loc, *d_ftask, *d_btask, *d_ubnd, *d_sep_tail;
   if ((rc = dalloc(h, &h->d_aval, (size_t)P.nnz))) return rc;
   if ((rc = dupload(h, &h->d_adest, std::vector<long long>(P.a_dest.begin(), P.a_dest.end())))) return rc;
   if ((rc = dupload(h, &d_rowptr, rowptr)) || (rc = dupload(h, &d_diag, diag)) ||
@@ -990,7 +1044,7 @@ int upload(snumat *h) {
       (rc = dupload(h, &d_bcol_glob, bcol_glob)) || (rc = dupload(h, &d_bcol_blk, bcol_blk)) ||
       (rc = dupload(h, &d_bcol_loc, bcol_loc)) || (rc = dupload(h, &h->d_order, P.order)) ||
       (rc = dupload(h, &d_ftask, ftask)) || (rc = dupload(h, &d_btask, btask)) ||
-      (rc = dupload(h, &d_ubnd, ubnd)))
+      (rc = dupload(h, &d_ubnd, ubnd)) || (rc = dupload(h, &d_sep_tail, sep_tail)))
     return rc;
   WorkDev &wd = h->wd;
   wd.nleaves = np;
@@ -1039,6 +1093,7 @@ int upload(snumat *h) {
     D.ftask = d_ftask + o.task;
     D.btask = d_btask + o.task;
     D.ubnd = d_ubnd + o.row;
+    D.sep_tail = d_sep_tail + o.brow;
     D.dd = h->d_dd;
   }
   if ((rc = dalloc(h, &wd.stall, 1)) || (rc = dalloc(h, &wd.half_done, np)) ||
@@ -1051,12 +1106,24 @@ int upload(snumat *h) {
       return rc;
     rc = alloc_values<float>(h, S_total);
     if (!rc) {
-      for (int r = 0; r < np; r++) h->h_leaves[r].S = h->v32.S + off[r].S;
+      for (int r = 0; r < np; r++) {
+        h->h_leaves[r].S = h->v32.S + off[r].S;
+        h->h_leaves[r].tail_u = h->v32.S + off[r].tu;
+        h->h_leaves[r].tail_l = h->v32.S + off[r].tl;
+        h->h_leaves[r].tail_p = h->v32.S + off[r].tp;
+      }
+      rc = upload_tail_gemm<float>(h);
     }
   } else {
     rc = alloc_values<double>(h, S_total);
     if (!rc) {
-      for (int r = 0; r < np; r++) h->h_leaves[r].S = h->v64.S + off[r].S;
+      for (int r = 0; r < np; r++) {
+        h->h_leaves[r].S = h->v64.S + off[r].S;
+        h->h_leaves[r].tail_u = h->v64.S + off[r].tu;
+        h->h_leaves[r].tail_l = h->v64.S + off[r].tl;
+        h->h_leaves[r].tail_p = h->v64.S + off[r].tp;
+      }
+      rc = upload_tail_gemm<double>(h);
     }
   }
   if (rc) return rc;
@@ -1156,6 +1223,9 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[2], st));
   launch_leaf_sep<T>(wd.leaves, wd.wg_leaf, wd.wg_first, wd.nwg, wd.sep_wsz, wd.sep_waves, st);
+  launch_leaf_tail<T>(wd.leaves, wd.nleaves, wd.wg_leaf, wd.wg_first, wd.nwg, wd.sep_waves,
+                      (const GemmProbT<T> *)wd.tail_gp, wd.tail_np, (const GemmSegT<T> *)wd.tail_gs,
+                      wd.tail_tiles, wd.max_nt, st);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[3], st));
   launch_assemble_schur<T>(wd.ntile, wd.tile_blk, wd.tile_row0, wd.tile_ptr, wd.seg_leaf, wd.seg_ri,

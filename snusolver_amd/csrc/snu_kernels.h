@@ -32,10 +32,15 @@ struct LeafDev {
   const int *bcol_glob, *bcol_blk, *bcol_loc;  // boundary columns: permuted index, block, offset
   void *vals;                            // leaf LU values (value type of the launch)
   void *S;                               // boundary Schur block, nbr x nbc row-major
+  void *tail_u;                          // nt x nbc: tail rows' U entries on boundary columns
+  void *tail_l;                          // nbr x nt: separator rows on the tail columns
+  void *tail_p;                          // packed upper triangle of the tail block (U)
+  const int *sep_tail;                   // per boundary row: first L entry on a tail column
   void *dense;                           // the matrix's dense separator arena
   const DenseDev *dd;                    // the matrix's separator tables
   int core_n, leaf_size, nbr, nbc, nlev, nulev;
   int split_a, split_b;                  // rows [split_a, split_b) depend on none below split_a
+  int t0, nt;                            // dense tail rows [t0, leaf_size), nt of them (0: none)
   int gstart;                            // permuted index of the first leaf row
   int cb_off;                            // rows of the rhs contribution buffer
   int vec;                               // the matrix's slot in the launch's VecDev table
@@ -124,6 +129,9 @@ using GemvSeg = GemvSegT<double>;
 using GemvProb = GemvProbT<double>;
 
 int device_lds_limit();
+// Leaves whose own top separator (the rows after the leaf's split) has at
+// most this many rows finish their separator rows on it as dense row solves.
+constexpr int kMaxLeafTail = 160;
 
 template <typename T>
 void launch_fill(T *p, long long n, hipStream_t st);
@@ -145,6 +153,14 @@ void launch_zero_i32(int *p, int n, hipStream_t st);
 void launch_copy(const void *src, void *dst, long long bytes, hipStream_t st);
 // separator rows: L_SL row and Schur row per wave (workspace: leaf and
 // boundary columns)
+// Dense leaf tails (leaves with nt > 0): gathers the tail rows' U entries,
+// finishes the separator rows' tail multipliers (wg lists of the
+// separator-row launch) and their boundary values (tgp: one GEMM problem per
+// leaf, S -= multipliers x tail U on boundary columns).
+template <typename T>
+void launch_leaf_tail(const LeafDev *leaves, int nleaves, const int *wg_leaf, const int *wg_first,
+                      int nwg, int waves, const GemmProbT<T> *tgp, int ntgp,
+                      const GemmSegT<T> *tgs, int ttiles, int max_nt, hipStream_t st);
 template <typename T>
 void launch_leaf_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
                      int wsz, int waves, hipStream_t st);

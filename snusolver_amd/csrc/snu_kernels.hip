@@ -96,7 +96,7 @@ void launch_fill(T *p, long long n, hipStream_t st) {
 // pivots it has reached; the row's own flag is raised after its values are
 // stored.
 template <typename T, bool LEAF_ROW, int NCH, int NST>
-__device__ __forceinline__ void factor_row(int i, const int *__restrict__ rowptr,
+__device__ __forceinline__ void factor_row(int i, int pend, const int *__restrict__ rowptr,
                            const int *__restrict__ diag,
                            const int *__restrict__ wcol,
                            T *__restrict__ val, T *w, int lane,
@@ -111,7 +111,7 @@ __device__ __forceinline__ void factor_row(int i, const int *__restrict__ rowptr
   // workspace value it touches (one LDS round trip, overlapping the
   // division) before it writes any of them back.
   const int p0 = rowptr[i], p1 = rowptr[i + 1];
-  const int pd = LEAF_ROW ? diag[i] : p1;
+  const int pd = LEAF_ROW ? diag[i] : pend;  // separator rows: pivots before pend only
   for (int p = p0 + lane; p < p1; p += 64) w[wcol[p]] = val[p];
   wave_lds_fence();
   int ca[NCH] = {}, cb[NCH] = {}, cc[NCH] = {}, cd[NCH] = {};
@@ -221,7 +221,8 @@ __device__ __forceinline__ void factor_row(int i, const int *__restrict__ rowptr
     }
     wave_lds_fence();
   }
-  for (int p = p0 + lane; p < p1; p += 64) {
+  // separator rows stored up to pend; the workspace keeps the rest
+  for (int p = p0 + lane; p < (LEAF_ROW ? p1 : pend); p += 64) {
     int c = wcol[p];
     val[p] = w[c];
     w[c] = T(0);
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ l
   T *v = (T *)L.vals;
   if (role == 0) {
     for (int r = sa + wave; r < sb; r += W)
-      factor_row<T, true, 4, 4>(r, rp, dg, wc, v, w, lane, eps, done, stall);
+      factor_row<T, true, 4, 4>(r, -1, rp, dg, wc, v, w, lane, eps, done, stall);
     __syncthreads();
     if (threadIdx.x == 0) {
       __threadfence();
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ l
     return;
   }
   for (int r = wave; r < sa; r += W)
-    factor_row<T, true, 4, 4>(r, rp, dg, wc, v, w, lane, eps, done, stall);
+    factor_row<T, true, 4, 4>(r, -1, rp, dg, wc, v, w, lane, eps, done, stall);
   if (sa < sb) {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ l
     __syncthreads();
   }
   for (int r = sb + wave; r < L.leaf_size; r += W)
-    factor_row<T, true, 4, 4>(r, rp, dg, wc, v, w, lane, eps, done, stall);
+    factor_row<T, true, 4, 4>(r, -1, rp, dg, wc, v, w, lane, eps, done, stall);
 }
 // LDS bytes of the completion flags for leaves of up to max_leaf rows
 int leaf_flag_bytes(int max_leaf) { return (max_leaf + 15) & ~15; }
@@ -341,8 +342,20 @@ __global__ __launch_bounds__(512) void k_leaf_sep(const LeafDev *__restrict__ le
   const int ls = L.leaf_size;
   for (int c = lane; c < ls + L.nbc; c += 64) w[c] = T(0);
   wave_lds_fence();
-  factor_row<T, false, 16, 2>(L.brow[ri], L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0), nullptr,
-                       nullptr);
+  // with a dense tail (nt > 0) only the pivots left of it are applied here;
+  // the row's tail values go to tail_l and its boundary values, still
+  // missing the tail pivots' updates, to S (k_leaf_tail_l / _s finish both)
+  const int i = L.brow[ri];
+  const int pend = L.nt ? L.sep_tail[ri] : L.rowptr[i + 1];
+  factor_row<T, false, 16, 2>(i, pend, L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0),
+                              nullptr, nullptr);
+  if (L.nt) {
+    T *trow = (T *)L.tail_l + (long long)ri * L.nt;
+    for (int c = lane; c < L.nt; c += 64) {
+      trow[c] = w[L.t0 + c];
+      w[L.t0 + c] = T(0);
+    }
+  }
   T *Srow = (T *)L.S + (long long)ri * L.nbc;
   for (int j = lane; j < L.nbc; j += 64) {
     Srow[j] = w[ls + j];
@@ -358,6 +371,135 @@ void launch_leaf_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_fi
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(k_leaf_sep<T>, dim3(nwg), dim3(64 * waves), lds, st, leaves, wg_leaf,
                      wg_first, wsz);
+}
+
+// ---------------------------------------------------------------------------
+// leaf tails: the last nt rows of a leaf (its own top separator, dense after
+// the elimination below it) finish the separator rows as dense row solves
+// ---------------------------------------------------------------------------
+// Packed upper triangle of the nt x nt tail block: row r, column c >= r.
+__device__ __forceinline__ int tail_pk(int r, int c, int nt) {
+  return r * nt - ((r * (r - 1)) >> 1) + (c - r);
+}
+// From the tail rows' U entries: tail_u (nt x nbc, row-major) <- those on
+// boundary columns, tail_p (packed upper triangle) <- those on tail columns,
+// zeros elsewhere.  One workgroup per leaf.
+template <typename T>
+__global__ __launch_bounds__(256) void k_leaf_tail_gather(const LeafDev *__restrict__ leaves) {
+  const LeafDev L = leaves[blockIdx.x];
+  if (!L.nt) return;
+  T *tu = (T *)L.tail_u, *tp = (T *)L.tail_p;
+  const int nt = L.nt;
+  const long long n = (long long)nt * L.nbc, npk = (long long)nt * (nt + 1) / 2;
+  for (long long e = threadIdx.x; e < n; e += blockDim.x) tu[e] = T(0);
+  for (long long e = threadIdx.x; e < npk; e += blockDim.x) tp[e] = T(0);
+  __syncthreads();
+  const int ls = L.leaf_size, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const T *v = (const T *)L.vals;
+  for (int r = wave; r < nt; r += blockDim.x >> 6) {
+    const int i = L.t0 + r;
+    for (int p = L.diag[i] + lane; p < L.rowptr[i + 1]; p += 64) {
+      const int c = L.wcol[p];
+      if (c >= ls) tu[(long long)r * L.nbc + (c - ls)] = v[p];
+      else tp[tail_pk(r, c - L.t0, nt)] = v[p];
+    }
+  }
+}
+// Separator rows on the tail: each wave takes two rows of tail_l (their
+// workspace values on the tail columns) and applies the tail pivots in
+// ascending order exactly as the sparse row elimination does (l = w_k /
+// U_kk, then w_c -= l U_kc for c > k; no FMA contraction), with the tail's
+// upper triangle packed in LDS; the multipliers replace the row in tail_l
+// and are stored into the row's L entries on tail columns.  Lane j holds
+// columns j, j + 64, j + 128.
+template <typename T>
+__global__ __launch_bounds__(512) void k_leaf_tail_l(const LeafDev *__restrict__ leaves,
+                                                     const int *__restrict__ wg_leaf,
+                                                     const int *__restrict__ wg_first, int group) {
+#pragma clang fp contract(off)
+  T *ut = (T *)lds_raw;
+  const LeafDev L = leaves[wg_leaf[blockIdx.x]];
+  const int nt = L.nt;
+  if (!nt) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int npk = nt * (nt + 1) / 2;
+  const T *tp = (const T *)L.tail_p;
+  for (int e = threadIdx.x; e < npk; e += blockDim.x) ut[e] = tp[e];
+  __syncthreads();
+  // two rows per wave (two independent dependency chains interleave)
+  // rows [first, first + group) of the separator-row group
+  const int first = wg_first[blockIdx.x], end = min(first + group, L.nbr);
+  const int ra = first + 2 * wave, rb = ra + 1;
+  if (ra >= end) return;
+  const bool has_b = rb < end;
+  T *trow_a = (T *)L.tail_l + (long long)ra * nt;
+  T *trow_b = (T *)L.tail_l + (long long)(has_b ? rb : ra) * nt;
+  T xa[3], xb[3];
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    const bool in = lane + 64 * s < nt;
+    xa[s] = in ? trow_a[lane + 64 * s] : T(0);
+    xb[s] = in ? trow_b[lane + 64 * s] : T(0);
+  }
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    const int kend = min(nt, 64 * (s + 1));
+    for (int k = 64 * s; k < kend; k++) {
+      // every LDS operand of the step first (addresses clamped to the
+      // diagonal for lanes the step does not update), one wait
+      const int base = tail_pk(k, k, nt);
+      T u[3];
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const int c = lane + 64 * q;
+        u[q] = ut[base + ((c > k && c < nt) ? c - k : 0)];
+      }
+      const T d = ut[base];
+      const T la = rld(xa[s], k - 64 * s) / d;
+      const T lb = rld(xb[s], k - 64 * s) / d;
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const int c = lane + 64 * q;
+        if (c > k && c < nt) {
+          T pa = la * u[q], pb = lb * u[q];
+          xa[q] = xa[q] - pa;
+          xb[q] = xb[q] - pb;
+        }
+      }
+      if (lane == k - 64 * s) xa[s] = la, xb[s] = lb;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 3; s++)
+    if (lane + 64 * s < nt) {
+      trow_a[lane + 64 * s] = xa[s];
+      if (has_b) trow_b[lane + 64 * s] = xb[s];
+    }
+  // the rows' L entries on tail columns (pattern order, columns ascending)
+  T *vw = (T *)L.vals;
+  for (int h = 0; h < (has_b ? 2 : 1); h++) {
+    const int ri = h ? rb : ra;
+    const T *trow = h ? trow_b : trow_a;
+    const int i = L.brow[ri];
+    for (int p = L.sep_tail[ri] + lane; p < L.rowptr[i + 1]; p += 64) vw[p] = trow[L.wcol[p] - L.t0];
+  }
+}
+template <typename T>
+void launch_leaf_tail(const LeafDev *leaves, int nleaves, const int *wg_leaf, const int *wg_first,
+                      int nwg, int waves, const GemmProbT<T> *tgp, int ntgp,
+                      const GemmSegT<T> *tgs, int ttiles, int max_nt, hipStream_t st) {
+  if (max_nt <= 0 || nleaves <= 0) return;
+  hipLaunchKernelGGL(k_leaf_tail_gather<T>, dim3(nleaves), dim3(256), 0, st, leaves);
+  const size_t lds_l = (size_t)max_nt * (max_nt + 1) / 2 * sizeof(T);
+  (void)hipFuncSetAttribute((const void *)k_leaf_tail_l<T>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_l);
+  if (nwg > 0)  // the separator-row groups of `waves` rows, two rows per wave
+    hipLaunchKernelGGL(k_leaf_tail_l<T>, dim3(nwg), dim3(64 * ((waves + 1) / 2)), lds_l, st, leaves,
+                       wg_leaf, wg_first, waves);
+  // S -= tail multipliers x tail U on boundary columns: one grouped GEMM,
+  // as S^T (nbc x nbr) -= tail_u^T (nbc x nt) tail_l^T (nt x nbr) in the
+  // column-major terms of the kernel
+  launch_gemm(tgp, ntgp, tgs, ttiles, 0, st);
 }
 
 // Slot of separator entry (bi, lr) x (bj, lc) in the dense arena.
@@ -1694,6 +1836,9 @@ int device_lds_limit() {
   template void launch_fill<T>(T *, long long, hipStream_t);                                  \
   template void launch_prepare<T>(const PrepDev *, int, long long, long long, hipStream_t);   \
   template void launch_leaf_rows<T>(const LeafDev *, int, int, int, int, double, int *, int *,\
+                                    hipStream_t);                                              \
+  template void launch_leaf_tail<T>(const LeafDev *, int, const int *, const int *, int, int,  \
+                                    const GemmProbT<T> *, int, const GemmSegT<T> *, int, int,   \
                                     hipStream_t);                                              \
   template void launch_leaf_sep<T>(const LeafDev *, const int *, const int *, int, int, int,  \
                                    hipStream_t);                                              \
