@@ -349,7 +349,9 @@ void build_dense_plan(snumat *h) {
         d.np++;
       }
       if (d.np) out.push_back(d);
-      // L21 <- L21 U11^-1 (rows of D below the block)
+      // L21 <- L21 U11^-1 (rows of D below the block) and U12 <- L11^-1 U12
+      // (row block of D right of the diagonal block): independent products,
+      // one grouped launch
       Launch tr{1, (int)h_gp.size(), 0, 0, 0, fkey(height, ks + 1)};
       for (int i = lo; i <= hi; i++) {
         int s = P.block_size[i];
@@ -359,20 +361,12 @@ void build_dense_plan(snumat *h) {
         T *X1 = R(i) + (k + nb) + (long long)k * s;
         tr.np += add_gemm(h_gp, h_gs, tr.tiles, X1, s, s - k - nb, nb,
                           {GemmSeg{X1, U11i, s, s, nb}}, tr.flops);
-      }
-      if (tr.np) out.push_back(tr);
-      // U12 <- L11^-1 U12 (row block of D right of the diagonal block)
-      Launch tl{1, (int)h_gp.size(), 0, 0, 0, fkey(height, ks + 2)};
-      for (int i = lo; i <= hi; i++) {
-        int s = P.block_size[i];
-        if (s <= k) continue;
-        int nb = std::min(64, s - k);
         const T *L11i = Li(i) + k + (long long)k * s;
         T *X = R(i) + k + (long long)(k + nb) * s;
-        tl.np += add_gemm(h_gp, h_gs, tl.tiles, X, s, nb, s - k - nb,
-                          {GemmSeg{L11i, X, s, s, nb}}, tl.flops);
+        tr.np += add_gemm(h_gp, h_gs, tr.tiles, X, s, nb, s - k - nb,
+                          {GemmSeg{L11i, X, s, s, nb}}, tr.flops);
       }
-      if (tl.np) out.push_back(tl);
+      if (tr.np) out.push_back(tr);
       // trailing update of D
       GemmBatch<T> tu;
       for (int i = lo; i <= hi; i++) {

@@ -738,10 +738,14 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
   };
   // one K chunk from LDS buffer buf: all LDS operands first (one wait),
   // then the MFMA block
+  // small wave tiles: every LDS operand of the chunk first (one wait);
+  // large ones (16 blocks per k step): operands per k step, which keeps the
+  // register count at two workgroups per CU
+  constexpr int KQL = BMB * BNB >= 16 ? 1 : KB / 4;
   auto compute = [&](int buf) {
       T bvk[KB / 4][BNB], avk[KB / 4][BMB];
 #pragma unroll
-      for (int kq = 0; kq < KB / 4; kq++) {
+      for (int kq = 0; kq < KQL; kq++) {
 #pragma unroll
         for (int b = 0; b < BNB; b++) bvk[kq][b] = Bs[buf][4 * kq + lk][wn + b * 16 + l16];
 #pragma unroll
@@ -749,6 +753,12 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
       }
 #pragma unroll
       for (int kq = 0; kq < KB / 4; kq++) {
+        if (kq >= KQL) {
+#pragma unroll
+          for (int b = 0; b < BNB; b++) bvk[kq][b] = Bs[buf][4 * kq + lk][wn + b * 16 + l16];
+#pragma unroll
+          for (int a = 0; a < BMB; a++) avk[kq][a] = As[buf][4 * kq + lk][wm + a * 16 + l16];
+        }
         const T *bv = bvk[kq], *av = avk[kq];
         if constexpr (F64 && M16) {
 #pragma unroll
@@ -799,14 +809,17 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
       n = n0 + wn + b * 16 + 4 * lk + r;
       v = accf[a][b][r];
     }
-  };  // C -= AB: after the K loop every C value of the tile is loaded before any
-  // is stored, so the loads are in flight together (one round trip, not one
-  // per element)
+  };
+  // C -= AB: after the K loop the C values of a group of block rows are all
+  // loaded before any is stored, so they are in flight together (one round
+  // trip per group, not one per element; groups of 2 block rows keep the
+  // registers of the large tile in bounds)
+  constexpr int AG = BMB > 2 ? 2 : BMB;
   T cold[BMB][BNB][4];
-  auto load_c = [&]() __attribute__((always_inline)) {
+  auto load_c = [&](int a0) __attribute__((always_inline)) {
     if constexpr (MODE == 0) {
 #pragma unroll
-      for (int a = 0; a < BMB; a++)
+      for (int a = a0; a < a0 + AG; a++)
 #pragma unroll
         for (int b = 0; b < BNB; b++)
 #pragma unroll
@@ -881,24 +894,26 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
       }
     }
   }
-  load_c();
-
 #pragma unroll
-  for (int a = 0; a < BMB; a++)
+  for (int a0 = 0; a0 < BMB; a0 += AG) {
+    load_c(a0);
 #pragma unroll
-    for (int b = 0; b < BNB; b++)
+    for (int a = a0; a < a0 + AG; a++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        int m, n;
-        T v;
-        elem(a, b, r, m, n, v);
-        if (m < P.M && n < P.N) {
-          auto c = gptr(P.C) + m + (long long)n * P.ldc;
-          if (MODE == 1) *c = v;
-          else if (MODE == 2) *c = -v;
-          else *c = cold[a][b][r] - v;
+      for (int b = 0; b < BNB; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int m, n;
+          T v;
+          elem(a, b, r, m, n, v);
+          if (m < P.M && n < P.N) {
+            auto c = gptr(P.C) + m + (long long)n * P.ldc;
+            if (MODE == 1) *c = v;
+            else if (MODE == 2) *c = -v;
+            else *c = cold[a][b][r] - v;
+          }
         }
-      }
+  }
 }
 // Grouped 2-D copy over GEMM descriptors: C (ldc) = A of the problem's first
 // segment (lda), one workgroup per 64 x 64 tile, rows contiguous per wave.
