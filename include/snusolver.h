@@ -287,7 +287,8 @@ int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,
 /* average time (ms) of the grouped C -= A B kernel on device-resident
  * operands, K split into nseg concatenated segments; tile 64 or 128 (K chunk
  * 16), 6432 / 12832 (K chunk 32), 6416 / 12816 (FP64 on the 16x16x4
- * instruction) or 642 / 1282 (K chunk 16, loads two
+ * instruction; 64162 / 128162 loading two chunks ahead, 64163 / 128163
+ * with K chunk 32) or 642 / 1282 (K chunk 16, loads two
  * chunks ahead); prec SNU_FP64 or SNU_FP32. */
 int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                    double *ms_out);

@@ -703,11 +703,14 @@ extern "C" int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec,
                               double *ms_out) {
   // tile: 64 or 128 (K chunk 16), 6432 / 12832 (K chunk 32), 642 / 1282
   // (K chunk 16, loads two chunks ahead), 6416 / 12816 (FP64 on the
-  // 16x16x4 instruction)
+  // 16x16x4 instruction; 64162 / 128162 with loads two chunks ahead,
+  // 64163 / 128163 with K chunk 32)
   int t = tile, kb = 16, depth = 1, m16 = 0;
   if (tile == 6432 || tile == 12832) t = tile / 100, kb = 32;
   if (tile == 642 || tile == 1282) t = tile / 10, depth = 2;
   if (tile == 6416 || tile == 12816) t = tile / 100, m16 = 1;
+  if (tile == 64162 || tile == 128162) t = tile / 1000, m16 = 1, depth = 2;
+  if (tile == 64163 || tile == 128163) t = tile / 1000, m16 = 1, kb = 32;
   if (M < 1 || N < 1 || K < 1 || nseg < 1 || reps < 1 || (t != 64 && t != 128) ||
       (prec != SNU_FP64 && prec != SNU_FP32) || !ms_out)
     return set_error(SNU_ERR_BAD_INPUT, "bad gemm bench shape");

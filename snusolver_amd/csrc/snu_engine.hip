@@ -130,38 +130,41 @@ template int add_gemm<float>(std::vector<GemmProbT<float>> &, std::vector<GemmSe
 
 // ---------------------------------------------------------------------------
 // tile-shape autotuner: per precision and size bucket of min(M, N), the
-// faster of the 64 x 64 and 128 x 128 C -= AB kernels, measured once per
-// process on device-resident operands (both tiles compute every element with
-// the same instruction sequence, so the choice never changes results)
+// faster of the 64 x 64 and 128 x 128 C -= AB kernels on a launch of many
+// problems of the bucket's probe size (enough tiles to fill the device, as
+// in the Schur-update launches), measured once per process on
+// device-resident operands (both tiles compute every element with the same
+// instruction sequence, so the choice never changes results)
 // ---------------------------------------------------------------------------
 namespace {
 std::mutex g_tune_mu;
 bool g_tuned[2] = {false, false};
 int g_tile[2][3] = {{128, 128, 128}, {128, 128, 128}};
 const int kBucketEdge[3] = {128, 512, 2048};
-const int kBucketProbe[3] = {256, 1024, 4096};
+const int kBucketProbe[3] = {128, 1024, 4096};
 
 template <typename T>
-double time_gemm(int M, int N, int K, int tile, int reps) {
+double time_gemm(int n, int count, int K, int tile, int reps) {
   T *dA = nullptr, *dB = nullptr, *dC = nullptr;
   GemmProbT<T> *dgp = nullptr;
   GemmSegT<T> *dgs = nullptr;
   double ms_out = -1;
-  if (hipMalloc((void **)&dA, sizeof(T) * (size_t)M * K) == hipSuccess &&
-      hipMalloc((void **)&dB, sizeof(T) * (size_t)K * N) == hipSuccess &&
-      hipMalloc((void **)&dC, sizeof(T) * (size_t)M * N) == hipSuccess) {
-    launch_fill(dA, (long long)M * K, nullptr);
-    launch_fill(dB, (long long)K * N, nullptr);
-    launch_fill(dC, (long long)M * N, nullptr);
+  if (hipMalloc((void **)&dA, sizeof(T) * (size_t)n * K) == hipSuccess &&
+      hipMalloc((void **)&dB, sizeof(T) * (size_t)K * n) == hipSuccess &&
+      hipMalloc((void **)&dC, sizeof(T) * (size_t)n * n * count) == hipSuccess) {
+    launch_fill(dA, (long long)n * K, nullptr);
+    launch_fill(dB, (long long)K * n, nullptr);
+    launch_fill(dC, (long long)n * n * count, nullptr);
     std::vector<GemmProbT<T>> gp;
     std::vector<GemmSegT<T>> gs;
     int tiles = 0;
     double fl = 0;
-    add_gemm<T>(gp, gs, tiles, dC, M, M, N, {GemmSegT<T>{dA, dB, M, K, K}}, fl, tile);
+    for (int c = 0; c < count; c++)
+      add_gemm<T>(gp, gs, tiles, dC + (size_t)n * n * c, n, n, n, {GemmSegT<T>{dA, dB, n, K, K}}, fl, tile);
     if (upload_list(&dgp, gp) == SNU_OK && upload_list(&dgs, gs) == SNU_OK) {
       auto run = [&]() {
-        if (tile == 128) launch_gemm128<T>(dgp, 1, dgs, tiles, nullptr);
-        else launch_gemm<T>(dgp, 1, dgs, tiles, 0, nullptr);
+        if (tile == 128) launch_gemm128<T>(dgp, count, dgs, tiles, nullptr);
+        else launch_gemm<T>(dgp, count, dgs, tiles, 0, nullptr);
       };
       hipEvent_t e0, e1;
       if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
@@ -191,8 +194,8 @@ void autotune_tiles() {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   if (g_tuned[pi]) return;
   for (int b = 0; b < 3; b++) {
-    const int n = kBucketProbe[b];
-    const double t64 = time_gemm<T>(n, n, 256, 64, 3), t128 = time_gemm<T>(n, n, 256, 128, 3);
+    const int n = kBucketProbe[b], count = (4096 / n) * (4096 / n);
+    const double t64 = time_gemm<T>(n, count, 1024, 64, 3), t128 = time_gemm<T>(n, count, 1024, 128, 3);
     if (t64 > 0 && t128 > 0) g_tile[pi][b] = t128 < t64 ? 128 : 64;
   }
   g_tuned[pi] = true;
@@ -213,20 +216,39 @@ int tile_table(int prec, int *out3) {
 
 namespace {
 
-// A grouped C -= sum A B launch, split by tile size (chosen per size bucket
-// by the autotuner): 128 x 128 tiles are launch kind 5, 64 x 64 kind 2.
+// A grouped C -= sum A B launch, split by tile size: problems whose size
+// bucket the autotuner gave to the 128 x 128 kernel take it when the launch
+// holds enough of those tiles to fill the device twice over (fewer would
+// leave CUs idle that 64 x 64 tiles keep busy); the rest take 64 x 64 tiles
+// (launch kind 5 and kind 2).
+constexpr int kMin128Tiles = 512;
 template <typename T>
 struct GemmBatch {
-  std::vector<GemmProbT<T>> p[2];
-  int tiles[2] = {0, 0};
-  double flops[2] = {0, 0};
-  int add(std::vector<GemmSegT<T>> &gs, T *C, long long ldc, int M, int N,
-          const std::vector<GemmSegT<T>> &segs) {
-    if (M <= 0 || N <= 0) return 0;
-    int t = tile_for(std::is_same<T, float>::value ? SNU_FP32 : SNU_FP64, M, N) == 128 ? 1 : 0;
-    return add_gemm(p[t], gs, tiles[t], C, ldc, M, N, segs, flops[t], t ? 128 : 64);
+  struct Pend {
+    T *C;
+    long long ldc;
+    int M, N;
+    std::vector<GemmSegT<T>> segs;
+  };
+  std::vector<Pend> pend;
+  void add(T *C, long long ldc, int M, int N, const std::vector<GemmSegT<T>> &segs) {
+    if (M > 0 && N > 0) pend.push_back(Pend{C, ldc, M, N, segs});
   }
-  void flush(std::vector<GemmProbT<T>> &gp, std::vector<Launch> &out, long long key) {
+  void flush(std::vector<GemmProbT<T>> &gp, std::vector<GemmSegT<T>> &gs, std::vector<Launch> &out,
+             long long key) {
+    const int prec = std::is_same<T, float>::value ? SNU_FP32 : SNU_FP64;
+    long long big = 0;
+    for (const Pend &q : pend)
+      if (tile_for(prec, q.M, q.N) == 128) big += (long long)((q.M + 127) / 128) * ((q.N + 127) / 128);
+    const bool use128 = big >= kMin128Tiles;
+    std::vector<GemmProbT<T>> p[2];
+    int tiles[2] = {0, 0};
+    double flops[2] = {0, 0};
+    for (const Pend &q : pend) {
+      const int t = use128 && tile_for(prec, q.M, q.N) == 128 ? 1 : 0;
+      add_gemm(p[t], gs, tiles[t], q.C, q.ldc, q.M, q.N, q.segs, flops[t], t ? 128 : 64);
+    }
+    pend.clear();
     for (int t = 0; t < 2; t++) {
       if (p[t].empty()) continue;
       Launch L{t ? 5 : 2, (int)gp.size(), (int)p[t].size(), tiles[t], flops[t], key};
@@ -375,10 +397,10 @@ void build_dense_plan(snumat *h) {
         int nb = std::min(64, s - k);
         T *Ccol = R(i) + k + (long long)(k + nb) * s;  // U12 rows
         const T *L21 = R(i) + (k + nb) + (long long)k * s;
-        tu.add(h_gs, R(i) + (k + nb) + (long long)(k + nb) * s, s, s - k - nb, s - k - nb,
+        tu.add(R(i) + (k + nb) + (long long)(k + nb) * s, s, s - k - nb, s - k - nb,
                {GemmSeg{L21, Ccol, s, s, nb}});
       }
-      tu.flush(h_gp, out, fkey(height, ks + 3));
+      tu.flush(h_gp, h_gs, out, fkey(height, ks + 3));
     }
     // full triangular inverses by recursive doubling:
     //   U: X[r0:, c0:] = -X[r0:, r0:] (U[r0:, c0:] X[c0:, c0:])
@@ -485,31 +507,49 @@ void build_dense_plan(snumat *h) {
               if (P.anyL(i, r0, r0 + mt) && P.anyU(i, c0, c0 + nt)) who[(size_t)a * tn + b].push_back(i);
             }
           }
-        // one problem per 64 x 64 target tile with its contributor set,
-        // emitted in strips of 8 tile rows, column by column inside a
-        // strip: consecutive tiles (which one XCD runs together) share A
-        // rows (same tile row and contributor) and B columns (same tile
-        // column), so their operands meet in that XCD's L2
+        // one problem per 64 x 64 target tile with its contributor set, or
+        // per 128 x 128 block when its four tiles share one set (the tile
+        // kernel is then chosen per size bucket), emitted in strips of 8
+        // tile rows, two tile columns at a time: consecutive tiles (which
+        // one XCD runs together) share A rows (same tile row and
+        // contributor) and B columns (same tile column), so their operands
+        // meet in that XCD's L2
+        auto segs_of = [&](const std::vector<int> &w, int m0, int n0) {
+          std::vector<GemmSeg> segs;
+          for (int i : w) {
+            const int si = P.block_size[i], ai = P.anc_sum[i], off = P.anc_off(i, p);
+            const int r0 = off + (pass == 0 ? 0 : sp) + m0, c0 = off + n0;
+            segs.push_back(GemmSeg{Lp(i) + r0, R(i) + (long long)(si + c0) * si, ai, si, si});
+          }
+          return segs;
+        };
+        auto emit = [&](int a, int b, int ta, int tb) {
+          const std::vector<int> &w = who[(size_t)a * tn + b];
+          const int m0 = 64 * a, n0 = 64 * b;
+          const int mt = std::min(64 * ta, M - m0), nt = std::min(64 * tb, N - n0);
+          T *C = pass == 0 ? R(p) + m0 + (long long)n0 * sp : Lp(p) + m0 + (long long)n0 * ap;
+          sc.add(C, pass == 0 ? sp : ap, mt, nt, segs_of(w, m0, n0));
+        };
+        auto nonempty = [&](int a, int b) { return a < tm && b < tn && !who[(size_t)a * tn + b].empty(); };
         constexpr int kStrip = 8;
         for (int a0 = 0; a0 < tm; a0 += kStrip)
-          for (int b = 0; b < tn; b++)
-            for (int a = a0; a < std::min(tm, a0 + kStrip); a++) {
+          for (int b = 0; b < tn; b += 2)
+            for (int a = a0; a < std::min(tm, a0 + kStrip); a += 2) {
               const std::vector<int> &w = who[(size_t)a * tn + b];
-              if (w.empty()) continue;
-              const int m0 = 64 * a, n0 = 64 * b;
-              const int mt = std::min(64, M - m0), nt = std::min(64, N - n0);
-              std::vector<GemmSeg> segs;
-              for (int i : w) {
-                const int si = P.block_size[i], ai = P.anc_sum[i], off = P.anc_off(i, p);
-                const int r0 = off + (pass == 0 ? 0 : sp) + m0, c0 = off + n0;
-                segs.push_back(GemmSeg{Lp(i) + r0, R(i) + (long long)(si + c0) * si, ai, si, si});
+              if (nonempty(a, b) && nonempty(a + 1, b) && nonempty(a, b + 1) && nonempty(a + 1, b + 1) &&
+                  who[(size_t)(a + 1) * tn + b] == w && who[(size_t)a * tn + b + 1] == w &&
+                  who[(size_t)(a + 1) * tn + b + 1] == w) {
+                emit(a, b, 2, 2);
+                continue;
               }
-              T *C = pass == 0 ? R(p) + m0 + (long long)n0 * sp : Lp(p) + m0 + (long long)n0 * ap;
-              sc.add(h_gs, C, pass == 0 ? sp : ap, mt, nt, segs);
+              for (int q = 0; q < 4; q++) {
+                const int x = a + (q & 1), y = b + (q >> 1);
+                if (nonempty(x, y)) emit(x, y, 1, 1);
+              }
             }
       }
     }
-    sc.flush(h_gp, out, fkey(height, 90000000LL));
+    sc.flush(h_gp, h_gs, out, fkey(height, 90000000LL));
   }
 }
 

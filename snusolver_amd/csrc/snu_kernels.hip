@@ -949,24 +949,28 @@ void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                  int total_tiles, int mode, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
   // FP64 on the 16x16x4 instruction: measured faster than four rotated
-  // 4x4x4 products per block on large products, equal on small ones
+  // 4x4x4 products per block on large products, equal on small ones.
+  // FP32: loads two chunks ahead (measured 8% faster on 64 x 64 tiles; the
+  // FP64 kernel gains nothing from it)
   constexpr bool m16 = std::is_same<T, double>::value;
+  constexpr int depth = m16 ? 1 : 2;
   if (mode == 1)
-    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
-                       nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st,
+                       probs, nprob, segs);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
-                       nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st,
+                       probs, nprob, segs);
   else
-    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
-                       nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st,
+                       probs, nprob, segs);
 }
 template <typename T>
 void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                     int total_tiles, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
   constexpr bool m16 = std::is_same<T, double>::value;
-  hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
+  constexpr int depth = m16 ? 1 : 2;
+  hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
                      nprob, segs);
 }
 // C -= AB variants for tuning: tile 64 or 128, K chunk 16 or 32, register
@@ -977,8 +981,15 @@ void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T>
   if (total_tiles <= 0 || nprob <= 0) return;
   const dim3 g(total_tiles), b(256);
   if (m16 && std::is_same<T, double>::value) {
-    if (tile == 128) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1, true>), g, b, 0, st, probs, nprob, segs);
-    else hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 1, true>), g, b, 0, st, probs, nprob, segs);
+    if (tile == 128) {
+      if (depth == 2) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 2, true>), g, b, 0, st, probs, nprob, segs);
+      else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 32, 1, true>), g, b, 0, st, probs, nprob, segs);
+      else hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1, true>), g, b, 0, st, probs, nprob, segs);
+    } else {
+      if (depth == 2) hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 2, true>), g, b, 0, st, probs, nprob, segs);
+      else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 32, 1, true>), g, b, 0, st, probs, nprob, segs);
+      else hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 1, true>), g, b, 0, st, probs, nprob, segs);
+    }
     return;
   }
   if (tile == 128) {
