@@ -163,7 +163,7 @@ double time_gemm(int n, int count, int K, int tile, int reps) {
       add_gemm<T>(gp, gs, tiles, dC + (size_t)n * n * c, n, n, n, {GemmSegT<T>{dA, dB, n, K, K}}, fl, tile);
     if (upload_list(&dgp, gp) == SNU_OK && upload_list(&dgs, gs) == SNU_OK) {
       auto run = [&]() {
-        if (tile == 128) launch_gemm128<T>(dgp, count, dgs, tiles, nullptr);
+        if (tile == 128) launch_gemm128<T>(dgp, count, dgs, tiles, 0, nullptr);
         else launch_gemm<T>(dgp, count, dgs, tiles, 0, nullptr);
       };
       hipEvent_t e0, e1;
@@ -439,12 +439,26 @@ void build_dense_plan(snumat *h) {
         out.push_back(t2);
       }
     }
-    // panels: Lp <- Lp U^-1 per 64-column strip of the result (U^-1 is upper
-    // triangular: strip [n0, n0 + 64) needs rows [0, n0 + 64) of it), Up <-
-    // L^-1 Up per 64-row strip (rows [0, m0 + 64) of Up), over the
-    // structurally nonzero row / column runs, into scratch; then copied back
+    // panels: Lp <- Lp U^-1 per W-column strip of the result (U^-1 is upper
+    // triangular: strip [n0, n0 + W) needs rows [0, n0 + W) of it), Up <-
+    // L^-1 Up per W-row strip (rows [0, m0 + W) of Up), over the
+    // structurally nonzero row / column runs, into scratch; then copied back.
+    // W = 128 (128 x 128 tiles) when the autotuner prefers that tile for the
+    // level's panels and they hold enough such tiles to fill the device
     {
-      Launch pm{1, (int)h_gp.size(), 0, 0, 0, fkey(height, 60000000LL)};
+      const int prec = std::is_same<T, float>::value ? SNU_FP32 : SNU_FP64;
+      long long big = 0;
+      for (int i = lo; i <= hi; i++) {
+        const int s = P.block_size[i], a = P.anc_sum[i];
+        for (auto rr : nz_runs(a, [&](int x0, int x1) { return P.anyL(i, x0, x1); }))
+          if (tile_for(prec, rr.second - rr.first, s) == 128)
+            big += (long long)((rr.second - rr.first + 127) / 128) * ((s + 127) / 128);
+        for (auto cr : nz_runs(a, [&](int x0, int x1) { return P.anyU(i, x0, x1); }))
+          if (tile_for(prec, s, cr.second - cr.first) == 128)
+            big += (long long)((s + 127) / 128) * ((cr.second - cr.first + 127) / 128);
+      }
+      const int W = big >= kMin128Tiles ? 128 : 64;
+      Launch pm{W == 128 ? 10 : 1, (int)h_gp.size(), 0, 0, 0, fkey(height, 60000000LL)};
       std::vector<GemmProbT<T>> cp;
       Launch cl{3, 0, 0, 0, 0, fkey(height, 60000001LL)};
       double cfl = 0;
@@ -456,10 +470,10 @@ void build_dense_plan(snumat *h) {
           T *dst = VV.tmp + toff;
           toff += (long long)m * s;
           T *src = Lp(i) + rr.first;
-          for (int n0 = 0; n0 < s; n0 += 64) {
-            const int nt = std::min(64, s - n0), kk = n0 + nt;
+          for (int n0 = 0; n0 < s; n0 += W) {
+            const int nt = std::min(W, s - n0), kk = n0 + nt;
             pm.np += add_gemm(h_gp, h_gs, pm.tiles, dst + (long long)n0 * m, m, m, nt,
-                              {GemmSeg{src, Ui(i) + (long long)n0 * s, a, s, kk}}, pm.flops);
+                              {GemmSeg{src, Ui(i) + (long long)n0 * s, a, s, kk}}, pm.flops, W);
           }
           cl.np += add_gemm(cp, h_gs, cl.tiles, src, a, m, s, {GemmSeg{dst, dst, m, m, 1}}, cfl);
         }
@@ -468,10 +482,10 @@ void build_dense_plan(snumat *h) {
           T *dst = VV.tmp + toff;
           toff += (long long)s * nc;
           T *src = R(i) + (long long)(s + cr.first) * s;
-          for (int m0 = 0; m0 < s; m0 += 64) {
-            const int mt = std::min(64, s - m0), kk = m0 + mt;
+          for (int m0 = 0; m0 < s; m0 += W) {
+            const int mt = std::min(W, s - m0), kk = m0 + mt;
             pm.np += add_gemm(h_gp, h_gs, pm.tiles, dst + m0, s, mt, nc,
-                              {GemmSeg{Li(i) + m0, src, s, s, kk}}, pm.flops);
+                              {GemmSeg{Li(i) + m0, src, s, s, kk}}, pm.flops, W);
           }
           cl.np += add_gemm(cp, h_gs, cl.tiles, src, s, s, nc, {GemmSeg{dst, dst, s, s, 1}}, cfl);
         }
@@ -1228,7 +1242,8 @@ void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
     case 1: launch_gemm(gp + L.p0, L.np, gs, L.tiles, 1, st); break;
     case 2: launch_gemm(gp + L.p0, L.np, gs, L.tiles, 0, st); break;
     case 3: launch_copy2d(gp + L.p0, L.np, gs, L.tiles, st); break;
-    case 5: launch_gemm128(gp + L.p0, L.np, gs, L.tiles, st); break;
+    case 5: launch_gemm128(gp + L.p0, L.np, gs, L.tiles, 0, st); break;
+    case 10: launch_gemm128(gp + L.p0, L.np, gs, L.tiles, 1, st); break;
     case 6: launch_gemm(gp + L.p0, L.np, gs, L.tiles, 2, st); break;
     case 7:
     case 8:
@@ -1331,7 +1346,7 @@ int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
     int g = 0;
     for (const Launch &L : plan.launches) {
       if (!is_gemm(L.kind)) continue;
-      gemm_ms[(L.kind == 1 || L.kind == 6) ? 1 : 0] += ev_ms((*pev)[2 * g], (*pev)[2 * g + 1]);
+      gemm_ms[is_assign_gemm(L.kind) ? 1 : 0] += ev_ms((*pev)[2 * g], (*pev)[2 * g + 1]);
       g++;
     }
   }

@@ -179,7 +179,7 @@ void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                  int total_tiles, int mode, hipStream_t st);
 template <typename T>
 void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
-                    int total_tiles, hipStream_t st);
+                    int total_tiles, int mode, hipStream_t st);
 template <typename T>
 void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                          int total_tiles, int tile, int kb, int depth, int m16, hipStream_t st);

@@ -964,14 +964,20 @@ void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
     hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st,
                        probs, nprob, segs);
 }
+// 128 x 128 tiles, mode 0 (C -= AB) or 1 (C = AB); loads one chunk ahead
+// (with accumulators in VGPRs the large tile is fastest that way in both
+// precisions)
 template <typename T>
 void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
-                    int total_tiles, hipStream_t st) {
+                    int total_tiles, int mode, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
   constexpr bool m16 = std::is_same<T, double>::value;
-  constexpr int depth = m16 ? 1 : 2;
-  hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
-                     nprob, segs);
+  if (mode == 1)
+    hipLaunchKernelGGL((k_gemm<T, 1, 4, 4, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
+                       nprob, segs);
+  else
+    hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
+                       nprob, segs);
 }
 // C -= AB variants for tuning: tile 64 or 128, K chunk 16 or 32, register
 // prefetch depth 1 or 2 (the tile count is that of the tile size)
@@ -1874,7 +1880,7 @@ int device_lds_limit() {
   template void launch_copy2d<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int, hipStream_t); \
   template void launch_gemm<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int, int,     \
                                hipStream_t);                                                  \
-  template void launch_gemm128<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,       \
+  template void launch_gemm128<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int, int,  \
                                   hipStream_t);                                               \
   template void launch_gemm_variant<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,  \
                                        int, int, int, int, hipStream_t);                      \
