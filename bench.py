@@ -127,24 +127,35 @@ def main():
     d_x.download(X16)
     berr16 = S.backward_error(A, X16, b16)
 
-    # dominant kernel: grouped FP64 MFMA GEMM (C -= sum A B), timed per launch
+    # dominant kernel: the grouped FP64 MFMA GEMM (C -= sum A B) instantiation
+    # (64 x 64 or 128 x 128 tiles) with the larger share of kernel time,
+    # timed per launch with HIP events on the library's stream
     M.set_profiling(True)
     M.factorize()
     M.set_profiling(False)
     pt = M.timings()
-    gemm_ms, gemm_launches, gemm_gflop = pt["gemm_sub_ms"], pt["gemm_sub_launches"], pt["gemm_sub_gflop"]
+    sp = M.gemm_split()
+    tiles = {64: (sp["t64_ms"], sp["t64_launches"], sp["t64_gflop"]),
+             128: (sp["t128_ms"], sp["t128_launches"], sp["t128_gflop"])}
+    dom = 128 if sp["t128_ms"] > sp["t64_ms"] else 64
+    gemm_ms, gemm_launches, gemm_gflop = tiles[dom]
+    oth_ms, oth_launches, oth_gflop = tiles[192 - dom]
     achieved = gemm_gflop / gemm_ms if gemm_ms > 0 else 0.0  # GFLOP/ms = TFLOP/s
+    blocks = 4 if dom == 128 else 2
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved, 3),
         "peak": PEAK_FP64_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-        "traffic": profiled_traffic(),
-        "kernel": "snu::k_gemm<double, 0, 2, 2, 16, 1, true> (grouped FP64 MFMA 16x16x4, C -= sum A B)",
+        "traffic": profiled_traffic(dom),
+        "kernel": f"snu::k_gemm<double, 0, {blocks}, {blocks}, 16, 1, true> "
+                  f"(grouped FP64 MFMA 16x16x4, {dom} x {dom} tiles, C -= sum A B)",
         "launches": int(gemm_launches),
         "avg_launch_ms": round(gemm_ms / max(gemm_launches, 1), 4),
         "gflop_per_launch": round(gemm_gflop / max(gemm_launches, 1), 4),
+        "other_tile": {"tile": 192 - dom, "ms": round(oth_ms, 3), "launches": int(oth_launches),
+                       "tflops": round(oth_gflop / oth_ms, 3) if oth_ms > 0 else None},
         "assign_variant": {"ms": round(pt["gemm_assign_ms"], 3),
                            "launches": int(pt["gemm_assign_launches"]),
                            "gflop": round(pt["gemm_assign_gflop"], 3)},
@@ -208,14 +219,15 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the dominant GEMM on this workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the counters'
-    reading for a copy kernel of known size); None if not recorded."""
+def profiled_traffic(tile):
+    """HBM bytes per launch of the dominant GEMM instantiation (tile 64 or
+    128) on this workload, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (scaled by the counters' reading for a copy kernel of
+    known size); None if not recorded."""
     import re
     path = os.path.join(ROOT, "profiles", "k_gemm_64cube_7pt_fp64.md")
     try:
-        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+        m = re.search(r"traffic_bytes_per_launch_tile%d:\s*([0-9.eE+]+)" % tile, open(path).read())
         return float(m.group(1)) if m else None
     except OSError:
         return None

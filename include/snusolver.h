@@ -230,6 +230,11 @@ int snumat_sep_block(const snumat_t *h, int i, int j, double *out);
  * launches; t[13] refinement steps of the last solve (SNU_FP32), t[14] its
  * final backward error (max over columns), t[15] reserved. */
 int snumat_timings(const snumat_t *h, double *t16);
+/* the C -= AB launches of the last factorize split by tile: out[0..2] the
+ * 64 x 64 ones (kernel time in ms with snumat_set_profiling, launches,
+ * GFLOP), out[3..5] the 128 x 128 ones.  Not part of the reference
+ * interface (profiling aid). */
+int snumat_gemm_split(const snumat_t *h, double *out6);
 /* algorithmic counts from the symbolic structure: c[0] F_dense (flops),
  * c[1] F_leaf (flops), c[2] leaf factor bytes, c[3] solve bytes per rhs. */
 int snumat_counts(const snumat_t *h, double *c4);

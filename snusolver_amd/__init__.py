@@ -357,6 +357,14 @@ class SnuMat:
                 "ir_steps", "ir_berr"]
         return {k: float(t[i]) for i, k in enumerate(keys)}
 
+    def gemm_split(self) -> dict:
+        """C -= AB launches of the last factorize by tile size (kernel ms
+        when profiling, launches, GFLOP)."""
+        t = (C.c_double * 6)()
+        _check(lib().snumat_gemm_split(self._h, t))
+        return {"t64_ms": t[0], "t64_launches": t[1], "t64_gflop": t[2],
+                "t128_ms": t[3], "t128_launches": t[4], "t128_gflop": t[5]}
+
     def counts(self) -> dict:
         c = (C.c_double * 4)()
         _check(lib().snumat_counts(self._h, c))

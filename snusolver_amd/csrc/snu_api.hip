@@ -235,7 +235,7 @@ int factorize_values(snumat *h, int stage) {
   fs.wd = &h->wd;
   fs.eps = h->opts.eps;
   static const DensePlan<T> empty;
-  double gms[2] = {0, 0};
+  double gms[3] = {0, 0, 0};
   int rc = run_factor<T>(fs, stage >= 2 ? VV.fact : empty, h->stream, &h->ev, h->profile && stage >= 2,
                          &h->pev, gms, h->use_graph && stage >= 2 ? &h->graph : nullptr);
   if (rc) return rc;
@@ -250,16 +250,24 @@ int factorize_values(snumat *h, int stage) {
   // t[7..9]: C -= AB GEMM launches (time when profiling, count, GFLOP);
   // t[10..12]: the same for the C = +-AB (triangular solve by inverse) ones
   for (int q = 7; q <= 12; q++) h->t[q] = 0;
+  for (double &v : h->gsplit) v = 0;
   if (stage >= 2) {
     for (const Launch &L : VV.fact.launches) {
       if (!is_gemm(L.kind)) continue;
       int base = is_assign_gemm(L.kind) ? 10 : 7;
       h->t[base + 1] += 1;
       h->t[base + 2] += L.flops * 1e-9;
+      if (L.kind == 2 || L.kind == 5) {
+        const int q = L.kind == 5 ? 3 : 0;
+        h->gsplit[q + 1] += 1;
+        h->gsplit[q + 2] += L.flops * 1e-9;
+      }
     }
     if (h->profile) {
       h->t[7] = gms[0];
       h->t[10] = gms[1];
+      h->gsplit[0] = gms[0] - gms[2];
+      h->gsplit[3] = gms[2];
     }
   }
   h->factored = stage >= 2;
@@ -515,6 +523,11 @@ extern "C" int snumat_sep_block(const snumat_t *h, int i, int j, double *out) {
 extern "C" int snumat_timings(const snumat_t *h, double *t16) {
   if (!h) return set_error(SNU_ERR_BAD_INPUT, "null");
   std::memcpy(t16, h->t, sizeof(h->t));
+  return SNU_OK;
+}
+extern "C" int snumat_gemm_split(const snumat_t *h, double *out6) {
+  if (!h || !out6) return set_error(SNU_ERR_BAD_INPUT, "null");
+  std::memcpy(out6, h->gsplit, sizeof(h->gsplit));
   return SNU_OK;
 }
 extern "C" int snumat_counts(const snumat_t *h, double *c4) {

@@ -159,6 +159,8 @@ struct snumat {
   snu::FactorGraph graph;
   std::vector<hipEvent_t> ev, pev;
   double t[16] = {0};
+  // C -= AB launches by tile: 64 x 64 (time ms, launches, GFLOP), then 128 x 128
+  double gsplit[6] = {0};
 };
 
 namespace snu {

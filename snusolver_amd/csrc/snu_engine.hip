@@ -1256,7 +1256,8 @@ void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
 
 // ev (optional): 6 events marking prepare / leaf rows / leaf sep / assembly /
 // dense; pev (optional, with profile): a begin/end pair per GEMM launch,
-// their times summed into gemm_ms[0] (C -= AB) and gemm_ms[1] (C = +-AB).
+// their times summed into gemm_ms[0] (C -= AB), gemm_ms[1] (C = +-AB) and
+// gemm_ms[2] (the C -= AB launches on 128 x 128 tiles, part of gemm_ms[0]).
 // The launches of one factorization on st (no host synchronisation).
 template <typename T>
 int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
@@ -1342,11 +1343,13 @@ int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
   if (stalled) return set_error(SNU_ERR_HIP, "leaf factorization: a row's wait for its pivot rows timed out");
   if (profile && pev && gemm_ms) {
     HIPCHK(hipStreamSynchronize(st));
-    gemm_ms[0] = gemm_ms[1] = 0;
+    gemm_ms[0] = gemm_ms[1] = gemm_ms[2] = 0;
     int g = 0;
     for (const Launch &L : plan.launches) {
       if (!is_gemm(L.kind)) continue;
-      gemm_ms[is_assign_gemm(L.kind) ? 1 : 0] += ev_ms((*pev)[2 * g], (*pev)[2 * g + 1]);
+      const double ms = ev_ms((*pev)[2 * g], (*pev)[2 * g + 1]);
+      gemm_ms[is_assign_gemm(L.kind) ? 1 : 0] += ms;
+      if (L.kind == 5) gemm_ms[2] += ms;
       g++;
     }
   }
