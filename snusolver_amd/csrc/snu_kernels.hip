@@ -672,22 +672,37 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
     }
   T ra[NA], rb[NB], ra2[NA], rb2[NB];
   // Loads are unconditional and unmasked with addresses clamped into the
-  // operands, so they stay branch-free and in flight across the MFMA block:
-  // a clamped row of A (column of B) only feeds a row (column) of the tile
-  // that is never stored; the K tail is zeroed in LDS after the store.
+  // operands, so they stay in flight across the MFMA block: a clamped row of
+  // A (column of B) only feeds a row (column) of the tile that is never
+  // stored; the K tail is zeroed in LDS after the store.  Element i of a
+  // thread's A chunk is row m0 + tid % BM, column kk0 + tid / BM + i (256 /
+  // BM); of its B chunk row kk0 + tid % KB, column n0 + tid / KB + i (256 /
+  // KB): on full chunks (no K tail) and full tiles (no N edge), the uniform
+  // case, the addresses are one base plus multiples of one stride.
+  static_assert(256 % BM == 0 && 256 % KB == 0, "thread-to-element map");
+  constexpr int ASTEP = 256 / BM, BSTEP = 256 / KB;
+  const int am = tid % BM, ak = tid / BM, bk = tid % KB, bn = tid / KB;
   auto load_into = [&](int sgi, int kk0, T (&xa)[NA], T (&xb)[NB]) {
     const GemmSegT<T> S = segs[P.seg0 + sgi];
+    const auto pa = gptr(S.A) + min(m0 + am, P.M - 1);
+    if (kk0 + KB <= S.K) {
+      const auto p = pa + (long long)(kk0 + ak) * S.lda;
+      const long long st = (long long)ASTEP * S.lda;
 #pragma unroll
-    for (int i = 0; i < NA; i++) {
-      int e = tid + 256 * i;
-      int gm = min(m0 + e % BM, P.M - 1), gk = min(kk0 + e / BM, S.K - 1);
-      xa[i] = gptr(S.A)[gm + (long long)gk * S.lda];
+      for (int i = 0; i < NA; i++) xa[i] = p[i * st];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NA; i++) xa[i] = pa[(long long)min(kk0 + ak + i * ASTEP, S.K - 1) * S.lda];
     }
+    const auto pb = gptr(S.B) + min(kk0 + bk, S.K - 1);
+    if (n0 + BN <= P.N) {
+      const auto p = pb + (long long)(n0 + bn) * S.ldb;
+      const long long st = (long long)BSTEP * S.ldb;
 #pragma unroll
-    for (int i = 0; i < NB; i++) {
-      int e = tid + 256 * i;
-      int gk = min(kk0 + e % KB, S.K - 1), gn = min(n0 + e / KB, P.N - 1);
-      xb[i] = gptr(S.B)[gk + (long long)gn * S.ldb];
+      for (int i = 0; i < NB; i++) xb[i] = p[i * st];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NB; i++) xb[i] = pb[(long long)min(n0 + bn + i * BSTEP, P.N - 1) * S.ldb];
     }
   };
   auto load = [&](int sgi, int kk0) { load_into(sgi, kk0, ra, rb); };
