@@ -72,7 +72,12 @@ typedef struct {
   int precision;   /* SNU_FP64 or SNU_FP32 */
   int device;      /* HIP device ordinal */
   uint64_t seed;   /* ordering seed (the ND is deterministic for a seed) */
-  int ir_max_steps;/* SNU_FP32: refinement steps per solve (default 10) */
+  int ir_max_steps;/* SNU_FP32: refinement steps per solve (0: default 10;
+                      negative: none, the FP32 sweep alone).  A snumat solve
+                      whose refinement ends above 1e-12 backward error is solved
+                      again with FP64 factors of the same matrix (built and
+                      factorized on first need, kept until the next
+                      factorize; about the memory of an FP64 handle) */
   double ir_tol;   /* SNU_FP32: target backward error per column (1e-15) */
 } snu_opts_t;
 
@@ -228,7 +233,8 @@ int snumat_sep_block(const snumat_t *h, int i, int j, double *out);
  * t[8] its launches, t[9] its algorithmic GFLOP (2 M N K per problem);
  * t[10..12] the same for the C = AB (triangular solve by block inverse)
  * launches; t[13] refinement steps of the last solve (SNU_FP32), t[14] its
- * final backward error (max over columns), t[15] reserved. */
+ * final backward error (max over columns), t[15] 1 when that solve fell back
+ * to FP64 factors (refinement stalled above 1e-12), else 0. */
 int snumat_timings(const snumat_t *h, double *t16);
 /* the C -= AB launches of the last factorize split by tile: out[0..2] the
  * 64 x 64 ones (kernel time in ms with snumat_set_profiling, launches,

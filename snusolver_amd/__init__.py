@@ -354,7 +354,7 @@ class SnuMat:
         keys = ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms",
                 "factor_ms", "solve_ms", "gemm_sub_ms", "gemm_sub_launches", "gemm_sub_gflop",
                 "gemm_assign_ms", "gemm_assign_launches", "gemm_assign_gflop",
-                "ir_steps", "ir_berr"]
+                "ir_steps", "ir_berr", "fp64_fallback"]
         return {k: float(t[i]) for i, k in enumerate(keys)}
 
     def gemm_split(self) -> dict:

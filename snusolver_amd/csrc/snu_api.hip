@@ -281,13 +281,45 @@ int factorize_stage(snumat_t *h, int stage) {
   int rc = ensure_stream(h);
   if (rc) return rc;
   h->factored = false;
+  h->fb64_current = false;
   return h->prec == SNU_FP32 ? factorize_values<float>(h, stage) : factorize_values<double>(h, stage);
 }
 
 // Solve with device pointers (caller's b / x, column-major).
+// FP64 twin of an FP32 handle (same plan and values, FP64 arena), factorized
+// for the handle's current values.
+int fp64_fallback(snumat *h) {
+  if (!h->fb64) {
+    snumat *f = new snumat();
+    f->P = h->P;
+    f->opts = h->opts;
+    f->opts.precision = SNU_FP64;
+    f->prec = SNU_FP64;
+    f->device = h->device;
+    f->on_device = true;
+    int rc = upload(f);
+    if (rc) {
+      free_device(f);
+      delete f;
+      return rc;
+    }
+    h->fb64 = f;
+    h->fb64_current = false;
+  }
+  if (!h->fb64_current) {
+    HIPCHK(hipMemcpy(h->fb64->d_aval, h->d_aval, sizeof(double) * (size_t)h->P.nnz,
+                     hipMemcpyDeviceToDevice));
+    int rc = factorize_stage(h->fb64, 2);
+    if (rc) return rc;
+    h->fb64_current = true;
+  }
+  return SNU_OK;
+}
+
 int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long long ldx, int nrhs) {
   int rc = ensure_stream(h);
   if (rc) return rc;
+  h->t[15] = 0;
   if ((rc = ensure_rhs(h, nrhs))) return rc;
   hipStream_t st = h->stream;
   const int n = h->P.n;
@@ -326,12 +358,20 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
     rs.slot_mat = &slot_mat;
     rs.anorm = &anorm;
     rs.d_active = h->d_active;
-    rs.max_steps = h->opts.ir_max_steps > 0 ? h->opts.ir_max_steps : 10;
+    rs.max_steps = h->opts.ir_max_steps > 0 ? h->opts.ir_max_steps : h->opts.ir_max_steps < 0 ? 0 : 10;
     rs.tol = h->opts.ir_tol > 0 ? h->opts.ir_tol : 1e-15;
     std::vector<double> berr;
     rc = run_refined(rs, h->v32.bp, h->v32.cbuf, h->v32.sp, st, &h->ir_steps, &berr);
     h->ir_berr = berr.empty() ? 0 : berr[0];
-    if (rc && rc != SNU_ERR_NOT_CONVERGED) return rc;
+    if (rc == SNU_ERR_NOT_CONVERGED) {
+      // refinement stalled above 1e-12: solve again with FP64 factors of
+      // the same matrix (t[15] = 1 reports it)
+      if ((rc = fp64_fallback(h))) return rc;
+      if ((rc = solve_dev(h->fb64, d_b, ldb, d_x, ldx, nrhs))) return rc;
+      HIPCHK(hipSetDevice(h->device));
+      h->t[15] = 1;
+    }
+    if (rc) return rc;
   } else {
     if ((rc = ensure_plan<double>(h, nrhs))) return rc;
     run_sweep<double>(ss, h->v64.bp, h->v64.cbuf, h->v64.sp, st);
@@ -393,6 +433,10 @@ extern "C" int snumat_solve(snumat_t *h, const double *b, double *x, int nrhs,
 
 extern "C" int snumat_destroy(snumat_t *h) {
   if (!h) return SNU_OK;
+  if (h->fb64) {
+    snumat_destroy(h->fb64);
+    h->fb64 = nullptr;
+  }
   if (h->on_device) {
     (void)hipSetDevice(h->device);
     free_device(h);

@@ -277,7 +277,7 @@ int batch_solve(snubatch *b, const double *const *d_b, double *const *d_x, const
     for (int m = 0; m < nm; m++) b->anorm[m] = b->mats[m]->anorm;
     rs.anorm = &b->anorm;
     rs.d_active = b->d_active;
-    rs.max_steps = o.ir_max_steps > 0 ? o.ir_max_steps : 10;
+    rs.max_steps = o.ir_max_steps > 0 ? o.ir_max_steps : o.ir_max_steps < 0 ? 0 : 10;
     rs.tol = o.ir_tol > 0 ? o.ir_tol : 1e-15;
     rc = run_refined(rs, (float *)b->bp, (float *)b->cbuf, b->s32, st, &b->ir_steps, &b->berr);
     if (rc && rc != SNU_ERR_NOT_CONVERGED) return rc;

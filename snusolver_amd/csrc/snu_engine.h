@@ -161,6 +161,11 @@ struct snumat {
   double t[16] = {0};
   // C -= AB launches by tile: 64 x 64 (time ms, launches, GFLOP), then 128 x 128
   double gsplit[6] = {0};
+  // SNU_FP32: FP64 factorization of the same matrix on the same device, built
+  // and factorized on the first solve whose refinement stalls above 1e-12,
+  // refactorized after every new factorization of this handle
+  snumat *fb64 = nullptr;
+  bool fb64_current = false;
 };
 
 namespace snu {

@@ -965,10 +965,10 @@ void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
   if (total_tiles <= 0 || nprob <= 0) return;
   // FP64 on the 16x16x4 instruction: measured faster than four rotated
   // 4x4x4 products per block on large products, equal on small ones.
-  // FP32: loads two chunks ahead (measured 8% faster on 64 x 64 tiles; the
-  // FP64 kernel gains nothing from it)
+  // Loads one chunk ahead (two ahead measured slower in both precisions
+  // once the accumulators stay in VGPRs)
   constexpr bool m16 = std::is_same<T, double>::value;
-  constexpr int depth = m16 ? 1 : 2;
+  constexpr int depth = 1;
   if (mode == 1)
     hipLaunchKernelGGL((k_gemm<T, 1, 2, 2, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st,
                        probs, nprob, segs);

@@ -217,6 +217,33 @@ def test_fp32_refined_multi_rhs(S):
     assert np.max(np.abs(X - X64)) <= REL_TOL * np.max(np.abs(X64))
 
 
+def test_fp32_stalled_refinement_falls_back_to_fp64(S):
+    """Refinement that cannot reach 1e-12 (forced from the host: no
+    correction steps allowed) is answered by FP64 factors of the same matrix:
+    the result equals the FP64 handle's bitwise, t[15] reports the fallback,
+    and a following factorize + solve of new values refactorizes the FP64
+    twin as well (same values scaled: x scales inversely)."""
+    A = S.gen_grid(20, 20, 10, 7, 0, 6.1, 3)
+    order, sizes = S.order_grid(20, 20, 10, 8)
+    _, b = S.gen_rhs(A, 1, 1)
+    b = b[:, 0]
+    with S.SnuMat(A, 8, order=order, sizes=sizes, device=0) as M64:
+        x64 = M64.factorize().solve(b)
+    with S.SnuMat(A, 8, order=order, sizes=sizes, device=0, precision="fp32", ir_max_steps=-1) as M:
+        x = M.factorize().solve(b)
+        t = M.timings()
+        assert t["fp64_fallback"] == 1 and t["ir_steps"] == 0 and t["ir_berr"] > 1e-12
+        assert np.array_equal(x, x64)
+        assert S.backward_error(A, x, b) <= BERR_TOL
+        M.update_values(2.0 * A.data)
+        x2 = M.factorize().solve(b)
+        assert M.timings()["fp64_fallback"] == 1
+        assert np.max(np.abs(2.0 * x2 - x64)) <= 1e-12 * np.max(np.abs(x64))
+    with S.SnuMat(A, 8, order=order, sizes=sizes, device=0, precision="fp32") as M:
+        x = M.factorize().solve(b)
+        assert M.timings()["fp64_fallback"] == 0
+
+
 def _batch_members(S):
     """Uneven matrices: 2D 9-point grids of several sides and leaf counts
     (np = 1 included), two 3D 7-point grids."""
