@@ -1063,6 +1063,20 @@ __global__ __launch_bounds__(256) void k_gemv(const GemvProbT<T> *__restrict__ p
       auto A = gptr(S.A) + mc;
       auto X = gptr(S.x) + (long long)c0 * S.ldx;
       int k = kb + w;
+      // eight columns of A in flight per wave (the loads are independent of
+      // the accumulation), added in two groups of four
+      for (; k + 28 < ke; k += 32) {
+        T a[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = A[(long long)(k + 4 * j) * S.lda];
+#pragma unroll
+        for (int c = 0; c < CG; c++)
+          if (c < nc) {
+            auto xc = X + (long long)c * S.ldx;
+            acc[c] += a[0] * xc[k] + a[1] * xc[k + 4] + a[2] * xc[k + 8] + a[3] * xc[k + 12];
+            acc[c] += a[4] * xc[k + 16] + a[5] * xc[k + 20] + a[6] * xc[k + 24] + a[7] * xc[k + 28];
+          }
+      }
       for (; k + 12 < ke; k += 16) {
         T a0 = A[(long long)k * S.lda], a1 = A[(long long)(k + 4) * S.lda];
         T a2 = A[(long long)(k + 8) * S.lda], a3 = A[(long long)(k + 12) * S.lda];
@@ -1137,6 +1151,20 @@ __global__ __launch_bounds__(256) void k_gemv_part(const GemvProbT<T> *__restric
       auto A = gptr(S.A) + mc;
       auto X = gptr(S.x) + (long long)c0 * S.ldx;
       int k = kb + w;
+      // eight columns of A in flight per wave (the loads are independent of
+      // the accumulation), added in two groups of four
+      for (; k + 28 < ke; k += 32) {
+        T a[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = A[(long long)(k + 4 * j) * S.lda];
+#pragma unroll
+        for (int c = 0; c < CG; c++)
+          if (c < nc) {
+            auto xc = X + (long long)c * S.ldx;
+            acc[c] += a[0] * xc[k] + a[1] * xc[k + 4] + a[2] * xc[k + 8] + a[3] * xc[k + 12];
+            acc[c] += a[4] * xc[k + 16] + a[5] * xc[k + 20] + a[6] * xc[k + 24] + a[7] * xc[k + 28];
+          }
+      }
       for (; k + 12 < ke; k += 16) {
         T a0 = A[(long long)k * S.lda], a1 = A[(long long)(k + 4) * S.lda];
         T a2 = A[(long long)(k + 8) * S.lda], a3 = A[(long long)(k + 12) * S.lda];
