@@ -290,7 +290,8 @@ int snumat_use_graph(snumat_t *h, int on);
  * Not part of the reference interface: they expose single kernels of this
  * library to the parity tests. */
 /* assign = 0: C -= A B (64 x 64 tiles), 2: C -= A B (128 x 128 tiles),
- * 1: C = A B, 3: C = -A B, 4: C = A B (128 x 128 tiles); col-major. */
+ * 1: C = A B, 3: C = -A B, 4: C = A B (128 x 128 tiles), 5: C = -A B
+ * (128 x 128 tiles); col-major. */
 int snu_test_gemm(int M, int N, int K, const double *A, int lda,
                   const double *B, int ldb, double *C, int ldc, int assign);
 int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,

@@ -603,7 +603,7 @@ int test_gemm(int M, int N, int K, const T *A, int lda, const T *B, int ldb, T *
   int tiles = 0;
   double fl = 0;
   add_gemm<T>(gp, gs, tiles, dC, ldc, M, N, {GemmSegT<T>{dA, dB, lda, ldb, K}}, fl,
-              assign == 2 || assign == 4 ? 128 : 64);
+              assign == 2 || assign >= 4 ? 128 : 64);
   GemmProbT<T> *dgp;
   GemmSegT<T> *dgs;
   HIPCHK(hipMalloc((void **)&dgp, sizeof(GemmProbT<T>) * std::max<size_t>(1, gp.size())));
@@ -611,7 +611,8 @@ int test_gemm(int M, int N, int K, const T *A, int lda, const T *B, int ldb, T *
   if (!gp.empty()) {
     HIPCHK(hipMemcpy(dgp, gp.data(), sizeof(GemmProbT<T>) * gp.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dgs, gs.data(), sizeof(GemmSegT<T>) * gs.size(), hipMemcpyHostToDevice));
-    if (assign == 2 || assign == 4) launch_gemm128<T>(dgp, (int)gp.size(), dgs, tiles, assign == 4 ? 1 : 0, nullptr);
+    if (assign == 2 || assign >= 4)
+      launch_gemm128<T>(dgp, (int)gp.size(), dgs, tiles, assign == 4 ? 1 : assign == 5 ? 2 : 0, nullptr);
     else launch_gemm<T>(dgp, (int)gp.size(), dgs, tiles, assign == 1 ? 1 : assign == 3 ? 2 : 0, nullptr);
     HIPCHK(hipGetLastError());
   }
@@ -649,8 +650,8 @@ int test_diag_lu(int nb, T *D, int ld, T *Linv, T *Uinv) {
 }  // namespace
 
 // C = C - A B (assign = 0), C = A B (1), C -= A B with 128 x 128 tiles (2),
-// C = -A B (3), C = A B with 128 x 128 tiles (4), one segment, through the
-// grouped MFMA kernel; col-major.
+// C = -A B (3), C = A B with 128 x 128 tiles (4), C = -A B with 128 x 128
+// tiles (5), one segment, through the grouped MFMA kernel; col-major.
 extern "C" int snu_test_gemm(int M, int N, int K, const double *A, int lda,
                              const double *B, int ldb, double *C, int ldc,
                              int assign) {

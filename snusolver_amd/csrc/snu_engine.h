@@ -27,17 +27,19 @@ namespace snu {
 // into one launch.
 struct Launch {
   int kind;  // 0 diag LU; GEMM (64 tile) 1: C = AB, 2: C -= AB, 6: C = -AB;
-             // GEMM (128 tile) 5: C -= AB, 10: C = AB; 3: copy C = A (GEMM
-             // descriptors, 64 x 64 tiles); GEMV 7: y = Ax, 8: y -= Ax
+             // GEMM (128 tile) 5: C -= AB, 10: C = AB, 11: C = -AB; 3: copy
+             // C = A (GEMM descriptors, 64 x 64 tiles); GEMV 7: y = Ax, 8: y -= Ax
   int p0, np, tiles;
   double flops;
   long long key;
   int wgs = 0;          // GEMV: workgroups of the split-K partial pass (0: unsplit)
   long long part = 0;   // GEMV: partial-sum values of the launch
 };
-inline bool is_gemm(int kind) { return kind == 1 || kind == 2 || kind == 5 || kind == 6 || kind == 10; }
+inline bool is_gemm(int kind) {
+  return kind == 1 || kind == 2 || kind == 5 || kind == 6 || kind == 10 || kind == 11;
+}
 // C = +-AB launches (products by triangular inverses) as opposed to C -= AB
-inline bool is_assign_gemm(int kind) { return kind == 1 || kind == 6 || kind == 10; }
+inline bool is_assign_gemm(int kind) { return kind == 1 || kind == 6 || kind == 10 || kind == 11; }
 
 // Launch list with its descriptors (device copies once uploaded).
 template <typename T>

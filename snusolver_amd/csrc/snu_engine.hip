@@ -333,6 +333,7 @@ void build_dense_plan(snumat *h) {
   using GemmSeg = GemmSegT<T>;
   using DiagProb = DiagProbT<T>;
   const Plan &P = h->P;
+  const int prec = std::is_same<T, float>::value ? SNU_FP32 : SNU_FP64;
   Values<T> &VV = vals<T>(h);
   T *dense = VV.arena + P.leaf_total;
   auto R = [&](int p) { return dense + P.offR[p]; };
@@ -412,6 +413,19 @@ void build_dense_plan(snumat *h) {
       Launch t2{6, 0, 0, 0, 0, fkey(height, base + 1)};
       std::vector<GemmProbT<T>> fin;
       long long toff = 0;
+      // per W-wide strip of each product, K limited to the triangle of the
+      // triangular factor it meets (the rest of that factor is zero)
+      long long big = 0;
+      for (int i = lo; i <= hi; i++) {
+        const int s = P.block_size[i];
+        for (int r0 = 0; r0 + hh < s; r0 += 2 * hh) {
+          const int hb = std::min(hh, s - r0 - hh);
+          if (tile_for(prec, hh, hb) == 128) big += 2LL * ((hh + 127) / 128) * ((hb + 127) / 128);
+        }
+      }
+      const int W = big >= kMin128Tiles ? 128 : 64;
+      t1.kind = W == 128 ? 10 : 1;
+      t2.kind = W == 128 ? 11 : 6;
       for (int i = lo; i <= hi; i++) {
         int s = P.block_size[i];
         for (int r0 = 0; r0 + hh < s; r0 += 2 * hh) {
@@ -420,16 +434,40 @@ void build_dense_plan(snumat *h) {
           toff += (long long)hh * hb;
           T *TL = VV.tmp + toff;
           toff += (long long)hb * hh;
-          t1.np += add_gemm(h_gp, h_gs, t1.tiles, TU, hh, hh, hb,
-                            {GemmSeg{R(i) + r0 + (long long)c0 * s, Ui(i) + c0 + (long long)c0 * s, s, s, hb}},
-                            t1.flops);
-          t1.np += add_gemm(h_gp, h_gs, t1.tiles, TL, hb, hb, hh,
-                            {GemmSeg{R(i) + c0 + (long long)r0 * s, Li(i) + r0 + (long long)r0 * s, s, s, hh}},
-                            t1.flops);
-          t2.np += add_gemm(fin, h_gs, t2.tiles, Ui(i) + r0 + (long long)c0 * s, s, hh, hb,
-                            {GemmSeg{Ui(i) + r0 + (long long)r0 * s, TU, s, hh, hh}}, t2.flops);
-          t2.np += add_gemm(fin, h_gs, t2.tiles, Li(i) + c0 + (long long)r0 * s, s, hb, hh,
-                            {GemmSeg{Li(i) + c0 + (long long)c0 * s, TL, s, hb, hb}}, t2.flops);
+          // TU = U12 X22, X22 upper triangular: columns [n0, n0 + W) need
+          // its rows [0, n0 + W)
+          for (int n0 = 0; n0 < hb; n0 += W) {
+            const int nt = std::min(W, hb - n0);
+            t1.np += add_gemm(h_gp, h_gs, t1.tiles, TU + (long long)n0 * hh, hh, hh, nt,
+                              {GemmSeg{R(i) + r0 + (long long)c0 * s, Ui(i) + c0 + (long long)(c0 + n0) * s,
+                                       s, s, n0 + nt}},
+                              t1.flops, W);
+          }
+          // TL = L21 X11, X11 lower triangular: columns [n0, n0 + W) need
+          // its rows [n0, hh)
+          for (int n0 = 0; n0 < hh; n0 += W) {
+            const int nt = std::min(W, hh - n0);
+            t1.np += add_gemm(h_gp, h_gs, t1.tiles, TL + (long long)n0 * hb, hb, hb, nt,
+                              {GemmSeg{R(i) + c0 + (long long)(r0 + n0) * s,
+                                       Li(i) + (r0 + n0) + (long long)(r0 + n0) * s, s, s, hh - n0}},
+                              t1.flops, W);
+          }
+          // Ui[r0:, c0:] = -X11 TU, X11 upper triangular: rows [m0, m0 + W)
+          // need its columns [m0, hh)
+          for (int m0 = 0; m0 < hh; m0 += W) {
+            const int mt = std::min(W, hh - m0);
+            t2.np += add_gemm(fin, h_gs, t2.tiles, Ui(i) + r0 + m0 + (long long)c0 * s, s, mt, hb,
+                              {GemmSeg{Ui(i) + (r0 + m0) + (long long)(r0 + m0) * s, TU + m0, s, hh, hh - m0}},
+                              t2.flops, W);
+          }
+          // Li[c0:, r0:] = -X22 TL, X22 lower triangular: rows [m0, m0 + W)
+          // need its columns [0, m0 + W)
+          for (int m0 = 0; m0 < hb; m0 += W) {
+            const int mt = std::min(W, hb - m0);
+            t2.np += add_gemm(fin, h_gs, t2.tiles, Li(i) + c0 + m0 + (long long)r0 * s, s, mt, hh,
+                              {GemmSeg{Li(i) + c0 + m0 + (long long)c0 * s, TL, s, hb, m0 + mt}},
+                              t2.flops, W);
+          }
         }
       }
       if (t1.np) out.push_back(t1);
@@ -446,7 +484,6 @@ void build_dense_plan(snumat *h) {
     // W = 128 (128 x 128 tiles) when the autotuner prefers that tile for the
     // level's panels and they hold enough such tiles to fill the device
     {
-      const int prec = std::is_same<T, float>::value ? SNU_FP32 : SNU_FP64;
       long long big = 0;
       for (int i = lo; i <= hi; i++) {
         const int s = P.block_size[i], a = P.anc_sum[i];
@@ -1244,6 +1281,7 @@ void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
     case 3: launch_copy2d(gp + L.p0, L.np, gs, L.tiles, st); break;
     case 5: launch_gemm128(gp + L.p0, L.np, gs, L.tiles, 0, st); break;
     case 10: launch_gemm128(gp + L.p0, L.np, gs, L.tiles, 1, st); break;
+    case 11: launch_gemm128(gp + L.p0, L.np, gs, L.tiles, 2, st); break;
     case 6: launch_gemm(gp + L.p0, L.np, gs, L.tiles, 2, st); break;
     case 7:
     case 8:

@@ -979,9 +979,9 @@ void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
     hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st,
                        probs, nprob, segs);
 }
-// 128 x 128 tiles, mode 0 (C -= AB) or 1 (C = AB); loads one chunk ahead
-// (with accumulators in VGPRs the large tile is fastest that way in both
-// precisions)
+// 128 x 128 tiles, mode 0 (C -= AB), 1 (C = AB) or 2 (C = -AB); loads one
+// chunk ahead (with accumulators in VGPRs the large tile is fastest that way
+// in both precisions)
 template <typename T>
 void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                     int total_tiles, int mode, hipStream_t st) {
@@ -989,6 +989,9 @@ void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *seg
   constexpr bool m16 = std::is_same<T, double>::value;
   if (mode == 1)
     hipLaunchKernelGGL((k_gemm<T, 1, 4, 4, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
+                       nprob, segs);
+  else if (mode == 2)
+    hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
                        nprob, segs);
   else
     hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,

@@ -104,12 +104,12 @@ def test_gemm_kernel_vs_numpy(S):
         A = np.asfortranarray(rng.standard_normal((M, K)))
         B = np.asfortranarray(rng.standard_normal((K, N)))
         C0 = np.asfortranarray(rng.standard_normal((M, N)))
-        for assign in (0, 1, 2, 3, 4):
+        for assign in (0, 1, 2, 3, 4, 5):
             Cg = C0.copy(order="F")
             rc = L.snu_test_gemm(M, N, K, A.ctypes.data_as(C.c_void_p), M, B.ctypes.data_as(C.c_void_p), K,
                                  Cg.ctypes.data_as(C.c_void_p), M, assign)
             assert rc == 0
-            ref = A @ B if assign in (1, 4) else (-(A @ B) if assign == 3 else C0 - A @ B)
+            ref = A @ B if assign in (1, 4) else (-(A @ B) if assign in (3, 5) else C0 - A @ B)
             err = np.max(np.abs(Cg - ref)) / max(1.0, np.max(np.abs(ref)))
             assert err < 1e-13, (M, N, K, assign, err)
 
@@ -150,13 +150,13 @@ def test_gemm_kernel_f32_vs_numpy(S):
         C0 = np.asfortranarray(rng.standard_normal((M, N)).astype(np.float32))
         A64, B64, C64 = A.astype(np.float64), B.astype(np.float64), C0.astype(np.float64)
         absprod = np.abs(A64) @ np.abs(B64)
-        for assign in (0, 1, 2, 3, 4):
+        for assign in (0, 1, 2, 3, 4, 5):
             Cg = C0.copy(order="F")
             rc = L.snu_test_gemm_f32(M, N, K, A.ctypes.data_as(C.c_void_p), M, B.ctypes.data_as(C.c_void_p), K,
                                      Cg.ctypes.data_as(C.c_void_p), M, assign)
             assert rc == 0
             P = A64 @ B64
-            ref = P if assign in (1, 4) else (-P if assign == 3 else C64 - P)
+            ref = P if assign in (1, 4) else (-P if assign in (3, 5) else C64 - P)
             bound = (K + 2) * u32 * (absprod + (np.abs(C64) if assign in (0, 2) else 0)) + 1e-30
             assert np.all(np.abs(Cg - ref) <= bound), (M, N, K, assign, np.max(np.abs(Cg - ref) / bound))
 
