@@ -547,16 +547,25 @@ __global__ __launch_bounds__(256) void k_assemble_schur(
         st[rr][lane] = v;
       }
       __syncthreads();
-      // add: wave w takes columns w, w + 4, ...; lane = tile row
+      // add: wave w takes columns w, w + 4, ...; lane = tile row.  The
+      // columns of one leaf go to distinct dense columns, so all 16 targets
+      // of a lane are read before any is written (one round trip, not 16)
       const int nc = min(64, L.nbc - c0);
-      for (int cc = wave; cc < nc; cc += 4) {
-        const T v = st[lane][cc];
-        if (v != T(0)) {
-          const long long idx =
-              sep_index(dd, bi, row0 + lane, L.bcol_blk[c0 + cc], L.bcol_loc[c0 + cc]);
-          dense[idx] += v;
+      T v[16], old[16];
+      long long idx[16];
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const int cc = wave + 4 * j;
+        v[j] = cc < nc ? st[lane][cc] : T(0);
+        idx[j] = 0;
+        if (v[j] != T(0)) {
+          idx[j] = sep_index(dd, bi, row0 + lane, L.bcol_blk[c0 + cc], L.bcol_loc[c0 + cc]);
+          old[j] = dense[idx[j]];
         }
       }
+#pragma unroll
+      for (int j = 0; j < 16; j++)
+        if (v[j] != T(0)) dense[idx[j]] = old[j] + v[j];
       __syncthreads();
     }
   }
