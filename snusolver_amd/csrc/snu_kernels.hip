@@ -1253,9 +1253,13 @@ void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
 // g, g + 8, ..., g + 56 of three 64 x 64 matrices in registers (the block is
 // padded with a unit diagonal): the factors being formed, X = L^-1 and
 // Y = U^-1.
-// Pivot step j (one barrier): row j of the factors (final: U row j) and of
-// X, column j of the factors and of Y are published in LDS; then with one
-// division m = (r > j ? A_rj : Y_rj) / U_jj per thread,
+// Pivot step j (one barrier): column j of the factors and of Y are published
+// in LDS (by the wave holding them); row j of the factors (final: U row j)
+// and of X is needed by each wave only on its own columns, which lane j of
+// the same wave holds, so it is read from that lane's registers (no LDS
+// round trip: the single-lane LDS stores of the row cost more than all the
+// arithmetic of a step); then with one division m = (r > j ? A_rj : Y_rj) /
+// U_jj per thread,
 //   rows r > j:  A_rc -= m U_jc (c > j), L_rj = m,  X_rc -= m X_jc (c <= j)
 //   rows r <= j: Y_rj = m,  Y_rc -= m U_jc (c > j)
 // which are right-looking LU, forward elimination L X = I (X_j is final at
@@ -1264,32 +1268,37 @@ void launch_gemv(const GemvProbT<T> *probs, int nprob, const GemvSegT<T> *segs,
 // of, at or right of J is a compile-time fact for most (J, q) and a uniform
 // select for the rest; runtime steps make the compiler shuffle the register
 // arrays between branches).
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int lane) {
+  if constexpr (std::is_same<T, double>::value) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+  } else {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+  }
+}
 template <typename T, int J>
 __device__ __forceinline__ void diag_step(T (&x)[8], T (&xl)[8], T (&xu)[8], int r, int g,
-                                          T (*ub)[64], T (*xr)[64], T (*lb)[64], T (*yb)[64]) {
+                                          T (*lb)[64], T (*yb)[64]) {
   constexpr int b = J & 1;
-  if (r == J) {
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      ub[b][g + 8 * q] = x[q];
-      xr[b][g + 8 * q] = xl[q];
-    }
-  }
   if (g == (J & 7)) {
     lb[b][r] = x[J >> 3];
     yb[b][r] = xu[J >> 3];
   }
-  __syncthreads();
-  // keep the scheduler from interleaving steps (register pressure)
-  __builtin_amdgcn_sched_barrier(0);
+  // row J on this wave's columns, from lane J
   T uq[8], xq[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    if (8 * q + 7 > J) uq[q] = ub[b][g + 8 * q];
-    if (8 * q <= J) xq[q] = xr[b][g + 8 * q];
+    if (8 * q + 7 > J) uq[q] = lane_value(x[q], J);
+    if (8 * q <= J) xq[q] = lane_value(xl[q], J);
   }
+  __syncthreads();
+  // keep the scheduler from interleaving steps (register pressure)
+  __builtin_amdgcn_sched_barrier(0);
   const bool below = r > J;
-  const T m = (below ? lb[b][r] : yb[b][r]) / ub[b][J];
+  // lb[b][J]: column J at row J, the pivot U_JJ
+  const T m = (below ? lb[b][r] : yb[b][r]) / lb[b][J];
   // multipliers zeroed on the lanes an update does not apply to (x - 0 u is
   // x exactly)
   const T mb = below ? m : T(0), ma = below ? T(0) : m;
@@ -1319,13 +1328,13 @@ __device__ __forceinline__ void diag_step(T (&x)[8], T (&xl)[8], T (&xu)[8], int
 }
 template <typename T, int... J>
 __device__ __forceinline__ void diag_steps(std::integer_sequence<int, J...>, T (&x)[8],
-                                           T (&xl)[8], T (&xu)[8], int r, int g, T (*ub)[64],
-                                           T (*xr)[64], T (*lb)[64], T (*yb)[64]) {
-  (diag_step<T, J>(x, xl, xu, r, g, ub, xr, lb, yb), ...);
+                                           T (&xl)[8], T (&xu)[8], int r, int g, T (*lb)[64],
+                                           T (*yb)[64]) {
+  (diag_step<T, J>(x, xl, xu, r, g, lb, yb), ...);
 }
 template <typename T>
 __global__ __launch_bounds__(512) void k_diag_lu(const DiagProbT<T> *__restrict__ probs) {
-  __shared__ T ub[2][64], xr[2][64], lb[2][64], yb[2][64];
+  __shared__ T lb[2][64], yb[2][64];
   const DiagProbT<T> P = probs[blockIdx.x];
   const int nb = P.nb, tid = threadIdx.x;
   const int r = tid & 63, g = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1338,7 +1347,7 @@ __global__ __launch_bounds__(512) void k_diag_lu(const DiagProbT<T> *__restrict_
   }
   // all 64 steps run: past nb the padding is an identity block, whose steps
   // change nothing (zero multipliers)
-  diag_steps<T>(std::make_integer_sequence<int, 64>{}, x, xl, xu, r, g, ub, xr, lb, yb);
+  diag_steps<T>(std::make_integer_sequence<int, 64>{}, x, xl, xu, r, g, lb, yb);
   // the descriptor is read again (through a pointer the compiler cannot
   // equate with the first), so no store address stays live across the steps
   const DiagProbT<T> *pq = probs;
