@@ -1607,8 +1607,10 @@ void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp
 // subtracted for all rows at once before the sweep, which then reads only
 // leaf columns.  Columns go in batches of cmax.
 // LDS: [level offsets | tasks at task_off | cmax x ls values at xs_off].
+// Threads per leaf: the waves take a level's rows in turn.
+constexpr int kLeafSolveThreads = 512;
 template <typename T, bool BWD>
-__global__ __launch_bounds__(256) void k_leaf_solve(const LeafDev *__restrict__ leaves,
+__global__ __launch_bounds__(kLeafSolveThreads) void k_leaf_solve(const LeafDev *__restrict__ leaves,
                                                     const VecDev *__restrict__ vd, T *bp,
                                                     int cmax, int task_off, int xs_off) {
   const LeafDev L = leaves[blockIdx.x];
@@ -1744,17 +1746,17 @@ void launch_leaf_solve(const LeafDev *leaves, int nleaves, const VecDev *vd, T *
   int dev = 0, cus = 0, per_cu = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kLeafSolveThreads, lds);
   if ((long long)nleaves > (long long)cus * std::max(per_cu, 1)) {
     if (bwd) launch_leaf_bwd(leaves, nleaves, vd, bp, st);
     else launch_leaf_fwd(leaves, nleaves, vd, bp, st);
     return;
   }
   if (bwd)
-    hipLaunchKernelGGL((k_leaf_solve<T, true>), dim3(nleaves), dim3(256), lds, st, leaves, vd, bp,
+    hipLaunchKernelGGL((k_leaf_solve<T, true>), dim3(nleaves), dim3(kLeafSolveThreads), lds, st, leaves, vd, bp,
                        cmax, task_off, xs_off);
   else
-    hipLaunchKernelGGL((k_leaf_solve<T, false>), dim3(nleaves), dim3(256), lds, st, leaves, vd, bp,
+    hipLaunchKernelGGL((k_leaf_solve<T, false>), dim3(nleaves), dim3(kLeafSolveThreads), lds, st, leaves, vd, bp,
                        cmax, task_off, xs_off);
 }
 
