@@ -1608,7 +1608,7 @@ void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp
 // leaf columns.  Columns go in batches of cmax.
 // LDS: [level offsets | tasks at task_off | cmax x ls values at xs_off].
 // Threads per leaf: the waves take a level's rows in turn.
-constexpr int kLeafSolveThreads = 512;
+constexpr int kLeafSolveThreads = 1024;
 template <typename T, bool BWD>
 __global__ __launch_bounds__(kLeafSolveThreads) void k_leaf_solve(const LeafDev *__restrict__ leaves,
                                                     const VecDev *__restrict__ vd, T *bp,
