@@ -350,6 +350,8 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
       const int nbr = h->P.leaves[r].nbr;
       for (int f = 0; f < nbr; f += sep_waves) W.wg_leaf.push_back(lbase + r), W.wg_first.push_back(f);
       for (int f = 0; f < nbr; f += 4) W.fwg_leaf.push_back(lbase + r), W.fwg_first.push_back(f);
+      if (h->h_leaves[r].nt > 0)
+        for (int f = 0; f < nbr; f += kTailRows) W.twg_leaf.push_back(lbase + r), W.twg_first.push_back(f);
     }
     const WorkLists &w = h->wl;
     W.trow_blk.insert(W.trow_blk.end(), w.trow_blk.begin(), w.trow_blk.end());
@@ -389,6 +391,7 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   wd.nleaves = (int)leaves.size();
   wd.nwg = (int)W.wg_leaf.size();
   wd.nfwg = (int)W.fwg_leaf.size();
+  wd.ntwg = (int)W.twg_leaf.size();
   wd.ntarget = (int)W.trow_glob.size();
   wd.ntile = (int)W.tile_blk.size();
   wd.max_nt = 0;
@@ -405,6 +408,7 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   if ((rc = bupload(b, &wd.leaves, leaves)) || (rc = bupload(b, &wd.wg_leaf, W.wg_leaf)) ||
       (rc = bupload(b, &wd.wg_first, W.wg_first)) || (rc = bupload(b, &wd.fwg_leaf, W.fwg_leaf)) ||
       (rc = bupload(b, &wd.fwg_first, W.fwg_first)) || (rc = bupload(b, &wd.trow_blk, W.trow_blk)) ||
+      (rc = bupload(b, &wd.twg_leaf, W.twg_leaf)) || (rc = bupload(b, &wd.twg_first, W.twg_first)) ||
       (rc = bupload(b, &wd.trow_loc, W.trow_loc)) || (rc = bupload(b, &wd.trow_glob, W.trow_glob)) ||
       (rc = bupload(b, &wd.tptr, W.tptr)) || (rc = bupload(b, &wd.tsrc_leaf, W.tsrc_leaf)) ||
       (rc = bupload(b, &wd.tsrc_ri, W.tsrc_ri)) || (rc = bupload(b, &b->d_prep, prep)) ||

@@ -81,6 +81,7 @@ struct Values {
 struct WorkLists {
   std::vector<int> wg_leaf, wg_first;    // leaf separator-row workgroups
   std::vector<int> fwg_leaf, fwg_first;  // forward boundary-row workgroups
+  std::vector<int> twg_leaf, twg_first;  // leaf tail-row workgroups (kTailRows rows)
   std::vector<int> trow_blk, trow_loc, trow_glob, tptr, tsrc_leaf, tsrc_ri;  // targets
   std::vector<int> tile_blk, tile_row0, tile_ptr, seg_leaf, seg_ri;          // assembly tiles
 };
@@ -91,6 +92,7 @@ struct WorkDev {
   int nleaves = 0;
   int *wg_leaf = nullptr, *wg_first = nullptr, nwg = 0;
   int *fwg_leaf = nullptr, *fwg_first = nullptr, nfwg = 0;
+  int *twg_leaf = nullptr, *twg_first = nullptr, ntwg = 0;
   int *trow_blk = nullptr, *trow_loc = nullptr, *trow_glob = nullptr, *tptr = nullptr;
   int *tsrc_leaf = nullptr, *tsrc_ri = nullptr, ntarget = 0;
   int *tile_blk = nullptr, *tile_row0 = nullptr, *tile_ptr = nullptr;

@@ -1044,6 +1044,9 @@ int upload(snumat *h) {
     for (int f = 0; f < P.leaves[r].nbr; f += h->wd.sep_waves) W.wg_leaf.push_back(r), W.wg_first.push_back(f);
   h->wd.max_nt = max_nt;
   for (int r = 0; r < np; r++)
+    if (h->h_leaves[r].nt > 0)
+      for (int f = 0; f < P.leaves[r].nbr; f += kTailRows) W.twg_leaf.push_back(r), W.twg_first.push_back(f);
+  for (int r = 0; r < np; r++)
     for (int f = 0; f < P.leaves[r].nbr; f += 4) W.fwg_leaf.push_back(r), W.fwg_first.push_back(f);
   struct Src { int R, leaf, ri; };
   std::vector<Src> src;
@@ -1135,9 +1138,11 @@ int upload(snumat *h) {
   wd.nleaves = np;
   wd.nwg = (int)W.wg_leaf.size();
   wd.nfwg = (int)W.fwg_leaf.size();
+  wd.ntwg = (int)W.twg_leaf.size();
   wd.ntarget = (int)W.trow_glob.size();
   wd.ntile = (int)W.tile_blk.size();
   if ((rc = dupload(h, &wd.wg_leaf, W.wg_leaf)) || (rc = dupload(h, &wd.wg_first, W.wg_first)) ||
+      (rc = dupload(h, &wd.twg_leaf, W.twg_leaf)) || (rc = dupload(h, &wd.twg_first, W.twg_first)) ||
       (rc = dupload(h, &wd.fwg_leaf, W.fwg_leaf)) || (rc = dupload(h, &wd.fwg_first, W.fwg_first)) ||
       (rc = dupload(h, &wd.trow_blk, W.trow_blk)) || (rc = dupload(h, &wd.trow_loc, W.trow_loc)) ||
       (rc = dupload(h, &wd.trow_glob, W.trow_glob)) || (rc = dupload(h, &wd.tptr, W.tptr)) ||
@@ -1311,7 +1316,7 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[2], st));
   launch_leaf_sep<T>(wd.leaves, wd.wg_leaf, wd.wg_first, wd.nwg, wd.sep_wsz, wd.sep_waves, st);
-  launch_leaf_tail<T>(wd.leaves, wd.nleaves, wd.wg_leaf, wd.wg_first, wd.nwg, wd.sep_waves,
+  launch_leaf_tail<T>(wd.leaves, wd.nleaves, wd.twg_leaf, wd.twg_first, wd.ntwg, kTailRows,
                       (const GemmProbT<T> *)wd.tail_gp, wd.tail_np, (const GemmSegT<T> *)wd.tail_gs,
                       wd.tail_tiles, wd.max_nt, st);
   HIPCHK(hipGetLastError());

@@ -132,6 +132,10 @@ int device_lds_limit();
 // Leaves whose own top separator (the rows after the leaf's split) has at
 // most this many rows finish their separator rows on it as dense row solves.
 constexpr int kMaxLeafTail = 160;
+// separator rows per workgroup of the leaf tail kernel (two per wave): the
+// packed tail triangle in LDS is loaded once per workgroup, so a workgroup
+// takes many rows
+constexpr int kTailRows = 32;
 
 template <typename T>
 void launch_fill(T *p, long long n, hipStream_t st);

@@ -413,7 +413,7 @@ __global__ __launch_bounds__(256) void k_leaf_tail_gather(const LeafDev *__restr
 // and are stored into the row's L entries on tail columns.  Lane j holds
 // columns j, j + 64, j + 128.
 template <typename T>
-__global__ __launch_bounds__(512) void k_leaf_tail_l(const LeafDev *__restrict__ leaves,
+__global__ __launch_bounds__(1024) void k_leaf_tail_l(const LeafDev *__restrict__ leaves,
                                                      const int *__restrict__ wg_leaf,
                                                      const int *__restrict__ wg_first, int group) {
 #pragma clang fp contract(off)
@@ -493,7 +493,7 @@ void launch_leaf_tail(const LeafDev *leaves, int nleaves, const int *wg_leaf, co
   const size_t lds_l = (size_t)max_nt * (max_nt + 1) / 2 * sizeof(T);
   (void)hipFuncSetAttribute((const void *)k_leaf_tail_l<T>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_l);
-  if (nwg > 0)  // the separator-row groups of `waves` rows, two rows per wave
+  if (nwg > 0)  // the tail-row groups of `waves` rows, two rows per wave
     hipLaunchKernelGGL(k_leaf_tail_l<T>, dim3(nwg), dim3(64 * ((waves + 1) / 2)), lds_l, st, leaves,
                        wg_leaf, wg_first, waves);
   // S -= tail multipliers x tail U on boundary columns: one grouped GEMM,
