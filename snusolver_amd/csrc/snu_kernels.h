@@ -121,7 +121,7 @@ struct GemvProbT {
   int parts, wg0;
   long long poff;
 };
-constexpr int kGemvPiece = 256;
+constexpr int kGemvPiece = 128;
 using GemmSeg = GemmSegT<double>;
 using GemmProb = GemmProbT<double>;
 using DiagProb = DiagProbT<double>;
