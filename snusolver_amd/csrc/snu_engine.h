@@ -196,7 +196,7 @@ void free_device(snumat *h);
 size_t estimate_device_bytes(const snumat *h);
 int device_alloc(snumat *h, void **p, size_t bytes);
 float ev_ms(hipEvent_t a, hipEvent_t b);
-int waves_for(int wsz, int vsize, int max_leaf);
+int waves_for(int wsz, int vsize, int max_leaf, int cap);
 
 template <typename T>
 int upload_list(T **p, const std::vector<T> &v);

@@ -75,9 +75,12 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
 
 // waves per leaf workgroup: the LDS of one workgroup holds the leaf's row
 // completion flags and one workspace row (wsz values) per wave
-int waves_for(int wsz, int vsize, int max_leaf) {
+// waves of a leaf workgroup: as many workspaces as the LDS holds, at most
+// cap (leaf rows: 16, FP32 workspaces being half the size; separator rows:
+// 8, their kernel needs more than 128 registers per lane)
+int waves_for(int wsz, int vsize, int max_leaf, int cap) {
   int fit = (device_lds_limit() - leaf_flag_bytes(max_leaf)) / std::max(1, wsz * vsize);
-  return std::min(8, fit);
+  return std::min(cap, fit);
 }
 
 // ---------------------------------------------------------------------------
@@ -1029,9 +1032,9 @@ int upload(snumat *h) {
   for (int r = 0; r < np; r++)
     h->wd.max_lev = std::max(h->wd.max_lev, (int)std::max(P.leaves[r].lev_ptr.size(), P.leaves[r].ulev_ptr.size()) - 1);
   const int vsize = h->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
-  h->wd.waves = waves_for(h->wd.wsz, vsize, h->wd.max_leaf);
+  h->wd.waves = waves_for(h->wd.wsz, vsize, h->wd.max_leaf, 16);
   h->wd.sep_wsz = h->wd.wsz;
-  h->wd.sep_waves = waves_for(h->wd.sep_wsz, vsize, 0);
+  h->wd.sep_waves = waves_for(h->wd.sep_wsz, vsize, 0, 8);
   if (h->wd.waves < 1)
     return set_error(SNU_ERR_OOM, "leaf workspace (" + std::to_string(maxw) +
                                       " columns) exceeds the LDS of one workgroup");

@@ -253,7 +253,7 @@ __device__ __forceinline__ void factor_row(int i, int pend, const int *__restric
 // wave.
 extern __shared__ __align__(16) char lds_raw[];
 template <typename T>
-__global__ __launch_bounds__(512) void k_leaf_rows(const LeafDev *__restrict__ leaves,
+__global__ __launch_bounds__(1024) void k_leaf_rows(const LeafDev *__restrict__ leaves,
                                                    int wsz, int fl, T eps, int *stall,
                                                    int *half_done, int split) {
   const int leaf = split ? blockIdx.x >> 1 : blockIdx.x, role = split ? blockIdx.x & 1 : 1;
