@@ -1033,6 +1033,20 @@ int upload(snumat *h) {
     h->wd.max_lev = std::max(h->wd.max_lev, (int)std::max(P.leaves[r].lev_ptr.size(), P.leaves[r].ulev_ptr.size()) - 1);
   const int vsize = h->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
   h->wd.waves = waves_for(h->wd.wsz, vsize, h->wd.max_leaf, 16);
+  {
+    // more leaves than CUs: narrower workgroups that keep every leaf
+    // resident at once (k per CU) when they still have 4 or more waves;
+    // a leaf's dependency chain, not its wave count, bounds its time
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        cus > 0 && np > cus) {
+      const int k = (np + cus - 1) / cus;
+      const int per = ((device_lds_limit() - leaf_flag_bytes(h->wd.max_leaf)) / k -
+                       leaf_flag_bytes(h->wd.max_leaf)) / std::max(1, h->wd.wsz * vsize);
+      if (per >= 4 && per < h->wd.waves) h->wd.waves = per;
+    }
+  }
   h->wd.sep_wsz = h->wd.wsz;
   h->wd.sep_waves = waves_for(h->wd.sep_wsz, vsize, 0, 8);
   if (h->wd.waves < 1)
