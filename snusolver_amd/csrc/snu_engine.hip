@@ -1035,8 +1035,9 @@ int upload(snumat *h) {
   h->wd.waves = waves_for(h->wd.wsz, vsize, h->wd.max_leaf, 16);
   {
     // more leaves than CUs: narrower workgroups that keep every leaf
-    // resident at once (k per CU) when they still have 4 or more waves;
-    // a leaf's dependency chain, not its wave count, bounds its time
+    // resident at once (k per CU) when they still have 6 or more waves
+    // (measured on 512 leaves: 7 resident waves beat 15 waves in two
+    // rounds, 4 resident waves lose to 7 in two rounds)
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
@@ -1044,7 +1045,7 @@ int upload(snumat *h) {
       const int k = (np + cus - 1) / cus;
       const int per = ((device_lds_limit() - leaf_flag_bytes(h->wd.max_leaf)) / k -
                        leaf_flag_bytes(h->wd.max_leaf)) / std::max(1, h->wd.wsz * vsize);
-      if (per >= 4 && per < h->wd.waves) h->wd.waves = per;
+      if (per >= 6 && per < h->wd.waves) h->wd.waves = per;
     }
   }
   h->wd.sep_wsz = h->wd.wsz;
