@@ -333,7 +333,7 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   for (snumat *h : b->mats) wsz = std::max(wsz, h->wd.wsz), max_leaf = std::max(max_leaf, h->wd.max_leaf);
   const int vsize = b->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
   const int waves = waves_for(wsz, vsize, max_leaf, 16);
-  const int sep_wsz = wsz, sep_waves = waves_for(sep_wsz, vsize, 0, 8);
+  const int sep_wsz = wsz, sep_waves = waves_for(sep_wsz, vsize, 0, vsize == 4 ? 16 : 8);
   if (waves < 1) return fail(set_error(SNU_ERR_OOM, "leaf workspace exceeds the LDS of one workgroup"));
   int cb = 0;
   std::vector<PrepDev> prep;

@@ -77,7 +77,7 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
 // completion flags and one workspace row (wsz values) per wave
 // waves of a leaf workgroup: as many workspaces as the LDS holds, at most
 // cap (leaf rows: 16, FP32 workspaces being half the size; separator rows:
-// 8, their kernel needs more than 128 registers per lane)
+// 16 in FP32, 8 in FP64, whose kernel needs more than 128 registers per lane)
 int waves_for(int wsz, int vsize, int max_leaf, int cap) {
   int fit = (device_lds_limit() - leaf_flag_bytes(max_leaf)) / std::max(1, wsz * vsize);
   return std::min(cap, fit);
@@ -1049,7 +1049,7 @@ int upload(snumat *h) {
     }
   }
   h->wd.sep_wsz = h->wd.wsz;
-  h->wd.sep_waves = waves_for(h->wd.sep_wsz, vsize, 0, 8);
+  h->wd.sep_waves = waves_for(h->wd.sep_wsz, vsize, 0, vsize == 4 ? 16 : 8);
   if (h->wd.waves < 1)
     return set_error(SNU_ERR_OOM, "leaf workspace (" + std::to_string(maxw) +
                                       " columns) exceeds the LDS of one workgroup");

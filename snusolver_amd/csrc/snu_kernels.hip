@@ -330,7 +330,7 @@ void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, in
 // halves into separate passes with smaller workspaces measured slower: the
 // cost is per pivot step, and the pass over the pivots is then made twice.)
 template <typename T>
-__global__ __launch_bounds__(512) void k_leaf_sep(const LeafDev *__restrict__ leaves,
+__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_leaf_sep(const LeafDev *__restrict__ leaves,
                                                   const int *__restrict__ wg_leaf,
                                                   const int *__restrict__ wg_first, int wsz) {
   T *lds = (T *)lds_raw;
@@ -347,7 +347,9 @@ __global__ __launch_bounds__(512) void k_leaf_sep(const LeafDev *__restrict__ le
   // missing the tail pivots' updates, to S (k_leaf_tail_l / _s finish both)
   const int i = L.brow[ri];
   const int pend = L.nt ? L.sep_tail[ri] : L.rowptr[i + 1];
-  factor_row<T, false, 16, 2>(i, pend, L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0),
+  // FP32: 16 waves fit the LDS; 12 prefetched chunks keep the registers
+  // within the 128 per lane that 16 waves leave
+  factor_row<T, false, sizeof(T) == 4 ? 12 : 16, 2>(i, pend, L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0),
                               nullptr, nullptr);
   if (L.nt) {
     T *trow = (T *)L.tail_l + (long long)ri * L.nt;
