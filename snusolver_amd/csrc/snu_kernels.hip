@@ -525,8 +525,9 @@ __device__ __forceinline__ long long sep_index(const DenseDev &d, int bi,
 // rows of the tile are contiguous: both sides coalesced.  Every element
 // receives its leaves' contributions in ascending rank order from one thread
 // (deterministic; no atomics); zero contributions are skipped.
+constexpr int kAsmWaves = 8;  // waves of an assembly workgroup
 template <typename T>
-__global__ __launch_bounds__(256) void k_assemble_schur(
+__global__ __launch_bounds__(64 * kAsmWaves) void k_assemble_schur(
     int ntile, const int *__restrict__ tile_blk, const int *__restrict__ tile_row0,
     const int *__restrict__ tile_ptr, const int *__restrict__ seg_leaf,
     const int *__restrict__ seg_ri, const LeafDev *__restrict__ leaves) {
@@ -541,23 +542,24 @@ __global__ __launch_bounds__(256) void k_assemble_schur(
     const T *S = (const T *)L.S;
     const int *ri = seg_ri + (long long)sg * 64;
     for (int c0 = 0; c0 < L.nbc; c0 += 64) {
-      // stage: wave w reads tile rows w, w + 4, ... (64 columns each)
-      for (int rr = wave; rr < 64; rr += 4) {
+      // stage: wave w reads tile rows w, w + kAsmWaves, ... (64 columns each)
+      for (int rr = wave; rr < 64; rr += kAsmWaves) {
         const int r = ri[rr];
         T v = T(0);
         if (r >= 0 && c0 + lane < L.nbc) v = S[(long long)r * L.nbc + c0 + lane];
         st[rr][lane] = v;
       }
       __syncthreads();
-      // add: wave w takes columns w, w + 4, ...; lane = tile row.  The
-      // columns of one leaf go to distinct dense columns, so all 16 targets
-      // of a lane are read before any is written (one round trip, not 16)
+      // add: wave w takes columns w, w + kAsmWaves, ...; lane = tile row.
+      // The columns of one leaf go to distinct dense columns, so all targets
+      // of a lane are read before any is written (one round trip)
       const int nc = min(64, L.nbc - c0);
-      T v[16], old[16];
-      long long idx[16];
+      constexpr int NJ = 64 / kAsmWaves;
+      T v[NJ], old[NJ];
+      long long idx[NJ];
 #pragma unroll
-      for (int j = 0; j < 16; j++) {
-        const int cc = wave + 4 * j;
+      for (int j = 0; j < NJ; j++) {
+        const int cc = wave + kAsmWaves * j;
         v[j] = cc < nc ? st[lane][cc] : T(0);
         idx[j] = 0;
         if (v[j] != T(0)) {
@@ -566,7 +568,7 @@ __global__ __launch_bounds__(256) void k_assemble_schur(
         }
       }
 #pragma unroll
-      for (int j = 0; j < 16; j++)
+      for (int j = 0; j < NJ; j++)
         if (v[j] != T(0)) dense[idx[j]] = old[j] + v[j];
       __syncthreads();
     }
@@ -577,7 +579,7 @@ void launch_assemble_schur(int ntile, const int *tile_blk, const int *tile_row0,
                            const int *tile_ptr, const int *seg_leaf, const int *seg_ri,
                            const LeafDev *leaves, hipStream_t st) {
   if (ntile <= 0) return;
-  hipLaunchKernelGGL(k_assemble_schur<T>, dim3(ntile), dim3(256), 0, st, ntile, tile_blk,
+  hipLaunchKernelGGL(k_assemble_schur<T>, dim3(ntile), dim3(64 * kAsmWaves), 0, st, ntile, tile_blk,
                      tile_row0, tile_ptr, seg_leaf, seg_ri, leaves);
 }
 
