@@ -525,7 +525,7 @@ __device__ __forceinline__ long long sep_index(const DenseDev &d, int bi,
 // rows of the tile are contiguous: both sides coalesced.  Every element
 // receives its leaves' contributions in ascending rank order from one thread
 // (deterministic; no atomics); zero contributions are skipped.
-constexpr int kAsmWaves = 8;  // waves of an assembly workgroup
+constexpr int kAsmWaves = 16;  // waves of an assembly workgroup
 template <typename T>
 __global__ __launch_bounds__(64 * kAsmWaves) void k_assemble_schur(
     int ntile, const int *__restrict__ tile_blk, const int *__restrict__ tile_row0,
