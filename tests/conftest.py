@@ -12,7 +12,7 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device 0)")
-    # build what is missing (the GPU box runs on the prebuilt files)
+    # build what is missing (a machine without the toolchain runs on the prebuilt files)
     lib = os.path.join(ROOT, "snusolver_amd", "libsnusolver.so")
     if not os.path.exists(lib):
         subprocess.run(["make", "-C", os.path.join(ROOT, "snusolver_amd", "csrc"), "-j8"], check=True)
