@@ -149,7 +149,7 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
         "traffic": profiled_traffic(dom),
-        "kernel": f"snu::k_gemm<double, 0, {blocks}, {blocks}, 16, 1, true> "
+        "kernel": f"snu::k_gemm<double, 0, {blocks}, {blocks}, 16, 1> "
                   f"(grouped FP64 MFMA 16x16x4, {dom} x {dom} tiles, C -= sum A B)",
         "launches": int(gemm_launches),
         "avg_launch_ms": round(gemm_ms / max(gemm_launches, 1), 4),
@@ -160,11 +160,11 @@ def main():
                            "launches": int(pt["gemm_assign_launches"]),
                            "gflop": round(pt["gemm_assign_gflop"], 3)},
     }
+    work = M.work()
     M.close()
     d_b.free()
     d_x.free()
 
-    factor_flops = counts["f_dense"] + counts["f_leaf"]
     line = {
         "metric": METRIC,
         "value": round(1e3 / ms_per_step, 4),
@@ -187,18 +187,23 @@ def main():
         "solve_ms": round(tim["solve_ms"], 3),
         "phases_ms": {k: round(tim[k], 3) for k in
                       ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms"]},
-        "factor_gflops": round(factor_flops / (tim["factor_ms"] * 1e6), 2),
-        # whole-phase fractions of roofline: algorithmic factor flops (dense
-        # nominal, zero tiles included although they are skipped, + leaf
-        # structural) over the FP64 MFMA peak, and the solve's compulsory
-        # bytes (factors + vectors) over HBM bandwidth
-        "factor_nominal_frac_of_fp64_peak": round(factor_flops / (tim["factor_ms"] * 1e-3) / (PEAK_FP64_TFLOPS * 1e12), 4),
-        "solve_hbm_gbs": round(counts["b_solve"] / (tim["solve_ms"] * 1e6), 1),
-        "solve_frac_of_hbm": round(counts["b_solve"] / (tim["solve_ms"] * 1e-3) / PEAK_HBM_BYTES, 4),
-        "factor_flops_algorithmic": {"dense": counts["f_dense"], "leaf": counts["f_leaf"]},
+        # achieved work: the flops the dense launches execute (zero tiles
+        # skipped) over the dense phase's time, and the bytes one solve sweep
+        # reads (factors actually touched + vectors) over the solve time
+        "dense_executed_flops": work["dense_flops"],
+        "dense_tflops": round(work["dense_flops"] / (tim["dense_ms"] * 1e9), 3),
+        "dense_frac_of_fp64_peak": round(work["dense_flops"] / (tim["dense_ms"] * 1e9) / PEAK_FP64_TFLOPS, 4),
+        "dense_launches": int(work["dense_launches"]),
+        "diag_lu_launches": int(work["diag_lu_launches"]),
+        "solve_executed_bytes_per_rhs": work["solve_bytes"],
+        "solve_hbm_gbs": round(work["solve_bytes"] / (tim["solve_ms"] * 1e6), 1),
+        "solve_frac_of_hbm": round(work["solve_bytes"] / (tim["solve_ms"] * 1e-3) / PEAK_HBM_BYTES, 4),
+        # nominal counts (zero tiles included, as the reference's dense
+        # blocks would compute them): not achieved work
+        "nominal_flops": {"dense": counts["f_dense"], "leaf": counts["f_leaf"]},
+        "nominal_solve_bytes_per_rhs": counts["b_solve"],
         "solves_per_s_nrhs1": round(solves1, 2),
         "solves_per_s_nrhs16": round(solves16, 2),
-        "solve_bytes_per_rhs": counts["b_solve"],
         "backward_error": berr,
         "backward_error_nrhs16": berr16,
         "construct_s": round(construct_s, 3),
@@ -422,11 +427,46 @@ def batch_leg(S, np, count=512):
     return out
 
 
+def host_cpu():
+    """The host cores this job may use: the CPUs of its affinity mask,
+    limited by its cgroup CPU quota and by OMP_NUM_THREADS when the
+    environment sets one (the job's share of a shared host); plus what they
+    are (all figures are reported)."""
+    info = {"os_cpu_count": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        info["affinity_cpus"] = os.cpu_count() or 1
+    cores = info["affinity_cpus"]
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            info["cgroup_quota_cpus"] = round(int(quota) / int(period), 2)
+            cores = min(cores, max(1, int(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        info["omp_num_threads"] = int(omp)
+        cores = min(cores, int(omp))
+    info["cpu_model"] = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return cores, info
+
+
 def cpu_baseline(A, grid, np_, b, threads):
     """The oracle's MKL host path (reference offlvl = -1 semantics), one full
-    factor+solve of the same workload, on this host's cores."""
-    cores = threads or min(16, os.cpu_count() or 1)
-    os.environ.setdefault("MKL_NUM_THREADS", str(cores))
+    factor+solve of the same workload, on every host core this job may use
+    (leaves one per thread, dense levels through MKL's threads)."""
+    usable, info = host_cpu()
+    cores = threads or usable
+    os.environ["MKL_NUM_THREADS"] = str(cores)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import snusolver_amd as S
     from oracle import Oracle, dense_backend
@@ -441,6 +481,7 @@ def cpu_baseline(A, grid, np_, b, threads):
         "value": round(1.0 / dt, 5),
         "unit": "factorize+solve/s",
         "cores": cores,
+        "host": info,
         "kind": "port",
         "sample": "one full factorize+solve of the same 64^3 workload (oracle C restatement, "
                   f"leaves on {cores} threads, dense levels through "

@@ -54,8 +54,8 @@ extern "C" {
                       (no reference counterpart) */
 
 /* Same field layout as the reference's csr_matrix (src/matrix.h:9-13):
- * int32 n, m, nnz; rowptr[n+1]; colidx[nnz] (0-based, column-sorted rows);
- * double data[nnz]. */
+ * int32 n, m, nnz; rowptr[n+1]; colidx[nnz] (0-based; any column order within
+ * a row, duplicates rejected); double data[nnz]. */
 typedef struct {
   int n, m, nnz;
   int *rowptr, *colidx;
@@ -197,6 +197,9 @@ int snu_solve_csr(const snu_csr_t *A, const double *b, double *x,
 
 const char *snu_last_error(void);
 
+/* Hash of the sources the library was built from (16 hex digits). */
+const char *snu_build_id(void);
+
 /* ---------------- introspection (tests, bench) ---------------- */
 /* info[0..15]: n, nnz, np, num_block, max_level, total leaf LU nnz,
  * dense entries, max core_n, max leaf_size, max boundary rows, max boundary
@@ -244,6 +247,14 @@ int snumat_gemm_split(const snumat_t *h, double *out6);
 /* algorithmic counts from the symbolic structure: c[0] F_dense (flops),
  * c[1] F_leaf (flops), c[2] leaf factor bytes, c[3] solve bytes per rhs. */
 int snumat_counts(const snumat_t *h, double *c4);
+/* work the launch plans execute (structurally zero 64-tiles skipped):
+ * w[0] dense factor flops (2 M N K per GEMM problem, 4/3 nb^3 per diagonal
+ * block LU with its triangular inverses), w[1] dense factor launches, w[2]
+ * of which diagonal-block LU launches, w[3] bytes of one solve sweep per
+ * right-hand side (w[4] dense blocks and inverses read by the GEMVs, w[5]
+ * leaf LU values and indices, plus the solution vector read and written),
+ * w[6] dense solve launches.  Device handles only. */
+int snumat_work(const snumat_t *h, double *w8);
 
 /* ---------------- host utilities ---------------- */
 /* Synthetic grid operators (seeded splitmix64 stream, one draw per
@@ -263,9 +274,24 @@ int snu_order_grid(int nx, int ny, int nz, int np, int *order, int *sizes);
 int snu_order_nd(int n, const int *rowptr, const int *colidx, int np,
                  uint64_t seed, int *order, int *sizes);
 /* Matrix Market: read header (n, m, nnz after symmetric expansion), then
- * data into caller arrays (CSR, rows column-sorted). */
+ * data into caller arrays (CSR, rows column-sorted)
+ * (<- read_matrix / coo_to_csr, src/read_matrix.cpp:6-42,50-107). */
 int snu_mtx_size(const char *path, int *n, int *m, int *nnz, int *symmetric);
 int snu_mtx_read(const char *path, int *rowptr, int *colidx, double *val);
+/* Same with flags: SNU_MTX_FILE_ORDER keeps each row's entries in file
+ * order (the reference's coo_to_csr), SNU_MTX_NO_EXPAND reads only the
+ * stored triangle of symmetric files (the reference expands nothing,
+ * src/read_matrix.cpp:73-104).  Both together read exactly what the
+ * reference's read_matrix reads.  Pass the same flags to both calls. */
+#define SNU_MTX_FILE_ORDER 1
+#define SNU_MTX_NO_EXPAND 2
+int snu_mtx_size_ex(const char *path, int flags, int *n, int *m, int *nnz,
+                    int *symmetric);
+int snu_mtx_read_ex(const char *path, int flags, int *rowptr, int *colidx,
+                    double *val);
+/* Dense n x 1 vector (a right-hand side b): "array" or one-column
+ * "coordinate" file (absent coordinate entries are 0). */
+int snu_mtx_read_vector(const char *path, int n, double *x);
 int snu_mtx_write_vector(const char *path, const double *x, int n);
 /* relative backward error ||Ax-b||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
 double snu_backward_error(const snu_csr_t *A, const double *x,
@@ -298,10 +324,8 @@ int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,
                       const float *B, int ldb, float *C, int ldc, int assign);
 /* average time (ms) of the grouped C -= A B kernel on device-resident
  * operands, K split into nseg concatenated segments; tile 64 or 128 (K chunk
- * 16), 6432 / 12832 (K chunk 32), 6416 / 12816 (FP64 on the 16x16x4
- * instruction; 64162 / 128162 loading two chunks ahead, 64163 / 128163
- * with K chunk 32) or 642 / 1282 (K chunk 16, loads two
- * chunks ahead); prec SNU_FP64 or SNU_FP32. */
+ * 16), 6432 / 12832 (K chunk 32) or 642 / 1282 (K chunk 16, loads two chunks
+ * ahead); prec SNU_FP64 or SNU_FP32. */
 int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                    double *ms_out);
 /* tiles3: 64 or 128, the tile of the grouped C -= A B kernel per size bucket

@@ -71,6 +71,29 @@ class Oracle:
             raise RuntimeError(f"oracle_factor_solve failed: {rc}")
         return x
 
+    def factor_solve_columns(self, B, nthreads: int = 1) -> np.ndarray:
+        """The reference path has one right-hand side: each column of B is
+        solved by its own factor_solve."""
+        B = np.asarray(B, np.float64)
+        if B.ndim == 1:
+            return self.factor_solve(B, nthreads)
+        return np.stack([self.factor_solve(B[:, j], nthreads) for j in range(B.shape[1])], axis=1)
+
+    def assemble_only(self, b=None, nthreads: int = 1):
+        """Leaves and the assembly of the separator blocks only (stage 1)."""
+        b = np.zeros(self.A.n) if b is None else np.ascontiguousarray(b, np.float64)
+        rc = lib().oracle_assemble_only(self._h, _dp(b), nthreads)
+        if rc != 0:
+            raise RuntimeError(f"oracle_assemble_only failed: {rc}")
+
+    def sep_block(self, i: int, j: int) -> np.ndarray:
+        """Dense block (i, j) of the separator storage, s_i x s_j."""
+        _, sz = self.blocks()
+        si, sj = int(sz[i - 1]), int(sz[j - 1])
+        out = np.zeros(si * sj, np.float64)
+        lib().oracle_sep_block(self._h, i, j, _dp(out))
+        return out.reshape(si, sj, order="F")
+
     def leaf_factor(self, nthreads: int = 1):
         lib().oracle_leaf_factor(self._h, nthreads)
 

@@ -25,7 +25,7 @@ import numpy as np
 __all__ = [
     "SnuError", "CSR", "SnuMat", "lib", "gen_grid", "gen_rhs", "order_grid",
     "order_nd", "read_mtx", "backward_error", "solve", "config_matrix", "Batch",
-    "config_batch", "batch_sides", "batch_np", "Server",
+    "config_batch", "batch_sides", "batch_np", "Server", "read_vector", "write_vector",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -157,17 +157,37 @@ def order_nd(A: CSR, np_: int, seed: int = 0):
     return order, sizes
 
 
-def read_mtx(path: str) -> CSR:
+MTX_FILE_ORDER, MTX_NO_EXPAND = 1, 2
+
+
+def read_mtx(path: str, file_order: bool = False, expand: bool = True) -> CSR:
+    """Matrix Market coordinate file -> CSR.  Default: symmetric files
+    expanded, rows column-sorted.  file_order=True, expand=False reads what
+    the reference's read_matrix reads (rows in file order, stored triangle
+    only)."""
+    flags = (MTX_FILE_ORDER if file_order else 0) | (0 if expand else MTX_NO_EXPAND)
     n, m, nnz, sym = C.c_int(), C.c_int(), C.c_int(), C.c_int()
     p = path.encode()
-    _check(lib().snu_mtx_size(p, C.byref(n), C.byref(m), C.byref(nnz), C.byref(sym)))
+    _check(lib().snu_mtx_size_ex(p, flags, C.byref(n), C.byref(m), C.byref(nnz), C.byref(sym)))
     if n.value != m.value:
         raise SnuError(-1, "matrix is not square")
     rp = np.zeros(n.value + 1, np.int32)
     ci = np.zeros(nnz.value, np.int32)
     v = np.zeros(nnz.value, np.float64)
-    _check(lib().snu_mtx_read(p, _ip(rp), _ip(ci), _dp(v)))
+    _check(lib().snu_mtx_read_ex(p, flags, _ip(rp), _ip(ci), _dp(v)))
     return CSR(n.value, rp, ci, v)
+
+
+def read_vector(path: str, n: int) -> np.ndarray:
+    """Dense n x 1 Matrix Market vector (array or one-column coordinate)."""
+    x = np.zeros(n, np.float64)
+    _check(lib().snu_mtx_read_vector(path.encode(), n, _dp(x)))
+    return x
+
+
+def write_vector(path: str, x: np.ndarray):
+    x = np.ascontiguousarray(x, np.float64)
+    _check(lib().snu_mtx_write_vector(path.encode(), _dp(x), len(x)))
 
 
 class DeviceBuffer:
@@ -348,6 +368,20 @@ class SnuMat:
         _check(lib().snumat_sep_diag(self._h, e, _dp(out)))
         return out.reshape(s, s, order="F")
 
+    def factorize_stage(self, stage: int):
+        """Debug: stage 1 runs the leaves and the boundary Schur assembly only
+        (the separator blocks then hold A_SS plus the leaf contributions)."""
+        _check(lib().snumat_factorize_stage(self._h, stage))
+        return self
+
+    def sep_block(self, i: int, j: int) -> np.ndarray:
+        """Dense separator block (i, j) (i, j on one root path), s_i x s_j."""
+        sz = self.blocks()[1]
+        si, sj = int(sz[i - 1]), int(sz[j - 1])
+        out = np.zeros(si * sj, np.float64)
+        _check(lib().snumat_sep_block(self._h, i, j, _dp(out)))
+        return out.reshape(si, sj, order="F")
+
     def timings(self) -> dict:
         t = (C.c_double * 16)()
         _check(lib().snumat_timings(self._h, t))
@@ -364,6 +398,14 @@ class SnuMat:
         _check(lib().snumat_gemm_split(self._h, t))
         return {"t64_ms": t[0], "t64_launches": t[1], "t64_gflop": t[2],
                 "t128_ms": t[3], "t128_launches": t[4], "t128_gflop": t[5]}
+
+    def work(self) -> dict:
+        """Work the launch plans execute (structurally zero tiles skipped)."""
+        w = (C.c_double * 8)()
+        _check(lib().snumat_work(self._h, w))
+        return {"dense_flops": w[0], "dense_launches": w[1], "diag_lu_launches": w[2],
+                "solve_bytes": w[3], "solve_dense_bytes": w[4], "solve_leaf_bytes": w[5],
+                "solve_launches": w[6]}
 
     def counts(self) -> dict:
         c = (C.c_double * 4)()

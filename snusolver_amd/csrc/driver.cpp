@@ -6,7 +6,12 @@
 // ||Ax-b||_inf / (||A||_inf ||x||_inf + ||b||_inf) besides max|Ax-b|.
 //
 // usage: snu_driver <matrix.mtx> [--np N] [--out x.mtx] [--seed S]
-//                   [--device D] [--grid nx ny nz] [--nrhs K]
+//                   [--device D] [--grid nx ny nz] [--nrhs K] [--b b.mtx]
+//                   [--fp32] [--file-order]
+// --b reads one right-hand side from a Matrix Market vector file instead of
+// drawing it; --file-order reads the matrix as the reference's read_matrix
+// does (file order within rows, no symmetric expansion); --fp32 factors in
+// FP32 and refines in FP64.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -18,20 +23,24 @@
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "Usage: %s [matrix-market-filename] [--np N] [--out file] [--seed S] [--device D] [--grid nx ny nz] [--nrhs K]\n", argv[0]);
+    std::fprintf(stderr, "Usage: %s [matrix-market-filename] [--np N] [--out file] [--seed S] [--device D] "
+                 "[--grid nx ny nz] [--nrhs K] [--b file] [--fp32] [--file-order]\n", argv[0]);
     return 1;
   }
   snu_opts_t o;
   snu_default_opts(&o);
-  const char *out = nullptr;
+  const char *out = nullptr, *bfile = nullptr;
   unsigned long long seed = 1;
-  int nrhs = 1;
+  int nrhs = 1, flags = 0;
   for (int i = 2; i < argc; i++) {
     if (!std::strcmp(argv[i], "--np") && i + 1 < argc) o.np = std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--out") && i + 1 < argc) out = argv[++i];
     else if (!std::strcmp(argv[i], "--seed") && i + 1 < argc) seed = std::strtoull(argv[++i], nullptr, 10);
     else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) o.device = std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--nrhs") && i + 1 < argc) nrhs = std::atoi(argv[++i]);
+    else if (!std::strcmp(argv[i], "--b") && i + 1 < argc) bfile = argv[++i];
+    else if (!std::strcmp(argv[i], "--fp32")) o.precision = SNU_FP32;
+    else if (!std::strcmp(argv[i], "--file-order")) flags = SNU_MTX_FILE_ORDER | SNU_MTX_NO_EXPAND;
     else if (!std::strcmp(argv[i], "--grid") && i + 3 < argc) {
       o.ordering = SNU_ORDER_GRID;
       o.grid[0] = std::atoi(argv[++i]);
@@ -40,20 +49,31 @@ int main(int argc, char **argv) {
     }
   }
   int n, m, nnz, sym;
-  if (snu_mtx_size(argv[1], &n, &m, &nnz, &sym)) {
+  if (snu_mtx_size_ex(argv[1], flags, &n, &m, &nnz, &sym)) {
     std::fprintf(stderr, "%s\n", snu_last_error());
     return 1;
   }
   std::vector<int> rp(n + 1), ci(nnz);
   std::vector<double> v(nnz);
-  if (snu_mtx_read(argv[1], rp.data(), ci.data(), v.data())) {
+  if (snu_mtx_read_ex(argv[1], flags, rp.data(), ci.data(), v.data())) {
     std::fprintf(stderr, "%s\n", snu_last_error());
     return 1;
   }
   std::printf("Read matrix done! n=%d nnz=%d\n", n, nnz);
   snu_csr_t A{n, m, nnz, rp.data(), ci.data(), v.data()};
   std::vector<double> xt((size_t)n * nrhs), b((size_t)n * nrhs), x((size_t)n * nrhs);
-  snu_gen_rhs(&A, nrhs, seed, xt.data(), b.data());
+  if (bfile) {
+    if (nrhs != 1) {
+      std::fprintf(stderr, "--b reads one right-hand side (--nrhs 1)\n");
+      return 1;
+    }
+    if (snu_mtx_read_vector(bfile, n, b.data())) {
+      std::fprintf(stderr, "%s\n", snu_last_error());
+      return 1;
+    }
+  } else {
+    snu_gen_rhs(&A, nrhs, seed, xt.data(), b.data());
+  }
   auto t0 = std::chrono::steady_clock::now();
   snumat_t *h = nullptr;
   int rc = snumat_construct(&A, &o, &h);

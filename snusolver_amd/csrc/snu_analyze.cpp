@@ -304,13 +304,28 @@ int analyze(const snu_csr_t *A, const snu_opts_t *opts, Plan &P) {
     return set_error(SNU_ERR_BAD_INPUT, "null CSR arrays");
   if (np < 1 || (np & (np - 1)) || np > (1 << 20)) return set_error(SNU_ERR_BAD_INPUT, "np must be a power of two");
   if (A->rowptr[0] != 0 || A->rowptr[n] != nnz) return set_error(SNU_ERR_BAD_INPUT, "rowptr inconsistent with nnz");
-  for (int i = 0; i < n; i++) {
-    if (A->rowptr[i + 1] < A->rowptr[i]) return set_error(SNU_ERR_BAD_INPUT, "rowptr not monotone");
-    for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; q++) {
-      if (A->colidx[q] < 0 || A->colidx[q] >= n) return set_error(SNU_ERR_BAD_INPUT, "column index out of range");
-      if (q > A->rowptr[i] && A->colidx[q] <= A->colidx[q - 1])
-        return set_error(A->colidx[q] == A->colidx[q - 1] ? SNU_ERR_DUPLICATE : SNU_ERR_BAD_INPUT,
-                         A->colidx[q] == A->colidx[q - 1] ? "duplicate entry in a row" : "rows must be column-sorted");
+  // Rows may come in any column order (the reference's reader keeps the
+  // file order, src/read_matrix.cpp:28-35, and its permutate sorts,
+  // src/SnuMat_construct.cu:40-80): nothing below needs sorted input rows.
+  // Duplicate (row, col) entries are rejected.
+  {
+    std::vector<int> tmp;
+    for (int i = 0; i < n; i++) {
+      if (A->rowptr[i + 1] < A->rowptr[i]) return set_error(SNU_ERR_BAD_INPUT, "rowptr not monotone");
+      bool sorted = true;
+      for (int q = A->rowptr[i]; q < A->rowptr[i + 1]; q++) {
+        if (A->colidx[q] < 0 || A->colidx[q] >= n) return set_error(SNU_ERR_BAD_INPUT, "column index out of range");
+        if (q > A->rowptr[i] && A->colidx[q] <= A->colidx[q - 1]) {
+          if (A->colidx[q] == A->colidx[q - 1]) return set_error(SNU_ERR_DUPLICATE, "duplicate entry in a row");
+          sorted = false;
+        }
+      }
+      if (!sorted) {
+        tmp.assign(A->colidx + A->rowptr[i], A->colidx + A->rowptr[i + 1]);
+        std::sort(tmp.begin(), tmp.end());
+        if (std::adjacent_find(tmp.begin(), tmp.end()) != tmp.end())
+          return set_error(SNU_ERR_DUPLICATE, "duplicate entry in a row");
+      }
     }
   }
   P.n = n; P.nnz = nnz; P.np = np;

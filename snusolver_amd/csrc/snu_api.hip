@@ -551,17 +551,30 @@ extern "C" int snumat_sep_diag(const snumat_t *h, int e, double *out) {
   return read_values(h, h->P.leaf_total + h->P.offR[e], (size_t)s * s, out);
 }
 extern "C" int snumat_sep_block(const snumat_t *h, int i, int j, double *out) {
+  if (!h || !out) return set_error(SNU_ERR_BAD_INPUT, "null handle or output");
   const Plan &P = h->P;
-  if (!h || i < 1 || j < 1 || i >= P.np || j >= P.np) return set_error(SNU_ERR_BAD_INPUT, "bad block");
+  if (i < 1 || j < 1 || i >= P.np || j >= P.np) return set_error(SNU_ERR_BAD_INPUT, "bad block");
+  if (!h->on_device) return set_error(SNU_ERR_NO_DEVICE, "handle was built host-only");
   int x = std::max(i, j), y = std::min(i, j);
   while (x > y) x /= 2;
   if (x != y) return set_error(SNU_ERR_BAD_INPUT, "blocks not on one root path");
-  int si = P.block_size[i], sj = P.block_size[j];
-  std::vector<double> arena((size_t)P.dense_total);
-  int rc = read_values(h, P.leaf_total, arena.size(), arena.data());
-  if (rc) return rc;
-  for (int c = 0; c < sj; c++)
-    for (int r = 0; r < si; r++) out[r + (size_t)c * si] = arena[P.sep_index(i, r, j, c)];
+  const int si = P.block_size[i], sj = P.block_size[j];
+  if (!si || !sj) return SNU_OK;
+  // every block is a strided view of the arena: column c of block (i, j)
+  // holds rows 0..si-1 contiguously at sep_index(i, 0, j, c), and the
+  // columns are equally spaced, so one 2-D copy reads exactly the block
+  const long long c0 = P.leaf_total + P.sep_index(i, 0, j, 0);
+  const long long ld = sj > 1 ? P.sep_index(i, 0, j, 1) - P.sep_index(i, 0, j, 0) : si;
+  HIPCHK(hipSetDevice(h->device));
+  if (h->prec == SNU_FP32) {
+    std::vector<float> tmp((size_t)si * sj);
+    HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * si, h->v32.arena + c0, sizeof(float) * ld,
+                       sizeof(float) * si, sj, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < tmp.size(); q++) out[q] = tmp[q];
+  } else {
+    HIPCHK(hipMemcpy2D(out, sizeof(double) * si, h->v64.arena + c0, sizeof(double) * ld,
+                       sizeof(double) * si, sj, hipMemcpyDeviceToHost));
+  }
   return SNU_OK;
 }
 extern "C" int snumat_timings(const snumat_t *h, double *t16) {
@@ -580,6 +593,51 @@ extern "C" int snumat_counts(const snumat_t *h, double *c4) {
   c4[1] = h->P.f_leaf;
   c4[2] = h->P.b_leaf;
   c4[3] = h->P.b_solve;
+  return SNU_OK;
+}
+
+namespace {
+// Work the launch plans execute (zero tiles skipped), as opposed to the
+// nominal dense counts of snumat_counts.
+template <typename T>
+void executed_work(const snumat *h, double *w) {
+  const Values<T> &VV = vals<T>(h);
+  double fl = 0, nl = 0, nd = 0;
+  for (const Launch &L : VV.fact.launches) {
+    nl += 1;
+    if (L.kind == 0) {
+      nd += 1;
+      for (int q = 0; q < L.np; q++) {
+        const double nb = VV.fact.dp[L.p0 + q].nb;
+        fl += 4.0 / 3.0 * nb * nb * nb;  // LU (2/3 nb^3) + both triangular inverses
+      }
+    } else if (is_gemm(L.kind)) {
+      fl += L.flops;
+    }
+  }
+  GemvPlan<T> sp;
+  build_solve_plan<T>(h, nullptr, nullptr, 1, sp);
+  double gb = 0;
+  for (const std::vector<Launch> *ls : {&sp.fwd, &sp.bwd})
+    for (const Launch &L : *ls) gb += L.flops;  // matrix bytes read by the launch
+  double lb = 0;
+  for (const LeafPlan &L : h->P.leaves) lb += (double)(sizeof(T) + 4) * L.lu_nnz;
+  w[0] = fl;
+  w[1] = nl;
+  w[2] = nd;
+  w[3] = gb + lb + 2.0 * sizeof(T) * h->P.n;
+  w[4] = gb;
+  w[5] = lb;
+  w[6] = (double)(sp.fwd.size() + sp.bwd.size());
+  w[7] = 0;
+}
+}  // namespace
+
+extern "C" int snumat_work(const snumat_t *h, double *w8) {
+  if (!h || !w8) return set_error(SNU_ERR_BAD_INPUT, "null");
+  if (!h->on_device) return set_error(SNU_ERR_NO_DEVICE, "handle was built host-only");
+  if (h->prec == SNU_FP32) executed_work<float>(h, w8);
+  else executed_work<double>(h, w8);
   return SNU_OK;
 }
 
@@ -711,7 +769,7 @@ extern "C" int snumat_set_profiling(snumat_t *h, int on) {
 // time over reps launches (HIP events).
 namespace {
 template <typename T>
-int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int m16, int reps,
+int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int reps,
                double *ms_out) {
   T *dA, *dB, *dC;
   HIPCHK(hipMalloc((void **)&dA, sizeof(T) * (size_t)M * K));
@@ -738,7 +796,7 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int m
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  auto run = [&]() { launch_gemm_variant<T>(dgp, 1, dgs, tiles, tile, kb, depth, m16, nullptr); };
+  auto run = [&]() { launch_gemm_variant<T>(dgp, 1, dgs, tiles, tile, kb, depth, nullptr); };
   run();
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipEventRecord(e0, nullptr));
@@ -761,20 +819,15 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int m
 extern "C" int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                               double *ms_out) {
   // tile: 64 or 128 (K chunk 16), 6432 / 12832 (K chunk 32), 642 / 1282
-  // (K chunk 16, loads two chunks ahead), 6416 / 12816 (FP64 on the
-  // 16x16x4 instruction; 64162 / 128162 with loads two chunks ahead,
-  // 64163 / 128163 with K chunk 32)
-  int t = tile, kb = 16, depth = 1, m16 = 0;
+  // (K chunk 16, loads two chunks ahead)
+  int t = tile, kb = 16, depth = 1;
   if (tile == 6432 || tile == 12832) t = tile / 100, kb = 32;
   if (tile == 642 || tile == 1282) t = tile / 10, depth = 2;
-  if (tile == 6416 || tile == 12816) t = tile / 100, m16 = 1;
-  if (tile == 64162 || tile == 128162) t = tile / 1000, m16 = 1, depth = 2;
-  if (tile == 64163 || tile == 128163) t = tile / 1000, m16 = 1, kb = 32;
   if (M < 1 || N < 1 || K < 1 || nseg < 1 || reps < 1 || (t != 64 && t != 128) ||
       (prec != SNU_FP64 && prec != SNU_FP32) || !ms_out)
     return set_error(SNU_ERR_BAD_INPUT, "bad gemm bench shape");
-  return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, t, kb, depth, m16, reps, ms_out)
-                          : bench_gemm<double>(M, N, K, nseg, t, kb, depth, m16, reps, ms_out);
+  return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, t, kb, depth, reps, ms_out)
+                          : bench_gemm<double>(M, N, K, nseg, t, kb, depth, reps, ms_out);
 }
 
 // Average time (ms) of a plain device copy of `bytes` (multiple of 16) over

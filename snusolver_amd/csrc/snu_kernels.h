@@ -186,7 +186,7 @@ void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *seg
                     int total_tiles, int mode, hipStream_t st);
 template <typename T>
 void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
-                         int total_tiles, int tile, int kb, int depth, int m16, hipStream_t st);
+                         int total_tiles, int tile, int kb, int depth, hipStream_t st);
 // mode 0: y -= A x; 1: y = A x
 // Split-K GEMV: partial sums per (tile, piece) into part, then a fixed-order
 // sum per row applied to y (mode 1: y = sum, 0: y -= sum).

@@ -584,20 +584,15 @@ void launch_assemble_schur(int ntile, const int *tile_blk, const int *tile_row0,
 }
 
 // ---------------------------------------------------------------------------
-// dense: grouped GEMM on MFMA f64 4x4x4 (4 blocks)
+// dense: grouped GEMM on MFMA (FP64 v_mfma_f64_16x16x4f64, FP32
+// v_mfma_f32_16x16x4f32)
 // ---------------------------------------------------------------------------
-// On gfx950 v_mfma_f64_4x4x4f64 sustains twice the rate of
-// v_mfma_f64_16x16x4f64 (measured 74 vs 38 TFLOP/s with register operands).
-// Lane layouts (checked by lane-layout probes): A[m][k] in lane m + 16k,
-// B[k][n] in lane n + 16k (m, n < 16) -- the same registers as the 16x16x4
-// instruction -- but the 4 blocks compute only the 4 diagonal 4 x 4 blocks of
-// that 16 x 16 product: D_blk[i][j] in lane 4 blk + 16 i + j.  A full 16 x 16
-// block therefore takes 4 instructions whose second operand is rotated by
-// 4 r lanes inside each 16-lane row (DPP row_ror:(16 - 4r)).
-// Operands are swapped (first operand <- B tile, second <- A tile), so that
-// for rotation r lane l holds C[m0 + 4((blk + r) % 4) + l % 4][n0 + 4 blk + l / 16]
-// with blk = (l / 4) % 4: 16 lanes of a row cover 16 consecutive rows of
-// one column of C (coalesced epilogue).
+// Operands are swapped (first operand <- B tile, second <- A tile): the
+// product is D = B^T A^T, so the 16 lanes of a row group hold 16 consecutive
+// rows of one column of C (coalesced epilogue).  Lane layouts (checked by
+// lane-layout probes and the GEMM parity tests): A[i][k] in lane i + 16k,
+// B[k][j] in lane j + 16k; FP64 D[i][j] in lane j + 16 (i mod 4), register
+// i / 4; FP32 D[i][j] in lane j + 16 (i / 4), register i mod 4.
 //
 // Workgroup tile (32 BMB) x (32 BNB), 4 waves in 2 x 2, each (16 BMB) x
 // (16 BNB); K chunk 16; double-buffered LDS fed from a register prefetch that
@@ -635,11 +630,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // DEPTH: chunks of A and B in flight in registers ahead of the chunk being
 // multiplied (1: the next chunk's loads overlap one MFMA block; 2: loads are
 // issued two chunks ahead, alternating two register stages).
-// M16 (FP64 only): v_mfma_f64_16x16x4f64 per 16 x 16 block, operands
-// swapped as in the FP32 path (no DPP rotations); its D[i][j] is in lane
-// j + 16 (i mod 4), register i / 4 (the FP32 instruction: lane
-// j + 16 (i / 4), register i mod 4).
-template <typename T, int MODE, int BMB, int BNB, int KB, int DEPTH = 1, bool M16 = false>
+template <typename T, int MODE, int BMB, int BNB, int KB, int DEPTH = 1>
 __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ probs,
                                               int nprob,
                                               const GemmSegT<T> *__restrict__ segs) {
@@ -669,17 +660,13 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 16 * BMB, wn = (wave >> 1) * 16 * BNB;
   const int l16 = lane & 15, lk = lane >> 4;
-  // FP64: four 4x4x4 products per 16 x 16 block (one per rotation r);
-  // FP32: one 16x16x4 product per block, 4 output registers
-  double acc[BMB][BNB][4];
+  // one 16x16x4 product per 16 x 16 block, 4 output registers
   f4 accf[BMB][BNB];
   d4 accd[BMB][BNB];
 #pragma unroll
   for (int a = 0; a < BMB; a++)
 #pragma unroll
     for (int b = 0; b < BNB; b++) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) acc[a][b][r] = 0.0;
       accf[a][b] = f4{0.f, 0.f, 0.f, 0.f};
       accd[a][b] = d4{0.0, 0.0, 0.0, 0.0};
     }
@@ -788,28 +775,12 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
           for (int a = 0; a < BMB; a++) avk[kq][a] = As[buf][4 * kq + lk][wm + a * 16 + l16];
         }
         const T *bv = bvk[kq], *av = avk[kq];
-        if constexpr (F64 && M16) {
+        if constexpr (F64) {
 #pragma unroll
           for (int a = 0; a < BMB; a++)
 #pragma unroll
             for (int b = 0; b < BNB; b++)
               accd[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b], av[a], accd[a][b], 0, 0, 0);
-        } else if constexpr (F64) {
-          double avr[BMB][4];
-#pragma unroll
-          for (int a = 0; a < BMB; a++) {
-            avr[a][0] = av[a];
-            avr[a][1] = dpp_d<0x12C>(av[a]);  // lane i <- lane i + 4 (row of 16)
-            avr[a][2] = dpp_d<0x128>(av[a]);  // lane i <- lane i + 8
-            avr[a][3] = dpp_d<0x124>(av[a]);  // lane i <- lane i + 12
-          }
-#pragma unroll
-          for (int a = 0; a < BMB; a++)
-#pragma unroll
-            for (int b = 0; b < BNB; b++)
-#pragma unroll
-              for (int r = 0; r < 4; r++)
-                acc[a][b][r] = __builtin_amdgcn_mfma_f64_4x4x4f64(bv[b], avr[a][r], acc[a][b][r], 0, 0, 0);
         } else {
 #pragma unroll
           for (int a = 0; a < BMB; a++)
@@ -819,18 +790,13 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
         }
       }
   };
-  const int blk = (lane >> 2) & 3;
   // element (a, b, r) of this lane's accumulators: its place in C and value
   auto elem = [&](int a, int b, int r, int &m, int &n, T &v) __attribute__((always_inline)) {
-    if constexpr (F64 && M16) {
+    if constexpr (F64) {
       // D = B^T A^T: register r of lane l holds C[l % 16][4 r + l / 16]
       m = m0 + wm + a * 16 + l16;
       n = n0 + wn + b * 16 + 4 * r + lk;
       v = accd[a][b][r];
-    } else if constexpr (F64) {
-      m = m0 + wm + a * 16 + 4 * ((blk + r) & 3) + (lane & 3);
-      n = n0 + wn + b * 16 + 4 * blk + lk;
-      v = acc[a][b][r];
     } else {
       // D = B^T A^T: register r of lane l holds C[l % 16][4 (l / 16) + r]
       m = m0 + wm + a * 16 + l16;
@@ -976,21 +942,17 @@ template <typename T>
 void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                  int total_tiles, int mode, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
-  // FP64 on the 16x16x4 instruction: measured faster than four rotated
-  // 4x4x4 products per block on large products, equal on small ones.
-  // Loads one chunk ahead (two ahead measured slower in both precisions
+  // loads one chunk ahead (two ahead measured slower in both precisions
   // once the accumulators stay in VGPRs)
-  constexpr bool m16 = std::is_same<T, double>::value;
-  constexpr int depth = 1;
   if (mode == 1)
-    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st,
-                       probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+                       segs);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st,
-                       probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+                       segs);
   else
-    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, depth, m16>), dim3(total_tiles), dim3(256), 0, st,
-                       probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+                       segs);
 }
 // 128 x 128 tiles, mode 0 (C -= AB), 1 (C = AB) or 2 (C = -AB); loads one
 // chunk ahead (with accumulators in VGPRs the large tile is fastest that way
@@ -999,44 +961,31 @@ template <typename T>
 void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                     int total_tiles, int mode, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
-  constexpr bool m16 = std::is_same<T, double>::value;
   if (mode == 1)
-    hipLaunchKernelGGL((k_gemm<T, 1, 4, 4, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
-                       nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 1, 4, 4, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+                       segs);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
-                       nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+                       segs);
   else
-    hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1, m16>), dim3(total_tiles), dim3(256), 0, st, probs,
-                       nprob, segs);
+    hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+                       segs);
 }
 // C -= AB variants for tuning: tile 64 or 128, K chunk 16 or 32, register
 // prefetch depth 1 or 2 (the tile count is that of the tile size)
 template <typename T>
 void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
-                         int total_tiles, int tile, int kb, int depth, int m16, hipStream_t st) {
+                         int total_tiles, int tile, int kb, int depth, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
   const dim3 g(total_tiles), b(256);
-  if (m16 && std::is_same<T, double>::value) {
-    if (tile == 128) {
-      if (depth == 2) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 2, true>), g, b, 0, st, probs, nprob, segs);
-      else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 32, 1, true>), g, b, 0, st, probs, nprob, segs);
-      else hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1, true>), g, b, 0, st, probs, nprob, segs);
-    } else {
-      if (depth == 2) hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 2, true>), g, b, 0, st, probs, nprob, segs);
-      else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 32, 1, true>), g, b, 0, st, probs, nprob, segs);
-      else hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 1, true>), g, b, 0, st, probs, nprob, segs);
-    }
-    return;
-  }
   if (tile == 128) {
     if (depth == 2) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 2>), g, b, 0, st, probs, nprob, segs);
-    else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 32>), g, b, 0, st, probs, nprob, segs);
-    else hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16>), g, b, 0, st, probs, nprob, segs);
+    else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 32, 1>), g, b, 0, st, probs, nprob, segs);
+    else hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1>), g, b, 0, st, probs, nprob, segs);
   } else {
     if (depth == 2) hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 2>), g, b, 0, st, probs, nprob, segs);
-    else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 32>), g, b, 0, st, probs, nprob, segs);
-    else hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16>), g, b, 0, st, probs, nprob, segs);
+    else if (kb == 32) hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 32, 1>), g, b, 0, st, probs, nprob, segs);
+    else hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 1>), g, b, 0, st, probs, nprob, segs);
   }
 }
 
@@ -1953,7 +1902,7 @@ int device_lds_limit() {
   template void launch_gemm128<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int, int,  \
                                   hipStream_t);                                               \
   template void launch_gemm_variant<T>(const GemmProbT<T> *, int, const GemmSegT<T> *, int,  \
-                                       int, int, int, int, hipStream_t);                      \
+                                       int, int, int, hipStream_t);                           \
   template void launch_gemv_split<T>(const GemvProbT<T> *, int, const GemvSegT<T> *, int, int,  \
                                      int, T *, hipStream_t);                                    \
   template void launch_gemv<T>(const GemvProbT<T> *, int, const GemvSegT<T> *, int, int,     \

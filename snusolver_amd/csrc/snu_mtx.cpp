@@ -6,9 +6,13 @@
 // become 0-based, entries are bucketed into rows by a stable counting sort.
 // Extensions (documented): "integer" and "pattern" fields (pattern values are
 // 1.0), "symmetric" / "skew-symmetric" files are expanded, and each row is
-// sorted by column (stable), which the A + A^T merge of the ordering
-// requires (src/parmetis_driver.cpp:42-56).  Complex files are rejected, as
-// in the reference (src/read_matrix.cpp:73-78).
+// sorted by column (stable).  With SNU_MTX_FILE_ORDER the rows keep the
+// file's entry order, as the reference's coo_to_csr does; with
+// SNU_MTX_NO_EXPAND symmetric files are not expanded (only the stored
+// triangle is read, as in the reference, src/read_matrix.cpp:73-104).
+// Complex files are rejected, as in the reference (src/read_matrix.cpp:73-78).
+// Dense vectors (b, x) are read from "array" files or from coordinate files
+// with one column.
 #include <algorithm>
 #include <cctype>
 #include <cstdio>
@@ -49,12 +53,12 @@ int read_header(FILE *f, MtxHeader &h) {
 }
 
 int read_all(const char *path, MtxHeader &h, std::vector<int> &I,
-             std::vector<int> &J, std::vector<double> &V) {
+             std::vector<int> &J, std::vector<double> &V, bool expand) {
   FILE *f = fopen(path, "r");
   if (!f) return set_error(SNU_ERR_BAD_INPUT, std::string("mtx: cannot open ") + path);
   int rc = read_header(f, h);
   if (rc) { fclose(f); return rc; }
-  I.reserve(h.entries * (h.symmetric || h.skew ? 2 : 1));
+  I.reserve((size_t)h.entries * (expand && (h.symmetric || h.skew) ? 2 : 1));
   for (int e = 0; e < h.entries; e++) {
     int i, j;
     double v = 1.0;
@@ -62,7 +66,7 @@ int read_all(const char *path, MtxHeader &h, std::vector<int> &I,
     if (got != (h.pattern ? 2 : 3)) { fclose(f); return set_error(SNU_ERR_BAD_INPUT, "mtx: truncated entry list"); }
     if (i < 1 || i > h.n || j < 1 || j > h.m) { fclose(f); return set_error(SNU_ERR_BAD_INPUT, "mtx: index out of range"); }
     I.push_back(i - 1); J.push_back(j - 1); V.push_back(v);
-    if ((h.symmetric || h.skew) && i != j) {
+    if (expand && (h.symmetric || h.skew) && i != j) {
       I.push_back(j - 1); J.push_back(i - 1); V.push_back(h.skew ? -v : v);
     }
   }
@@ -75,15 +79,17 @@ int read_all(const char *path, MtxHeader &h, std::vector<int> &I,
 
 using namespace snu;
 
-extern "C" int snu_mtx_size(const char *path, int *n, int *m, int *nnz,
-                            int *symmetric) {
+
+extern "C" int snu_mtx_size_ex(const char *path, int flags, int *n, int *m, int *nnz,
+                               int *symmetric) {
   FILE *f = fopen(path, "r");
   if (!f) return set_error(SNU_ERR_BAD_INPUT, std::string("mtx: cannot open ") + path);
   MtxHeader h;
   int rc = read_header(f, h);
   if (rc) { fclose(f); return rc; }
   long long cnt = h.entries;
-  if (h.symmetric || h.skew) {  // count off-diagonal entries to size the expansion
+  if ((h.symmetric || h.skew) && !(flags & SNU_MTX_NO_EXPAND)) {
+    // count off-diagonal entries to size the expansion
     cnt = 0;
     for (int e = 0; e < h.entries; e++) {
       int i, j;
@@ -100,12 +106,12 @@ extern "C" int snu_mtx_size(const char *path, int *n, int *m, int *nnz,
   return SNU_OK;
 }
 
-extern "C" int snu_mtx_read(const char *path, int *rowptr, int *colidx,
-                            double *val) {
+extern "C" int snu_mtx_read_ex(const char *path, int flags, int *rowptr, int *colidx,
+                               double *val) {
   MtxHeader h;
   std::vector<int> I, J;
   std::vector<double> V;
-  int rc = read_all(path, h, I, J, V);
+  int rc = read_all(path, h, I, J, V, !(flags & SNU_MTX_NO_EXPAND));
   if (rc) return rc;
   int nz = (int)I.size();
   // stable counting sort into rows (coo_to_csr, src/read_matrix.cpp:6-42)
@@ -120,16 +126,69 @@ extern "C" int snu_mtx_read(const char *path, int *rowptr, int *colidx,
     ci[d] = J[e];
     cv[d] = V[e];
   }
-  // column-sorted rows (stable)
-  std::vector<int> idx;
-  for (int i = 0; i < h.n; i++) {
-    int a = cnt[i], b = cnt[i + 1];
-    idx.resize(b - a);
-    for (int t = 0; t < b - a; t++) idx[t] = a + t;
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return ci[x] < ci[y]; });
-    for (int t = 0; t < b - a; t++) colidx[a + t] = ci[idx[t]], val[a + t] = cv[idx[t]];
+  if (flags & SNU_MTX_FILE_ORDER) {
+    std::copy(ci.begin(), ci.end(), colidx);
+    std::copy(cv.begin(), cv.end(), val);
+  } else {  // column-sorted rows (stable)
+    std::vector<int> idx;
+    for (int i = 0; i < h.n; i++) {
+      int a = cnt[i], b = cnt[i + 1];
+      idx.resize(b - a);
+      for (int t = 0; t < b - a; t++) idx[t] = a + t;
+      std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return ci[x] < ci[y]; });
+      for (int t = 0; t < b - a; t++) colidx[a + t] = ci[idx[t]], val[a + t] = cv[idx[t]];
+    }
   }
   for (int i = 0; i <= h.n; i++) rowptr[i] = cnt[i];
+  return SNU_OK;
+}
+
+extern "C" int snu_mtx_size(const char *path, int *n, int *m, int *nnz,
+                            int *symmetric) {
+  return snu_mtx_size_ex(path, 0, n, m, nnz, symmetric);
+}
+
+extern "C" int snu_mtx_read(const char *path, int *rowptr, int *colidx,
+                            double *val) {
+  return snu_mtx_read_ex(path, 0, rowptr, colidx, val);
+}
+
+extern "C" int snu_mtx_read_vector(const char *path, int n, double *x) {
+  FILE *f = fopen(path, "r");
+  if (!f) return set_error(SNU_ERR_BAD_INPUT, std::string("mtx: cannot open ") + path);
+  char line[4096];
+  if (!fgets(line, sizeof line, f)) { fclose(f); return set_error(SNU_ERR_BAD_INPUT, "mtx: empty file"); }
+  std::string s(line);
+  for (auto &c : s) c = (char)tolower((unsigned char)c);
+  if (s.rfind("%%matrixmarket", 0) != 0) { fclose(f); return set_error(SNU_ERR_BAD_INPUT, "mtx: missing banner"); }
+  if (s.find("complex") != std::string::npos || s.find("pattern") != std::string::npos) {
+    fclose(f);
+    return set_error(SNU_ERR_BAD_INPUT, "mtx: a vector must be real or integer");
+  }
+  const bool array = s.find("array") != std::string::npos;
+  do {
+    if (!fgets(line, sizeof line, f)) { fclose(f); return set_error(SNU_ERR_BAD_INPUT, "mtx: missing size line"); }
+  } while (line[0] == '%' || line[0] == '\n' || line[0] == '\r');
+  int rows = 0, cols = 0, entries = 0;
+  const int want = array ? 2 : 3;
+  if (sscanf(line, "%d %d %d", &rows, &cols, &entries) < want || rows != n || cols != 1) {
+    fclose(f);
+    return set_error(SNU_ERR_BAD_INPUT, "mtx: expected an n x 1 vector of length " + std::to_string(n));
+  }
+  if (array) {
+    for (int i = 0; i < n; i++)
+      if (fscanf(f, "%lg", &x[i]) != 1) { fclose(f); return set_error(SNU_ERR_BAD_INPUT, "mtx: truncated vector"); }
+  } else {
+    std::fill(x, x + n, 0.0);
+    for (int e = 0; e < entries; e++) {
+      int i, j;
+      double v;
+      if (fscanf(f, "%d %d %lg", &i, &j, &v) != 3) { fclose(f); return set_error(SNU_ERR_BAD_INPUT, "mtx: truncated vector"); }
+      if (i < 1 || i > n || j != 1) { fclose(f); return set_error(SNU_ERR_BAD_INPUT, "mtx: index out of range"); }
+      x[i - 1] = v;
+    }
+  }
+  fclose(f);
   return SNU_OK;
 }
 

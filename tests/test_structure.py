@@ -87,11 +87,9 @@ def test_construct_error_paths(S):
     with pytest.raises(S.SnuError) as e:
         S.SnuMat(B, 1, device=-1)
     assert e.value.code == -4
-    # unsorted row
+    # unsorted rows are accepted (the reference's reader keeps file order)
     B = S.CSR(2, rp, np.array([1, 0, 1], np.int32), np.array([1.0, 2.0, 3.0]))
-    with pytest.raises(S.SnuError) as e:
-        S.SnuMat(B, 1, device=-1)
-    assert e.value.code == -1
+    S.SnuMat(B, 1, device=-1).close()
     # structurally zero diagonal in a leaf
     B = S.CSR(2, np.array([0, 1, 2], np.int32), np.array([1, 0], np.int32), np.array([1.0, 1.0]))
     with pytest.raises(S.SnuError) as e:
