@@ -44,9 +44,13 @@ extern "C" {
 #define SNU_ERR_NOT_CONVERGED -10
 
 /* ordering kinds */
-#define SNU_ORDER_ND 0    /* project nested dissection of A + A^T */
+#define SNU_ORDER_ND 0    /* project nested dissection of A + A^T (the role of
+                             ParMETIS_V3_NodeND): multilevel or level-set
+                             separators, the smaller at each bisection */
 #define SNU_ORDER_GRID 1  /* geometric nested dissection of an nx*ny*nz grid */
 #define SNU_ORDER_GIVEN 2 /* caller's order[n] / sizes[2np-1] (ParMETIS layout) */
+#define SNU_ORDER_ND_LEVELSET 3 /* nested dissection by BFS level-set
+                                   separators (simpler, for comparison) */
 
 /* precision kinds */
 #define SNU_FP64 0 /* FP64 factors and solves (the reference's arithmetic) */
@@ -213,6 +217,9 @@ int snumat_ordering(const snumat_t *h, int *order, int *sizes);
 /* PAP^T rowptr/colidx and perm_map (original nnz -> merged COO position). */
 int snumat_permuted(const snumat_t *h, int *pa_rowptr, int *pa_colidx,
                     int *perm_map);
+/* values of one leaf-row workspace: the widest leaf's rows plus its
+ * boundary columns (rounded up to even); valid for host-only handles. */
+int snumat_leaf_workspace(const snumat_t *h, int *values);
 /* leaf of rank r (heap block 2np-1-r): sizes[4] = leaf_size, core_n,
  * core_nnz, lu_nnz. */
 int snumat_leaf_sizes(const snumat_t *h, int rank, int *sizes4);
@@ -273,6 +280,11 @@ int snu_gen_rhs(const snu_csr_t *A, int nrhs, uint64_t seed, double *x_true,
 int snu_order_grid(int nx, int ny, int nz, int np, int *order, int *sizes);
 int snu_order_nd(int n, const int *rowptr, const int *colidx, int np,
                  uint64_t seed, int *order, int *sizes);
+/* method 0 (= snu_order_nd): at every bisection the smaller of the
+ * multilevel vertex separator and the BFS level-set separator; 1: level-set
+ * separators only; 2: multilevel separators only. */
+int snu_order_nd_ex(int n, const int *rowptr, const int *colidx, int np,
+                    uint64_t seed, int method, int *order, int *sizes);
 /* Matrix Market: read header (n, m, nnz after symmetric expansion), then
  * data into caller arrays (CSR, rows column-sorted)
  * (<- read_matrix / coo_to_csr, src/read_matrix.cpp:6-42,50-107). */

@@ -149,11 +149,18 @@ def order_grid(nx: int, ny: int, nz: int, np_: int):
     return order, sizes
 
 
-def order_nd(A: CSR, np_: int, seed: int = 0):
+ND_METHODS = {"default": 0, "levelset": 1, "multilevel": 2}
+
+
+def order_nd(A: CSR, np_: int, seed: int = 0, method: str = "default"):
+    """Graph nested dissection of A + A^T in the ParMETIS order/sizes
+    contract.  method "default": at every bisection the smaller of the
+    multilevel and the level-set separator; "levelset" / "multilevel": one
+    kind only."""
     order = np.zeros(A.n, np.int32)
     sizes = np.zeros(2 * np_ - 1, np.int32)
-    _check(lib().snu_order_nd(A.n, _ip(A.rowptr), _ip(A.colidx), np_, C.c_uint64(seed),
-                              _ip(order), _ip(sizes)))
+    _check(lib().snu_order_nd_ex(A.n, _ip(A.rowptr), _ip(A.colidx), np_, C.c_uint64(seed),
+                                 ND_METHODS[method], _ip(order), _ip(sizes)))
     return order, sizes
 
 

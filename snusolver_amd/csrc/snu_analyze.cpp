@@ -351,8 +351,9 @@ int analyze(const snu_csr_t *A, const snu_opts_t *opts, Plan &P) {
     long long cells = (long long)opts->grid[0] * opts->grid[1] * opts->grid[2];
     if (cells != n) return set_error(SNU_ERR_BAD_INPUT, "grid dims do not match n");
     rc = order_grid(opts->grid[0], opts->grid[1], opts->grid[2], np, P.order.data(), P.sizes.data());
-  } else if (opts->ordering == SNU_ORDER_ND) {
-    rc = order_nd(n, A->rowptr, A->colidx, np, opts->seed, P.order.data(), P.sizes.data());
+  } else if (opts->ordering == SNU_ORDER_ND || opts->ordering == SNU_ORDER_ND_LEVELSET) {
+    rc = order_nd(n, A->rowptr, A->colidx, np, opts->seed, P.order.data(), P.sizes.data(),
+                  opts->ordering == SNU_ORDER_ND ? 0 : 1);  // level sets only for ND_LEVELSET
   } else {
     return set_error(SNU_ERR_BAD_INPUT, "unknown ordering kind");
   }

@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -169,6 +171,7 @@ extern "C" int snumat_construct(const snu_csr_t *A, const snu_opts_t *opts,
       hipError_t he = hipMemcpy(h->d_ir, &e, sizeof(e), hipMemcpyHostToDevice);
       if (he != hipSuccess) rc = set_error(SNU_ERR_HIP, hipGetErrorString(he));
     }
+    if (!rc) rc = ensure_stream(h);
     if (rc) {
       free_device(h);
       delete h;
@@ -203,6 +206,7 @@ extern "C" int snumat_attach(snumat_t *h, int device, const double *values) {
     hipError_t he = hipMemcpy(h->d_ir, &e, sizeof(e), hipMemcpyHostToDevice);
     if (he != hipSuccess) rc = set_error(SNU_ERR_HIP, hipGetErrorString(he));
   }
+  if (!rc) rc = ensure_stream(h);
   if (rc) {
     free_device(h);
     h->on_device = false;
@@ -285,7 +289,6 @@ int factorize_stage(snumat_t *h, int stage) {
   return h->prec == SNU_FP32 ? factorize_values<float>(h, stage) : factorize_values<double>(h, stage);
 }
 
-// Solve with device pointers (caller's b / x, column-major).
 // FP64 twin of an FP32 handle (same plan and values, FP64 arena), factorized
 // for the handle's current values.
 int fp64_fallback(snumat *h) {
@@ -316,18 +319,20 @@ int fp64_fallback(snumat *h) {
   return SNU_OK;
 }
 
-int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long long ldx, int nrhs) {
-  int rc = ensure_stream(h);
-  if (rc) return rc;
-  h->t[15] = 0;
-  if ((rc = ensure_rhs(h, nrhs))) return rc;
+int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long long ldx, int nrhs);
+}  // namespace
+
+namespace snu {
+int solve_core(snumat *h, const double *d_b, long long ldb, double *d_x, long long ldx, int nrhs,
+               const SolveBuffers &B, bool fallback) {
   hipStream_t st = h->stream;
+  h->t[15] = 0;
   const int n = h->P.n;
   VecDev vd{0, n, nrhs};
   PermDev pd[2];
   std::memset(pd, 0, sizeof(pd));
   pd[0] = PermDev{h->d_order, d_b, d_x, ldb, ldx, n, 0};
-  pd[1] = PermDev{h->d_order, h->d_r, h->d_dx, (long long)n, (long long)n, n, 0};
+  pd[1] = PermDev{h->d_order, B.r, B.dx, (long long)n, (long long)n, n, 0};
   HIPCHK(hipMemcpyAsync(h->d_vec, &vd, sizeof(vd), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(h->d_perm, pd, sizeof(pd), hipMemcpyHostToDevice, st));
   SolveSet ss;
@@ -339,11 +344,13 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
   ss.wd = &h->wd;
   ss.cb_ld = h->wd.cb_total;
   HIPCHK(hipEventRecord(h->ev[6], st));
+  int rc = SNU_OK;
   if (h->prec == SNU_FP32) {
-    if ((rc = ensure_plan<float>(h, nrhs))) return rc;
     std::vector<int> slot_mat(nrhs, 0);
     std::vector<double> anorm{h->anorm};
     IrDev e = ir_entry(h, d_b, ldb, d_x, ldx, nrhs);
+    e.r = B.r;
+    e.dx = B.dx;
     HIPCHK(hipMemcpyAsync(h->d_ir, &e, sizeof(e), hipMemcpyHostToDevice, st));
     RefineSet rs;
     rs.sweep_b = ss;
@@ -353,7 +360,7 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
     rs.nmat = 1;
     rs.max_n = n;
     rs.max_elems = (long long)n * nrhs;
-    rs.norms = h->d_colnorms;
+    rs.norms = B.colnorms;
     rs.nslots = nrhs;
     rs.slot_mat = &slot_mat;
     rs.anorm = &anorm;
@@ -361,9 +368,10 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
     rs.max_steps = h->opts.ir_max_steps > 0 ? h->opts.ir_max_steps : h->opts.ir_max_steps < 0 ? 0 : 10;
     rs.tol = h->opts.ir_tol > 0 ? h->opts.ir_tol : 1e-15;
     std::vector<double> berr;
-    rc = run_refined(rs, h->v32.bp, h->v32.cbuf, h->v32.sp, st, &h->ir_steps, &berr);
+    rc = run_refined(rs, (float *)B.bp, (float *)B.cbuf, *(const GemvPlan<float> *)B.plan, st, &h->ir_steps,
+                     &berr);
     h->ir_berr = berr.empty() ? 0 : berr[0];
-    if (rc == SNU_ERR_NOT_CONVERGED) {
+    if (rc == SNU_ERR_NOT_CONVERGED && fallback) {
       // refinement stalled above 1e-12: solve again with FP64 factors of
       // the same matrix (t[15] = 1 reports it)
       if ((rc = fp64_fallback(h))) return rc;
@@ -371,10 +379,9 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
       HIPCHK(hipSetDevice(h->device));
       h->t[15] = 1;
     }
-    if (rc) return rc;
+    if (rc && rc != SNU_ERR_NOT_CONVERGED) return rc;
   } else {
-    if ((rc = ensure_plan<double>(h, nrhs))) return rc;
-    run_sweep<double>(ss, h->v64.bp, h->v64.cbuf, h->v64.sp, st);
+    run_sweep<double>(ss, (double *)B.bp, (double *)B.cbuf, *(const GemvPlan<double> *)B.plan, st);
     h->ir_steps = 0;
     h->ir_berr = 0;
   }
@@ -385,6 +392,30 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
   h->t[13] = h->ir_steps;
   h->t[14] = h->ir_berr;
   return rc;
+}
+}  // namespace snu
+
+namespace {
+int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long long ldx, int nrhs) {
+  int rc = ensure_stream(h);
+  if (rc) return rc;
+  if ((rc = ensure_rhs(h, nrhs))) return rc;
+  SolveBuffers B;
+  if (h->prec == SNU_FP32) {
+    if ((rc = ensure_plan<float>(h, nrhs))) return rc;
+    B.bp = h->v32.bp;
+    B.cbuf = h->v32.cbuf;
+    B.plan = &h->v32.sp;
+  } else {
+    if ((rc = ensure_plan<double>(h, nrhs))) return rc;
+    B.bp = h->v64.bp;
+    B.cbuf = h->v64.cbuf;
+    B.plan = &h->v64.sp;
+  }
+  B.r = h->d_r;
+  B.dx = h->d_dx;
+  B.colnorms = h->d_colnorms;
+  return solve_core(h, d_b, ldb, d_x, ldx, nrhs, B, true);
 }
 }  // namespace
 
@@ -450,9 +481,30 @@ extern "C" int snu_solve_csr(const snu_csr_t *A, const double *b, double *x,
   snu_opts_t o;
   if (opts) o = *opts;
   else snu_default_opts(&o);
-  snumat_t *h = nullptr;
-  int rc = snumat_construct(A, &o, &h);
+  if (!A || !b || !x) return set_error(SNU_ERR_BAD_INPUT, "null matrix or vectors");
+  if (o.device < 0) return set_error(SNU_ERR_BAD_INPUT, "snu_solve_csr needs a device");
+  int rc = snu_initialize(o.device);
   if (rc) return rc;
+  // analyse on the host; while the widest leaf's workspace does not fit the
+  // LDS of one workgroup, double the leaves (a given ordering is used as is)
+  snumat_t *h = nullptr;
+  const int device = o.device;
+  for (;;) {
+    snu_opts_t oh = o;
+    oh.device = -1;
+    if ((rc = snumat_construct(A, &oh, &h))) return rc;
+    if (leaf_fits_lds(h->P, o.precision) || o.ordering == SNU_ORDER_GIVEN || 2LL * o.np > A->n ||
+        o.np >= (1 << 20))
+      break;
+    snumat_destroy(h);
+    h = nullptr;
+    o.np *= 2;
+  }
+  h->opts.device = device;
+  if ((rc = snumat_attach(h, device, A->data))) {
+    snumat_destroy(h);
+    return rc;
+  }
   rc = snumat_factorize(h);
   if (!rc) rc = snumat_solve(h, b, x, 1, A->n);
   snumat_destroy(h);
@@ -585,6 +637,11 @@ extern "C" int snumat_timings(const snumat_t *h, double *t16) {
 extern "C" int snumat_gemm_split(const snumat_t *h, double *out6) {
   if (!h || !out6) return set_error(SNU_ERR_BAD_INPUT, "null");
   std::memcpy(out6, h->gsplit, sizeof(h->gsplit));
+  return SNU_OK;
+}
+extern "C" int snumat_leaf_workspace(const snumat_t *h, int *values) {
+  if (!h || !values) return set_error(SNU_ERR_BAD_INPUT, "null");
+  *values = leaf_workspace(h->P);
   return SNU_OK;
 }
 extern "C" int snumat_counts(const snumat_t *h, double *c4) {

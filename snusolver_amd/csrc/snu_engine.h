@@ -186,6 +186,12 @@ const Values<T> &vals(const snumat *h) {
 }
 
 int upload(snumat *h);
+// values of one leaf-row workspace (the widest leaf's rows plus its boundary
+// columns, rounded up to even), and whether one such workspace and the row
+// flags fit the LDS of a workgroup on the current device (upload refuses
+// plans that do not)
+int leaf_workspace(const Plan &P);
+bool leaf_fits_lds(const Plan &P, int prec);
 // GEMM tile (64 or 128) for a C block of M x N, by precision; autotune_tiles
 // measures the table once per process (on the current device)
 template <typename T>
@@ -222,6 +228,10 @@ void build_solve_plan(const snumat *h, T *bp, T *ys, int nrhs, GemvPlan<T> &out)
 // unchanged until st has passed the copy).
 template <typename T>
 int upload_gemv_plan(GemvPlan<T> &g, hipStream_t st = nullptr);
+// The split-K layout of the plan's launches (no device work); returns the
+// partial-sum values its largest launch needs.
+template <typename T>
+size_t layout_gemv_plan(GemvPlan<T> &g);
 template <typename T>
 void free_gemv_plan(GemvPlan<T> &g);
 template <typename T>
@@ -281,5 +291,23 @@ struct RefineSet {
 };
 int run_refined(const RefineSet &rs, float *bp, float *cbuf, const GemvPlan<float> &plan,
                 hipStream_t st, int *steps_out, std::vector<double> *berr_mat);
+
+// Buffers of one standalone solve: the permuted right-hand sides and the
+// boundary contributions (value type of the handle), the dense-solve launch
+// plan with its device descriptors, and (SNU_FP32) the FP64 residual /
+// correction and the column norms of the refinement.
+struct SolveBuffers {
+  void *bp = nullptr, *cbuf = nullptr;
+  const void *plan = nullptr;  // GemvPlan<T> of the handle's value type
+  double *r = nullptr, *dx = nullptr;
+  unsigned long long *colnorms = nullptr;
+};
+// Solve with device pointers (caller's b / x, column-major) on the given
+// buffers, on the handle's stream; returns when the work is done.  fallback:
+// a refinement that stalls above 1e-12 is answered by the handle's FP64 twin
+// (built on first need); otherwise SNU_ERR_NOT_CONVERGED is returned with x
+// holding the last iterate.
+int solve_core(snumat *h, const double *d_b, long long ldb, double *d_x, long long ldx, int nrhs,
+               const SolveBuffers &B, bool fallback);
 
 }  // namespace snu

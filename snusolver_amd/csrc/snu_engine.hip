@@ -83,6 +83,18 @@ int waves_for(int wsz, int vsize, int max_leaf, int cap) {
   return std::min(cap, fit);
 }
 
+int leaf_workspace(const Plan &P) {
+  int w = 1;
+  for (const LeafPlan &L : P.leaves) w = std::max(w, L.leaf_size + L.nbc);
+  return (w + 1) & ~1;
+}
+bool leaf_fits_lds(const Plan &P, int prec) {
+  int ml = 0;
+  for (const LeafPlan &L : P.leaves) ml = std::max(ml, L.leaf_size);
+  const long long vs = prec == SNU_FP32 ? 4 : 8;
+  return leaf_flag_bytes(ml) + (long long)leaf_workspace(P) * vs <= device_lds_limit();
+}
+
 // ---------------------------------------------------------------------------
 // dense plan (structure only; pointers into the matrix's value arrays)
 // ---------------------------------------------------------------------------
@@ -710,20 +722,12 @@ void build_solve_plan(const snumat *h, T *bp, T *ys, int nrhs, GemvPlan<T> &S) {
   }
 }
 
+// Split-K layout: pieces per problem from its longest segment (the problem
+// alone decides), workgroup and partial-sum offsets per launch; a launch
+// whose problems are all single-piece runs unsplit.  Returns the partial-sum
+// values the largest launch needs (launches run in turn and share them).
 template <typename T>
-int upload_gemv_plan(GemvPlan<T> &g, hipStream_t st) {
-  auto grow = [](auto **p, size_t &cap, size_t need) -> int {
-    if (need <= cap && *p) return SNU_OK;
-    if (*p) HIPCHK(hipFree(*p));
-    *p = nullptr;
-    const size_t c = std::max<size_t>(need + need / 2, 64);
-    HIPCHK(hipMalloc((void **)p, c * sizeof(**p)));
-    cap = c;
-    return SNU_OK;
-  };
-  // split-K layout: pieces per problem from its longest segment (the
-  // problem alone decides), workgroup and partial-sum offsets per launch;
-  // a launch whose problems are all single-piece runs unsplit
+size_t layout_gemv_plan(GemvPlan<T> &g) {
   size_t need_part = 0;
   for (std::vector<Launch> *ls : {&g.fwd, &g.bwd})
     for (Launch &L : *ls) {
@@ -745,6 +749,21 @@ int upload_gemv_plan(GemvPlan<T> &g, hipStream_t st) {
       L.part = split ? off : 0;
       need_part = std::max(need_part, (size_t)L.part);
     }
+  return need_part;
+}
+
+template <typename T>
+int upload_gemv_plan(GemvPlan<T> &g, hipStream_t st) {
+  auto grow = [](auto **p, size_t &cap, size_t need) -> int {
+    if (need <= cap && *p) return SNU_OK;
+    if (*p) HIPCHK(hipFree(*p));
+    *p = nullptr;
+    const size_t c = std::max<size_t>(need + need / 2, 64);
+    HIPCHK(hipMalloc((void **)p, c * sizeof(**p)));
+    cap = c;
+    return SNU_OK;
+  };
+  const size_t need_part = layout_gemv_plan(g);
   int rc;
   if ((rc = grow(&g.d_vp, g.cap_vp, g.vp.size())) || (rc = grow(&g.d_vs, g.cap_vs, g.vs.size())) ||
       (need_part && (rc = grow(&g.d_part, g.cap_part, need_part))))
@@ -1051,8 +1070,10 @@ int upload(snumat *h) {
   h->wd.sep_wsz = h->wd.wsz;
   h->wd.sep_waves = waves_for(h->wd.sep_wsz, vsize, 0, vsize == 4 ? 16 : 8);
   if (h->wd.waves < 1)
-    return set_error(SNU_ERR_OOM, "leaf workspace (" + std::to_string(maxw) +
-                                      " columns) exceeds the LDS of one workgroup");
+    return set_error(SNU_ERR_OOM, "leaf workspace of " + std::to_string(maxw) +
+                                      " values (the widest leaf's rows plus its boundary columns) exceeds the " +
+                                      std::to_string(device_lds_limit()) +
+                                      " bytes of LDS of one workgroup: use more leaves (np)");
   // work lists: leaf separator-row workgroups, forward boundary workgroups,
   // and boundary targets (separator rows receiving leaf contributions, each
   // with its sources in ascending rank order)
@@ -1497,6 +1518,7 @@ int run_refined(const RefineSet &rs, float *bp, float *cbuf, const GemvPlan<floa
 #define SNU_ENGINE_INST(T)                                                                    \
   template void build_solve_plan<T>(const snumat *, T *, T *, int, GemvPlan<T> &);            \
   template int upload_gemv_plan<T>(GemvPlan<T> &, hipStream_t);                               \
+  template size_t layout_gemv_plan<T>(GemvPlan<T> &);                                         \
   template void free_gemv_plan<T>(GemvPlan<T> &);                                             \
   template int upload_dense_plan<T>(DensePlan<T> &);                                          \
   template void free_dense_plan<T>(DensePlan<T> &);                                           \

@@ -80,7 +80,13 @@ struct Plan {
 int analyze(const snu_csr_t *A, const snu_opts_t *opts, Plan &P);
 
 int order_grid(int nx, int ny, int nz, int np, int *order, int *sizes);
+// method 0: the smaller of the multilevel and the level-set separator at
+// every bisection, 1: level-set separators, 2: multilevel separators
 int order_nd(int n, const int *rowptr, const int *colidx, int np,
-             uint64_t seed, int *order, int *sizes);
+             uint64_t seed, int *order, int *sizes, int method = 0);
+// Multilevel vertex separator of a symmetric graph without self loops (CSR):
+// part[v] = 0 or 1 (the two sides, no edge between them) or 2 (separator).
+void ml_separator(int n, const std::vector<int> &ptr, const std::vector<int> &adj, uint64_t seed,
+                  std::vector<int> &part);
 
 }  // namespace snu

@@ -11,8 +11,11 @@
 //    its middle with a one-vertex-thick plane; recurse inside leaves too).
 //    A one-thick plane separates 5/7/9/27-point stencils alike.
 //  - order_nd: graph ND of A + A^T without the diagonal
-//    (addTransposeEliminateDiag, src/parmetis_driver.cpp:15-71), bisected by
-//    the middle BFS level set from a pseudo-peripheral vertex, then thinned.
+//    (addTransposeEliminateDiag, src/parmetis_driver.cpp:15-71).  Method 0
+//    (the default): each bisection takes the smaller of the multilevel
+//    vertex separator of snu_nd_ml.cpp and the middle BFS level set from a
+//    pseudo-peripheral vertex (thinned).  Method 1: level sets only (the
+//    simpler orderer); method 2: multilevel separators only.
 #include <algorithm>
 #include <cstring>
 #include <queue>
@@ -124,9 +127,62 @@ struct GraphCtx {
     for (int v : vs) dist[v] = -1;
   }
 
-  // Splits vs (all carrying tag t) into A, B, S.
+  int method = 0;
+  std::vector<int> lid;
+  // Splits vs (all carrying tag t) into A, B, S by the multilevel separator
+  // of the induced subgraph (the seed varies per call, deterministically).
+  void bisect_ml(const std::vector<int> &vs, int t, std::vector<int> &A,
+                 std::vector<int> &B, std::vector<int> &S) {
+    std::vector<int> sorted(vs);
+    std::sort(sorted.begin(), sorted.end());
+    const int m = (int)sorted.size();
+    for (int i = 0; i < m; i++) lid[sorted[i]] = i;
+    std::vector<int> lptr(m + 1, 0), ladj;
+    for (int i = 0; i < m; i++) {
+      const int v = sorted[i];
+      for (int q = ptr[v]; q < ptr[v + 1]; q++)
+        if (tag[adj[q]] == t) ladj.push_back(lid[adj[q]]);
+      lptr[i + 1] = (int)ladj.size();
+    }
+    std::vector<int> part;
+    ml_separator(m, lptr, ladj, seed + (uint64_t)tagc, part);
+    A.clear(); B.clear(); S.clear();
+    for (int i = 0; i < m; i++) (part[i] == 0 ? A : part[i] == 1 ? B : S).push_back(sorted[i]);
+  }
+  // Splits vs (all carrying tag t) into A, B, S.  Method 0 computes both
+  // the multilevel and the level-set separator and keeps the smaller one
+  // among those that leave both sides at least 30% of the rest (the more
+  // balanced one if neither does): level sets win on graphs whose
+  // breadth-first layers are thin (7-point grids, where they run
+  // diagonally), the multilevel separator everywhere else.
   void bisect(const std::vector<int> &vs, int t, std::vector<int> &A,
               std::vector<int> &B, std::vector<int> &S) {
+    if (method == 0) {
+      std::vector<int> A2, B2, S2;
+      bisect_ml(vs, t, A, B, S);
+      bisect_ls(vs, t, A2, B2, S2);
+      auto ok = [](const std::vector<int> &a, const std::vector<int> &b) {
+        return 10 * std::min(a.size(), b.size()) >= 3 * (a.size() + b.size());
+      };
+      auto bal = [](const std::vector<int> &a, const std::vector<int> &b) {
+        return (double)std::min(a.size(), b.size()) / std::max<size_t>(1, a.size() + b.size());
+      };
+      const bool ok1 = ok(A, B), ok2 = ok(A2, B2);
+      const bool take2 = (ok1 && ok2) ? S2.size() < S.size() : ok2 ? true : ok1 ? false : bal(A2, B2) > bal(A, B);
+      if (take2) {
+        A.swap(A2);
+        B.swap(B2);
+        S.swap(S2);
+      }
+      return;
+    }
+    if (method == 2) bisect_ml(vs, t, A, B, S);
+    else bisect_ls(vs, t, A, B, S);
+  }
+  // Level-set separator: the middle BFS level of each component's level
+  // structure from a pseudo-peripheral vertex, thinned.
+  void bisect_ls(const std::vector<int> &vs, int t, std::vector<int> &A,
+                 std::vector<int> &B, std::vector<int> &S) {
     // level structures of every component, concatenated
     std::vector<std::vector<int>> levels;
     std::vector<int> visit, lstart;
@@ -246,12 +302,14 @@ int order_grid(int nx, int ny, int nz, int np, int *order, int *sizes) {
 }
 
 int order_nd(int n, const int *rowptr, const int *colidx, int np,
-             uint64_t seed, int *order, int *sizes) {
+             uint64_t seed, int *order, int *sizes, int method) {
   if (n < 0 || !pow2(np)) return set_error(SNU_ERR_BAD_INPUT, "order_nd: bad n or np");
+  if (method < 0 || method > 2) return set_error(SNU_ERR_BAD_INPUT, "order_nd: unknown method");
   GraphCtx g;
   g.n = n;
   g.order = order;
   g.seed = seed;
+  g.method = method;
   // A + A^T without the diagonal, rows sorted and deduplicated
   std::vector<int> deg(n + 1, 0);
   for (int i = 0; i < n; i++)
@@ -283,6 +341,7 @@ int order_nd(int n, const int *rowptr, const int *colidx, int np,
   g.tag.assign(n, 0);
   g.dist.assign(n, -1);
   g.where.assign(n, 0);
+  g.lid.assign(n, 0);
   std::vector<int> all(n);
   for (int i = 0; i < n; i++) all[i] = i;
   g.tree(all, np, 1, np, sizes);
@@ -297,5 +356,9 @@ extern "C" int snu_order_grid(int nx, int ny, int nz, int np, int *order,
 }
 extern "C" int snu_order_nd(int n, const int *rowptr, const int *colidx,
                             int np, uint64_t seed, int *order, int *sizes) {
-  return snu::order_nd(n, rowptr, colidx, np, seed, order, sizes);
+  return snu::order_nd(n, rowptr, colidx, np, seed, order, sizes, 0);
+}
+extern "C" int snu_order_nd_ex(int n, const int *rowptr, const int *colidx, int np,
+                               uint64_t seed, int method, int *order, int *sizes) {
+  return snu::order_nd(n, rowptr, colidx, np, seed, order, sizes, method);
 }

@@ -344,3 +344,30 @@ def test_driver_writes_x_vs_oracle(S, oracle_mod, precision):
             x = _read_vec(xpath, A.n)
             assert _rel(x, xo) <= REL_TOL
             assert S.backward_error(A, x, b) <= BERR_TOL
+
+
+def test_wide_leaves_refused_or_refined(S):
+    """A plan whose widest leaf workspace exceeds the LDS is refused by
+    construct with a clear error; the one-shot solve (the reference's
+    solve()) doubles the leaves until it fits and solves; x against SuperLU
+    (scipy) and the backward error."""
+    import ctypes as C
+
+    import scipy.sparse.linalg as spla
+    A = S.gen_grid(220, 220, 1, 5, 0, 4.1, 9)
+    _, b = S.gen_rhs(A, 1, 1)
+    b = b[:, 0]
+    with pytest.raises(S.SnuError) as e:
+        S.SnuMat(A, 1, grid=(220, 220, 1), device=0)
+    assert e.value.code == -7 and "LDS" in str(e.value) and "np" in str(e.value)
+    o = S._Opts()
+    S.lib().snu_default_opts(C.byref(o))
+    o.np = 1
+    o.ordering = S.ORDER_GRID
+    o.grid[0], o.grid[1], o.grid[2] = 220, 220, 1
+    x = np.zeros(A.n)
+    a = A.c()
+    S._check(S.lib().snu_solve_csr(C.byref(a), S._dp(b), S._dp(x), C.byref(o)))
+    xs = spla.spsolve(A.to_scipy().tocsc(), b)
+    assert _rel(x, xs) <= REL_TOL
+    assert S.backward_error(A, x, b) <= BERR_TOL

@@ -110,3 +110,20 @@ def test_golden_matrix_file_order(S):
 def test_drivers_usage(name):
     r = subprocess.run([os.path.join(ROOT, "snusolver_amd", name)], capture_output=True, text=True)
     assert r.returncode == 1 and "Usage" in r.stderr
+
+
+def test_leaf_workspace_shrinks_with_np(S):
+    """The leaf-row workspace (widest leaf's rows plus its boundary columns)
+    is what the LDS must hold; doubling the leaves shrinks it, which is how
+    the one-shot solve fits large matrices."""
+    A = S.gen_grid(200, 200, 1, 5, 0, 4.1, 0)
+    import ctypes as C
+    ws = []
+    for np_ in (1, 4, 16):
+        with S.SnuMat(A, np_, grid=(200, 200, 1), device=-1) as M:
+            v = C.c_int()
+            S._check(S.lib().snumat_leaf_workspace(M._h, C.byref(v)))
+            ls = M.info()["max_leaf_size"]
+            assert v.value >= ls and v.value % 2 == 0
+            ws.append(v.value)
+    assert ws[0] >= 40000 and ws[0] > ws[1] > ws[2]
