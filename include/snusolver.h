@@ -152,7 +152,10 @@ int snubatch_solve(snubatch_t *b, const double *const *hb, double *const *hx,
                    const int *nrhs, const int *ld);
 /* t[0..5] factor phases as snumat_timings, t[6] last solve, t[13] the most
  * refinement steps of any member (SNU_FP32), t[14] the largest final
- * backward error. */
+ * backward error of the refinement, t[15] the number of members whose
+ * refinement stalled above 1e-12 and that were answered by FP64 factors of
+ * the same matrix instead (each member's FP64 twin, built on first need, as
+ * snumat_solve does for a single handle). */
 int snubatch_timings(const snubatch_t *b, double *t16);
 /* number of dense factor launches and dense solve launches of the batch */
 int snubatch_launches(const snubatch_t *b, int *factor_launches,

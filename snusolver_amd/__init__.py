@@ -494,6 +494,7 @@ class Batch:
         out = {k: float(t[i]) for i, k in enumerate(keys)}
         out["ir_steps"] = float(t[13])
         out["ir_berr"] = float(t[14])
+        out["fp64_fallbacks"] = float(t[15])
         return out
 
     def launches(self):

@@ -419,6 +419,15 @@ int solve_dev(snumat *h, const double *d_b, long long ldb, double *d_x, long lon
 }
 }  // namespace
 
+namespace snu {
+int fp64_fallback_solve(snumat *h, const double *d_b, long long ldb, double *d_x, long long ldx, int nrhs) {
+  HIPCHK(hipSetDevice(h->device));
+  int rc = fp64_fallback(h);
+  if (rc) return rc;
+  return solve_dev(h->fb64, d_b, ldb, d_x, ldx, nrhs);
+}
+}  // namespace snu
+
 extern "C" int snumat_factorize(snumat_t *h) { return factorize_stage(h, 2); }
 extern "C" int snumat_factorize_stage(snumat_t *h, int stage) {
   return factorize_stage(h, stage);

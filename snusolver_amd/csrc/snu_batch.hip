@@ -201,6 +201,7 @@ int batch_factorize(snubatch *b) {
   }
   for (int m = 0; m < nm; m++) {
     b->mats[m]->factored = true;
+    b->mats[m]->fb64_current = false;  // an FP64 twin refactorizes on its next use
     b->mats[m]->anorm = b->anorm[m];
   }
   return SNU_OK;
@@ -281,6 +282,19 @@ int batch_solve(snubatch *b, const double *const *d_b, double *const *d_x, const
     rs.tol = o.ir_tol > 0 ? o.ir_tol : 1e-15;
     rc = run_refined(rs, (float *)b->bp, (float *)b->cbuf, b->s32, st, &b->ir_steps, &b->berr);
     if (rc && rc != SNU_ERR_NOT_CONVERGED) return rc;
+    b->t[15] = 0;
+    if (rc == SNU_ERR_NOT_CONVERGED) {
+      // members whose refinement stalled above 1e-12 are answered by FP64
+      // factors of the same matrix (each member's FP64 twin); the batch's
+      // own work is complete (run_refined has read the final norms back)
+      for (int m = 0; m < nm; m++) {
+        if (!nrhs[m] || b->berr[m] <= 1e-12) continue;
+        if ((rc = fp64_fallback_solve(b->mats[m], d_b[m], ld[m], d_x[m], ld[m], nrhs[m]))) return rc;
+        b->t[15] += 1;
+      }
+      HIPCHK(hipSetDevice(b->device));
+      rc = SNU_OK;
+    }
   } else {
     run_sweep<double>(ss, (double *)b->bp, (double *)b->cbuf, b->s64, st);
     b->ir_steps = 0;

@@ -309,5 +309,10 @@ struct SolveBuffers {
 // holding the last iterate.
 int solve_core(snumat *h, const double *d_b, long long ldb, double *d_x, long long ldx, int nrhs,
                const SolveBuffers &B, bool fallback);
+// x = A^-1 b with FP64 factors of an SNU_FP32 handle's matrix (its FP64
+// twin: same plan, FP64 arena, the handle's current values; built and
+// factorized on first need, refactorized after the handle's next
+// factorization); returns when the work is done.
+int fp64_fallback_solve(snumat *h, const double *d_b, long long ldb, double *d_x, long long ldx, int nrhs);
 
 }  // namespace snu

@@ -269,7 +269,7 @@ def test_stage1_blocks_vs_oracle(S, oracle_mod):
 
 def _write_vec(path, v):
     with open(path, "w") as f:
-        f.write("%%MatrixMarket matrix array real general\n%d 1\n" % len(v))
+        f.write("%%%%MatrixMarket matrix array real general\n%d 1\n" % len(v))
         for x in v:
             f.write("%.17g\n" % x)
 
@@ -371,3 +371,49 @@ def test_wide_leaves_refused_or_refined(S):
     xs = spla.spsolve(A.to_scipy().tocsc(), b)
     assert _rel(x, xs) <= REL_TOL
     assert S.backward_error(A, x, b) <= BERR_TOL
+
+
+def test_batch_and_serve_stalled_refinement_fall_back_to_fp64(S, oracle_mod):
+    """FP32 members whose refinement cannot reach 1e-12 (forced from the
+    host: no correction steps allowed) are answered by FP64 factors of the
+    same matrix, in a batch and in the serving loop: each answer equals the
+    FP64 handle's solve of the same columns and meets the oracle bars."""
+    mem = _batch_members(S)[:5]
+    rng = np.random.default_rng(11)
+    counts = [1, 2, 0, 3, 1]
+    bs = [None if k == 0 else rng.standard_normal((A.n, k)) for (A, _, _), k in zip(mem, counts)]
+    ref = []
+    for (A, np_, (o, z)), b in zip(mem, bs):
+        if b is None:
+            ref.append(None)
+            continue
+        with S.SnuMat(A, np_, order=o, sizes=z, device=0) as M64:
+            ref.append(M64.factorize().solve(b))
+    mats = [S.SnuMat(A, np_, order=o, sizes=z, device=0, precision="fp32", ir_max_steps=-1)
+            for (A, np_, (o, z)) in mem]
+    try:
+        with S.Batch(mats) as B:
+            xs = B.factorize().solve(bs)
+            t = B.timings()
+    finally:
+        for M in mats:
+            M.close()
+    print(f"batch: {t['fp64_fallbacks']:.0f} members fell back, refinement backward error {t['ir_berr']:.2e}",
+          flush=True)
+    assert t["fp64_fallbacks"] == sum(1 for k in counts if k) and t["ir_berr"] > 1e-12
+    for q, ((A, np_, (o, z)), b, x, x64) in enumerate(zip(mem, bs, xs, ref)):
+        if b is None:
+            assert x is None
+            continue
+        assert np.array_equal(x, x64), (q, float(np.max(np.abs(x - x64))))
+        _vs_oracle(S, oracle_mod, A, np_, o, z, x, b, f"batch member {q} (FP64 fallback)")
+    with S.Server(budget_bytes=16 << 30, slots=1, max_cols=64) as srv:
+        ids = [srv.add(A, np_, order=o, sizes=z, precision="fp32", ir_max_steps=-1)
+               for (A, np_, (o, z)) in mem]
+        tickets = [(q, srv.submit(mid, b)) for q, (mid, b) in enumerate(zip(ids, bs)) if b is not None]
+        for q, tk in tickets:
+            x, _ = srv.wait(tk)
+            A, np_, (o, z) = mem[q]
+            rel = _rel(x, ref[q])
+            assert rel <= 1e-12, (q, rel)
+            _vs_oracle(S, oracle_mod, A, np_, o, z, x, bs[q], f"served request {q} (FP64 fallback)")
