@@ -104,7 +104,7 @@ __device__ __forceinline__ void factor_row(int i, int pend, const int *__restric
 #pragma clang fp contract(off)
   // Pivot rows are streamed NST - 1 pivots ahead: while pivot t is applied,
   // the first NCH chunks (64 entries each) of the U rows of the next pivots
-  // are in flight into a ring of NST (2 or 4) register sets (chunks past a
+  // are in flight into a ring of NST register sets (chunks past a
   // row's end are not loaded; lanes past it read a clamped entry that is
   // never used).  Longer U rows are loaded inside the update.  A pivot's U
   // entries name distinct workspace columns, so its update gathers every
@@ -114,8 +114,8 @@ __device__ __forceinline__ void factor_row(int i, int pend, const int *__restric
   const int pd = LEAF_ROW ? diag[i] : pend;  // separator rows: pivots before pend only
   for (int p = p0 + lane; p < p1; p += 64) w[wcol[p]] = val[p];
   wave_lds_fence();
-  int ca[NCH] = {}, cb[NCH] = {}, cc[NCH] = {}, cd[NCH] = {};
-  T ua[NCH] = {}, ub[NCH] = {}, uc[NCH] = {}, ud[NCH] = {};
+  int cs[NST][NCH] = {};
+  T us[NST][NCH] = {};
   // always inlined: a call would pass the register arrays through memory
   auto fetch = [&](int s, int e, int(&c)[NCH], T(&u)[NCH]) __attribute__((always_inline)) {
     const int n = (e - s + 63) >> 6;
@@ -183,33 +183,21 @@ __device__ __forceinline__ void factor_row(int i, int pend, const int *__restric
       ke = rowptr[k + 1];
       piv = val[kd];
     }
-    // pivot t of the batch is held in set t mod NST of (a, b, c, d)
-    fetch(rl(ks, 0), rl(ke, 0), ca, ua);
-    if constexpr (NST == 2) {
-      for (int t = 0; t < cnt; t += 2) {
-        if (t + 1 < cnt) fetch(rl(ks, t + 1), rl(ke, t + 1), cb, ub);
-        apply(rl(k, t), rl(ks, t), rl(ke, t), rld(piv, t), ca, ua);
-        if (t + 1 >= cnt) break;
-        if (t + 2 < cnt) fetch(rl(ks, t + 2), rl(ke, t + 2), ca, ua);
-        apply(rl(k, t + 1), rl(ks, t + 1), rl(ke, t + 1), rld(piv, t + 1), cb, ub);
+    // pivot t of the batch is held in register set t mod NST; the loads of
+    // pivot t + NST - 1 are issued before pivot t is applied (the set
+    // indices are compile-time constants once the inner loop is unrolled)
+#pragma unroll
+    for (int j = 0; j < NST - 1; j++)
+      if (j < cnt) fetch(rl(ks, j), rl(ke, j), cs[j], us[j]);
+    for (int t = 0; t < cnt; t += NST) {
+#pragma unroll
+      for (int j = 0; j < NST; j++) {
+        const int tt = t + j;
+        if (tt >= cnt) break;
+        if (tt + NST - 1 < cnt)
+          fetch(rl(ks, tt + NST - 1), rl(ke, tt + NST - 1), cs[(j + NST - 1) % NST], us[(j + NST - 1) % NST]);
+        apply(rl(k, tt), rl(ks, tt), rl(ke, tt), rld(piv, tt), cs[j], us[j]);
       }
-      pb += cnt;
-      continue;
-    }
-    if (1 < cnt) fetch(rl(ks, 1), rl(ke, 1), cb, ub);
-    if (2 < cnt) fetch(rl(ks, 2), rl(ke, 2), cc, uc);
-    for (int t = 0; t < cnt; t += 4) {
-      if (t + 3 < cnt) fetch(rl(ks, t + 3), rl(ke, t + 3), cd, ud);
-      apply(rl(k, t), rl(ks, t), rl(ke, t), rld(piv, t), ca, ua);
-      if (t + 1 >= cnt) break;
-      if (t + 4 < cnt) fetch(rl(ks, t + 4), rl(ke, t + 4), ca, ua);
-      apply(rl(k, t + 1), rl(ks, t + 1), rl(ke, t + 1), rld(piv, t + 1), cb, ub);
-      if (t + 2 >= cnt) break;
-      if (t + 5 < cnt) fetch(rl(ks, t + 5), rl(ke, t + 5), cb, ub);
-      apply(rl(k, t + 2), rl(ks, t + 2), rl(ke, t + 2), rld(piv, t + 2), cc, uc);
-      if (t + 3 >= cnt) break;
-      if (t + 6 < cnt) fetch(rl(ks, t + 6), rl(ke, t + 6), cc, uc);
-      apply(rl(k, t + 3), rl(ks, t + 3), rl(ke, t + 3), rld(piv, t + 3), cd, ud);
     }
     pb += cnt;
   }
@@ -348,8 +336,11 @@ __global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_leaf_sep(const 
   const int i = L.brow[ri];
   const int pend = L.nt ? L.sep_tail[ri] : L.rowptr[i + 1];
   // FP32: 16 waves fit the LDS; 12 prefetched chunks keep the registers
-  // within the 128 per lane that 16 waves leave
-  factor_row<T, false, sizeof(T) == 4 ? 12 : 16, 2>(i, pend, L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0),
+  // within the 128 per lane that 16 waves leave.  FP64: the first 8 chunks
+  // of the U rows of the next 3 pivots in flight (8 x 4 measured 7.6 ms on
+  // the 64^3 separator rows against 10.0 for 16 x 2, 7.8 for 4 x 8 and
+  // 8.9 for 2 x 16)
+  factor_row<T, false, sizeof(T) == 4 ? 12 : 8, sizeof(T) == 4 ? 2 : 4>(i, pend, L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0),
                               nullptr, nullptr);
   if (L.nt) {
     T *trow = (T *)L.tail_l + (long long)ri * L.nt;
