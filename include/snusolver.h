@@ -128,6 +128,25 @@ int snumat_solve(snumat_t *h, const double *b, double *x, int nrhs, int ld);
 int snumat_solve_device(snumat_t *h, const double *d_b, double *d_x, int nrhs,
                         int ld);
 
+/* Allocation-free form of snumat_solve_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Every working buffer of the solve (permuted right-hand sides,
+ * dense-solve vectors and descriptors, split-K partial sums, and for
+ * SNU_FP32 handles the FP64 residual, correction and column norms) comes
+ * from one 256 MiB scratch array per device that the library holds until
+ * the process ends; the call makes no hipMalloc / hipFree.  Needs a handle
+ * constructed on a device and factorized.  A call whose buffers exceed the
+ * scratch (about 16 n nrhs bytes for FP64 handles, 24 n nrhs for SNU_FP32)
+ * returns SNU_ERR_BAD_INPUT with the size in the message.  All calls on one
+ * device share the scratch: calls that overlap on one device (other host
+ * threads, other handles) race and must be serialised by the caller.
+ * SNU_FP32 handles refine as snumat_solve_device does but have no FP64
+ * fallback here (building it would allocate): a stalled refinement returns
+ * SNU_ERR_NOT_CONVERGED with x holding the last iterate. */
+int snumat_solve_device_scratch(snumat_t *h, const double *d_b, double *d_x,
+                                int nrhs, int ld);
+
 int snumat_destroy(snumat_t *h);
 
 /* ---------------- batches (no reference counterpart) ----------------

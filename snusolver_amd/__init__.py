@@ -314,6 +314,12 @@ class SnuMat:
     def solve_device_ptr(self, d_b: int, d_x: int, nrhs: int):
         _check(lib().snumat_solve_device(self._h, C.c_void_p(d_b), C.c_void_p(d_x), nrhs, self.A.n))
 
+    def solve_device_scratch_ptr(self, d_b: int, d_x: int, nrhs: int):
+        """snumat_solve_device_scratch: the same solve with every working
+        buffer in the library's per-device scratch (no device allocation)."""
+        _check(lib().snumat_solve_device_scratch(self._h, C.c_void_p(d_b), C.c_void_p(d_x), nrhs,
+                                                 self.A.n))
+
     def set_profiling(self, on: bool):
         _check(lib().snumat_set_profiling(self._h, 1 if on else 0))
 

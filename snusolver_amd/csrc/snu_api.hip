@@ -133,13 +133,47 @@ extern "C" void snu_default_opts(snu_opts_t *o) {
 
 extern "C" const char *snu_last_error(void) { return g_err.c_str(); }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free solve (snumat_solve_device_scratch): one
+// fixed array per device, held from the library's load until the process
+// ends.  In host code `scratch` names a host-side stand-in, not the device
+// array: host code uses only the per-device address that
+// hipGetSymbolAddress returns (scratch_base), never the name.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchBytes = size_t(256) << 20;  // 256 MiB per device
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch_base[kMaxDevices] = {};
+
+// Address of this device's scratch (looked up once per device, with that
+// device current); the caller has made `device` current.
+int scratch_base(int device, unsigned char **out) {
+  if (device < 0 || device >= kMaxDevices)
+    return set_error(SNU_ERR_NO_DEVICE, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_base[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(SNU_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") +
+                                        hipGetErrorString(e));
+    g_scratch_base[device] = p;
+  }
+  *out = (unsigned char *)g_scratch_base[device];
+  return SNU_OK;
+}
+}  // namespace
+
 extern "C" int snu_initialize(int device) {
   int cnt = 0;
   if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0)
     return set_error(SNU_ERR_NO_DEVICE, "no HIP device available");
   if (device < 0 || device >= cnt) return set_error(SNU_ERR_NO_DEVICE, "device ordinal out of range");
   HIPCHK(hipSetDevice(device));
-  return SNU_OK;
+  unsigned char *base = nullptr;
+  return scratch_base(device, &base);
 }
 
 extern "C" int snumat_construct(const snu_csr_t *A, const snu_opts_t *opts,
@@ -441,6 +475,94 @@ extern "C" int snumat_solve_device(snumat_t *h, const double *d_b, double *d_x,
   if (d_b == d_x) return set_error(SNU_ERR_BAD_INPUT, "b and x must not alias");
   HIPCHK(hipSetDevice(h->device));
   return solve_dev(h, d_b, ld, d_x, ld, nrhs);
+}
+
+namespace {
+// The solve of snumat_solve_device over buffers carved from the device's
+// scratch: permuted right-hand sides, dense-solve vectors, boundary
+// contributions, (SNU_FP32) residual, correction and column norms, then the
+// dense-solve descriptors and split-K partial sums.  Every buffer is written
+// before it is read within the call (the column norms are zeroed by their
+// own kernel), so no clearing is needed; the descriptors are copied in on
+// every call, as another handle's call may have overwritten them.
+template <typename T>
+int solve_scratch(snumat *h, const double *d_b, double *d_x, int nrhs, int ld) {
+  unsigned char *base = nullptr;
+  int rc = scratch_base(h->device, &base);
+  if (rc) return rc;
+  const size_t n = (size_t)h->P.n;
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~size_t(255);
+    return o;
+  };
+  const size_t o_bp = carve(sizeof(T) * n * nrhs);
+  const size_t o_ys = carve(sizeof(T) * n * nrhs);
+  const size_t o_cb = carve(sizeof(T) * (size_t)std::max(1, h->wd.cb_total) * nrhs);
+  size_t o_r = 0, o_dx = 0, o_cn = 0;
+  if (h->prec == SNU_FP32) {
+    o_r = carve(sizeof(double) * n * nrhs);
+    o_dx = carve(sizeof(double) * n * nrhs);
+    o_cn = carve(sizeof(double) * 3 * (size_t)nrhs);
+  }
+  Values<T> &VV = vals<T>(h);
+  GemvPlan<T> &g = VV.sp_scratch;
+  if (VV.scratch_nrhs != nrhs) {
+    VV.scratch_nrhs = -1;
+    g = GemvPlan<T>();  // device pointers point into the scratch: nothing to free
+    build_solve_plan<T>(h, (T *)(base + o_bp), (T *)(base + o_ys), nrhs, g);
+  }
+  const size_t need_part = layout_gemv_plan(g);
+  const size_t o_vp = carve(sizeof(GemvProbT<T>) * std::max<size_t>(1, g.vp.size()));
+  const size_t o_vs = carve(sizeof(GemvSegT<T>) * std::max<size_t>(1, g.vs.size()));
+  const size_t o_part = carve(sizeof(T) * std::max<size_t>(1, need_part));
+  if (off > kScratchBytes)
+    return set_error(SNU_ERR_BAD_INPUT, "allocation-free solve needs " + std::to_string(off) +
+                                         " bytes of scratch for n = " + std::to_string(n) +
+                                         ", nrhs = " + std::to_string(nrhs) + "; the limit is " +
+                                         std::to_string(kScratchBytes) +
+                                         " (use snumat_solve_device, or fewer columns per call)");
+  VV.scratch_nrhs = nrhs;
+  g.d_vp = (GemvProbT<T> *)(base + o_vp);
+  g.d_vs = (GemvSegT<T> *)(base + o_vs);
+  g.d_part = (T *)(base + o_part);
+  g.cap_vp = g.vp.size();
+  g.cap_vs = g.vs.size();
+  g.cap_part = need_part;
+  if (!g.vp.empty())
+    HIPCHK(hipMemcpyAsync(g.d_vp, g.vp.data(), g.vp.size() * sizeof(g.vp[0]), hipMemcpyHostToDevice,
+                          h->stream));
+  if (!g.vs.empty())
+    HIPCHK(hipMemcpyAsync(g.d_vs, g.vs.data(), g.vs.size() * sizeof(g.vs[0]), hipMemcpyHostToDevice,
+                          h->stream));
+  SolveBuffers B;
+  B.bp = base + o_bp;
+  B.cbuf = base + o_cb;
+  B.plan = &g;
+  if (h->prec == SNU_FP32) {
+    B.r = (double *)(base + o_r);
+    B.dx = (double *)(base + o_dx);
+    B.colnorms = (unsigned long long *)(base + o_cn);
+  }
+  // no FP64 twin here: building one allocates (a stalled refinement
+  // returns SNU_ERR_NOT_CONVERGED with x = the last iterate)
+  return solve_core(h, d_b, ld, d_x, ld, nrhs, B, false);
+}
+}  // namespace
+
+extern "C" int snumat_solve_device_scratch(snumat_t *h, const double *d_b, double *d_x,
+                                           int nrhs, int ld) {
+  if (!h || !d_b || !d_x || nrhs < 1) return set_error(SNU_ERR_BAD_INPUT, "bad solve arguments");
+  if (!h->on_device) return set_error(SNU_ERR_NO_DEVICE, "handle was built host-only");
+  if (!h->factored) return set_error(SNU_ERR_NOT_FACTORED, "solve before factorize");
+  if (ld < h->P.n) return set_error(SNU_ERR_BAD_INPUT, "ld < n");
+  if (d_b == d_x) return set_error(SNU_ERR_BAD_INPUT, "b and x must not alias");
+  if (!h->stream || h->ev.size() < 8)
+    return set_error(SNU_ERR_BAD_INPUT, "handle has no stream (construct it on a device)");
+  HIPCHK(hipSetDevice(h->device));
+  return h->prec == SNU_FP32 ? solve_scratch<float>(h, d_b, d_x, nrhs, ld)
+                             : solve_scratch<double>(h, d_b, d_x, nrhs, ld);
 }
 
 extern "C" int snumat_solve(snumat_t *h, const double *b, double *x, int nrhs,

@@ -75,6 +75,11 @@ struct Values {
   T *bp = nullptr, *ys = nullptr, *cbuf = nullptr;
   int plan_nrhs = -1;
   GemvPlan<T> sp;
+  // allocation-free solves: the same plan over buffers in the library's
+  // static per-device scratch (its device pointers point into the scratch
+  // and are never freed)
+  GemvPlan<T> sp_scratch;
+  int scratch_nrhs = -1;
 };
 
 // Host copies of the per-leaf work lists (kept for batching).
