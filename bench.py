@@ -186,7 +186,7 @@ def main():
         "factor_ms": round(tim["factor_ms"], 3),
         "solve_ms": round(tim["solve_ms"], 3),
         "phases_ms": {k: round(tim[k], 3) for k in
-                      ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms"]},
+                      ["scatter_ms", "leaf_fronts_ms", "leaf_schur_ms", "assemble_ms", "dense_ms"]},
         # achieved work: the flops the dense launches execute (zero tiles
         # skipped) over the dense phase's time, and the bytes one solve sweep
         # reads (factors actually touched + vectors) over the solve time
@@ -262,7 +262,7 @@ def precision_leg(S, np, reps=3):
             ft = M.timings()
             if not fms or ft["factor_ms"] < min(fms):
                 phases = {k: round(ft[k], 3) for k in
-                          ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms"]}
+                          ["scatter_ms", "leaf_fronts_ms", "leaf_schur_ms", "assemble_ms", "dense_ms"]}
             fms.append(ft["factor_ms"])
             M.solve_device_ptr(d_b.ptr.value, d_x.ptr.value, 1)
             t = M.timings()

@@ -229,8 +229,8 @@ const char *snu_build_id(void);
 /* ---------------- introspection (tests, bench) ---------------- */
 /* info[0..15]: n, nnz, np, num_block, max_level, total leaf LU nnz,
  * dense entries, max core_n, max leaf_size, max boundary rows, max boundary
- * cols, on device (0/1), factored (0/1), device bytes held, waves per leaf
- * workgroup, boundary target rows. */
+ * cols, on device (0/1), factored (0/1), device bytes held, leaf fronts
+ * (all leaves), boundary target rows. */
 int snumat_info(const snumat_t *h, int64_t *info16);
 /* block_start/block_size of e = 1..2np-1, written at [e-1]. */
 int snumat_blocks(const snumat_t *h, int *block_start, int *block_size);
@@ -239,15 +239,22 @@ int snumat_ordering(const snumat_t *h, int *order, int *sizes);
 /* PAP^T rowptr/colidx and perm_map (original nnz -> merged COO position). */
 int snumat_permuted(const snumat_t *h, int *pa_rowptr, int *pa_colidx,
                     int *perm_map);
-/* values of one leaf-row workspace: the widest leaf's rows plus its
- * boundary columns (rounded up to even); valid for host-only handles. */
-int snumat_leaf_workspace(const snumat_t *h, int *values);
 /* leaf of rank r (heap block 2np-1-r): sizes[4] = leaf_size, core_n,
  * core_nnz, lu_nnz. */
 int snumat_leaf_sizes(const snumat_t *h, int rank, int *sizes4);
 int snumat_leaf_struct(const snumat_t *h, int rank, int *lu_rowptr,
                        int *lu_colidx, int *lu_diag, int *lu_bias,
                        int *core_map);
+/* Multifrontal structure of the leaf of rank r (test hook): sizes4 = fronts,
+ * B entries, owned LU slots, child links; then per front {first pivot,
+ * pivots, order, parent front (-1: root), height}, B lists (idx_ptr / idx,
+ * core indices), each B entry's index in the parent front (rel), owned LU
+ * slots with their column-major front positions, children, and for root
+ * fronts each B entry's boundary row / column (-1: none). */
+int snumat_leaf_front_sizes(const snumat_t *h, int rank, int *sizes4);
+int snumat_leaf_fronts(const snumat_t *h, int rank, int *front5, int *idx_ptr,
+                       int *idx, int *rel, int *slot_ptr, int *slot, int *pos,
+                       int *child_ptr, int *child, int *srow, int *scol);
 /* leaf LU values after factorize (copied back from the device). */
 int snumat_leaf_values(const snumat_t *h, int rank, double *lu_data);
 /* dense diagonal block of separator e (1 <= e < np), s_e x s_e col-major. */
@@ -258,8 +265,8 @@ int snumat_sep_diag(const snumat_t *h, int e, double *out);
 int snumat_factorize_stage(snumat_t *h, int stage);
 int snumat_sep_block(const snumat_t *h, int i, int j, double *out);
 /* phase timings of the last factorize / solve in ms (GPU events):
- * t[0] scatter, t[1] leaf rows, t[2] leaf separator rows + Schur,
- * t[3] Schur assembly, t[4] dense levels, t[5] factorize total,
+ * t[0] scatter, t[1] leaf fronts (the leaves' LU), t[2] the leaves'
+ * boundary Schur blocks gathered from their root fronts, t[3] Schur assembly, t[4] dense levels, t[5] factorize total,
  * t[6] solve total; dense GEMM launches of the factorization:
  * t[7] C -= AB kernel time (sum over launches, needs snumat_set_profiling),
  * t[8] its launches, t[9] its algorithmic GFLOP (2 M N K per problem);

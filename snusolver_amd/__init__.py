@@ -335,7 +335,7 @@ class SnuMat:
         _check(lib().snumat_info(self._h, a))
         keys = ["n", "nnz", "np", "num_block", "max_level", "leaf_lu_nnz", "dense_entries",
                 "max_core_n", "max_leaf_size", "max_boundary_rows", "max_boundary_cols",
-                "on_device", "factored", "device_bytes", "waves", "boundary_targets"]
+                "on_device", "factored", "device_bytes", "leaf_fronts", "boundary_targets"]
         return {k: int(a[i]) for i, k in enumerate(keys)}
 
     def blocks(self):
@@ -369,6 +369,25 @@ class SnuMat:
         _check(lib().snumat_leaf_struct(self._h, rank, _ip(rp), _ip(ci), _ip(dg), _ip(bias), _ip(cm)))
         return rp, ci, dg, bias, cm
 
+    def leaf_fronts(self, rank: int) -> dict:
+        """Multifrontal structure of a leaf (test hook): per front first
+        pivot k0, pivots nk, order f, parent, height; B lists, extend-add
+        indices, owned LU slots and positions, children, root-front boundary
+        rows / cols."""
+        s = np.zeros(4, np.int32)
+        _check(lib().snumat_leaf_front_sizes(self._h, rank, _ip(s)))
+        nf, ni, ns, nc = (int(v) for v in s)
+        a = {"front": np.zeros(5 * nf, np.int32), "idx_ptr": np.zeros(nf + 1, np.int32),
+             "idx": np.zeros(ni, np.int32), "rel": np.zeros(ni, np.int32),
+             "slot_ptr": np.zeros(nf + 1, np.int32), "slot": np.zeros(ns, np.int32),
+             "pos": np.zeros(ns, np.int32), "child_ptr": np.zeros(nf + 1, np.int32),
+             "child": np.zeros(nc, np.int32), "srow": np.zeros(ni, np.int32), "scol": np.zeros(ni, np.int32)}
+        keys = ["front", "idx_ptr", "idx", "rel", "slot_ptr", "slot", "pos", "child_ptr", "child", "srow", "scol"]
+        _check(lib().snumat_leaf_fronts(self._h, rank, *[_ip(a[k]) for k in keys]))
+        fr = a.pop("front").reshape(nf, 5)
+        a.update(k0=fr[:, 0], nk=fr[:, 1], f=fr[:, 2], parent=fr[:, 3], height=fr[:, 4])
+        return a
+
     def leaf_values(self, rank: int) -> np.ndarray:
         lnz = self.leaf_sizes(rank)[3]
         v = np.zeros(lnz, np.float64)
@@ -398,7 +417,7 @@ class SnuMat:
     def timings(self) -> dict:
         t = (C.c_double * 16)()
         _check(lib().snumat_timings(self._h, t))
-        keys = ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms",
+        keys = ["scatter_ms", "leaf_fronts_ms", "leaf_schur_ms", "assemble_ms", "dense_ms",
                 "factor_ms", "solve_ms", "gemm_sub_ms", "gemm_sub_launches", "gemm_sub_gflop",
                 "gemm_assign_ms", "gemm_assign_launches", "gemm_assign_gflop",
                 "ir_steps", "ir_berr", "fp64_fallback"]
@@ -495,7 +514,7 @@ class Batch:
     def timings(self) -> dict:
         t = (C.c_double * 16)()
         _check(lib().snubatch_timings(self._h, t))
-        keys = ["scatter_ms", "leaf_rows_ms", "leaf_sep_ms", "assemble_ms", "dense_ms",
+        keys = ["scatter_ms", "leaf_fronts_ms", "leaf_schur_ms", "assemble_ms", "dense_ms",
                 "factor_ms", "solve_ms"]
         out = {k: float(t[i]) for i, k in enumerate(keys)}
         out["ir_steps"] = float(t[13])

@@ -86,51 +86,6 @@ void symbfact(LeafPlan &L) {
   L.lu_nnz = (int)L.lu_colidx.size();
 }
 
-// Splits a leaf's rows as [0, a) [a, b) [b, ls) so that no row of [a, b)
-// has an L entry below a: the two ranges then factor concurrently and only
-// [b, ls) waits for both.  Under a nested-dissection order of the leaf the
-// best split is its two halves and their separator.  Chooses the a that
-// minimises max(work[0, a), work[a, b)) + work[b, ls), work per row being
-// its pivot updates; a == b (no split) unless that saves 15% of the work.
-static void leaf_split(LeafPlan &L) {
-  const int ls = L.leaf_size;
-  L.split_a = L.split_b = 0;
-  if (ls < 64) return;
-  std::vector<int> minl(ls);
-  std::vector<double> pre(ls + 1, 0.0);
-  for (int i = 0; i < ls; i++) {
-    int m = i;
-    double w = 1.0 + (L.lu_rowptr[i + 1] - L.lu_rowptr[i]);
-    for (int p = L.lu_rowptr[i]; p < L.lu_diag[i]; p++) {
-      int k = L.lu_colidx[p];
-      m = std::min(m, k);
-      w += 1.0 + 2.0 * (L.lu_rowptr[k + 1] - L.lu_diag[k] - 1);
-    }
-    minl[i] = m;
-    pre[i + 1] = pre[i] + w;
-  }
-  // sparse table of range minima of minl
-  int lg = 1;
-  while ((1 << lg) <= ls) lg++;
-  std::vector<std::vector<int>> st(lg);
-  st[0] = minl;
-  for (int k = 1; k < lg; k++) {
-    st[k].assign(ls, 0);
-    for (int i = 0; i + (1 << k) <= ls; i++)
-      st[k][i] = std::min(st[k - 1][i], st[k - 1][i + (1 << (k - 1))]);
-  }
-  const double total = pre[ls];
-  double best = 0.85 * total;
-  for (int a = 1; a < ls; a++) {
-    if (minl[a] < a) continue;  // the first row of [a, b) has no L entry
-    int b = a;                  // extend while the range minimum stays >= a
-    for (int k = lg - 1; k >= 0; k--)
-      if (b + (1 << k) <= ls && st[k][b] >= a) b += 1 << k;
-    double c = std::max(pre[a], pre[b] - pre[a]) + (total - pre[b]);
-    if (c < best) best = c, L.split_a = a, L.split_b = b;
-  }
-}
-
 int build_leaf(const Plan &P, LeafPlan &L) {
   const int leaf = L.leaf;
   L.npath = 0;
@@ -256,7 +211,8 @@ int build_leaf(const Plan &P, LeafPlan &L) {
     std::vector<int> f(L.ulev_ptr.begin(), L.ulev_ptr.end() - 1);
     for (int j = ls - 1; j >= 0; j--) L.ulev_rows[f[ulev[j]]++] = j;
   }
-  leaf_split(L);
+  int frc = build_fronts(L);
+  if (frc) return frc;
   // structural flops: every pivot of a row costs 1 division and 2 flops per
   // U entry of the pivot row it applies
   double fl = 0;

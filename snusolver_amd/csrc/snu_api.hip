@@ -271,6 +271,7 @@ int factorize_values(snumat *h, int stage) {
   fs.anorm_out = h->d_norms;
   fs.max_n = h->P.n;
   fs.wd = &h->wd;
+  fs.fronts = &vals<T>(h).fronts;
   fs.eps = h->opts.eps;
   static const DensePlan<T> empty;
   double gms[3] = {0, 0, 0};
@@ -616,26 +617,8 @@ extern "C" int snu_solve_csr(const snu_csr_t *A, const double *b, double *x,
   if (o.device < 0) return set_error(SNU_ERR_BAD_INPUT, "snu_solve_csr needs a device");
   int rc = snu_initialize(o.device);
   if (rc) return rc;
-  // analyse on the host; while the widest leaf's workspace does not fit the
-  // LDS of one workgroup, double the leaves (a given ordering is used as is)
   snumat_t *h = nullptr;
-  const int device = o.device;
-  for (;;) {
-    snu_opts_t oh = o;
-    oh.device = -1;
-    if ((rc = snumat_construct(A, &oh, &h))) return rc;
-    if (leaf_fits_lds(h->P, o.precision) || o.ordering == SNU_ORDER_GIVEN || 2LL * o.np > A->n ||
-        o.np >= (1 << 20))
-      break;
-    snumat_destroy(h);
-    h = nullptr;
-    o.np *= 2;
-  }
-  h->opts.device = device;
-  if ((rc = snumat_attach(h, device, A->data))) {
-    snumat_destroy(h);
-    return rc;
-  }
+  if ((rc = snumat_construct(A, &o, &h))) return rc;
   rc = snumat_factorize(h);
   if (!rc) rc = snumat_solve(h, b, x, 1, A->n);
   snumat_destroy(h);
@@ -679,7 +662,9 @@ extern "C" int snumat_info(const snumat_t *h, int64_t *info) {
   info[11] = h->on_device;
   info[12] = h->factored;
   info[13] = (int64_t)h->dev_bytes;
-  info[14] = h->wd.waves;
+  int64_t nfr = 0;
+  for (const LeafPlan &L : P.leaves) nfr += (int64_t)L.fk0.size();
+  info[14] = nfr;
   info[15] = h->wd.ntarget;
   return SNU_OK;
 }
@@ -719,6 +704,40 @@ extern "C" int snumat_leaf_struct(const snumat_t *h, int rank, int *rp, int *ci,
   std::copy(L.lu_diag.begin(), L.lu_diag.end(), dg);
   std::copy(L.lu_bias.begin(), L.lu_bias.end(), bias);
   std::copy(L.core_map.begin(), L.core_map.end(), cmap);
+  return SNU_OK;
+}
+extern "C" int snumat_leaf_front_sizes(const snumat_t *h, int rank, int *s4) {
+  if (!h || !s4 || rank < 0 || rank >= h->P.np) return set_error(SNU_ERR_BAD_INPUT, "bad rank");
+  const LeafPlan &L = h->P.leaves[rank];
+  s4[0] = (int)L.fk0.size();
+  s4[1] = (int)L.fidx.size();
+  s4[2] = (int)L.fslot.size();
+  s4[3] = (int)L.fchild.size();
+  return SNU_OK;
+}
+extern "C" int snumat_leaf_fronts(const snumat_t *h, int rank, int *front5, int *idx_ptr, int *idx,
+                                  int *rel, int *slot_ptr, int *slot, int *pos, int *child_ptr,
+                                  int *child, int *srow, int *scol) {
+  if (!h || rank < 0 || rank >= h->P.np) return set_error(SNU_ERR_BAD_INPUT, "bad rank");
+  const LeafPlan &L = h->P.leaves[rank];
+  const size_t nf = L.fk0.size();
+  for (size_t f = 0; f < nf; f++) {
+    front5[5 * f + 0] = L.fk0[f];
+    front5[5 * f + 1] = L.fnk[f];
+    front5[5 * f + 2] = L.ff[f];
+    front5[5 * f + 3] = L.fpar[f];
+    front5[5 * f + 4] = L.fheight[f];
+  }
+  std::copy(L.fidx_ptr.begin(), L.fidx_ptr.end(), idx_ptr);
+  std::copy(L.fidx.begin(), L.fidx.end(), idx);
+  std::copy(L.frel.begin(), L.frel.end(), rel);
+  std::copy(L.fslot_ptr.begin(), L.fslot_ptr.end(), slot_ptr);
+  std::copy(L.fslot.begin(), L.fslot.end(), slot);
+  std::copy(L.fpos.begin(), L.fpos.end(), pos);
+  std::copy(L.fchild_ptr.begin(), L.fchild_ptr.end(), child_ptr);
+  std::copy(L.fchild.begin(), L.fchild.end(), child);
+  std::copy(L.fsrow.begin(), L.fsrow.end(), srow);
+  std::copy(L.fscol.begin(), L.fscol.end(), scol);
   return SNU_OK;
 }
 extern "C" int snumat_leaf_values(const snumat_t *h, int rank, double *out) {
@@ -768,11 +787,6 @@ extern "C" int snumat_timings(const snumat_t *h, double *t16) {
 extern "C" int snumat_gemm_split(const snumat_t *h, double *out6) {
   if (!h || !out6) return set_error(SNU_ERR_BAD_INPUT, "null");
   std::memcpy(out6, h->gsplit, sizeof(h->gsplit));
-  return SNU_OK;
-}
-extern "C" int snumat_leaf_workspace(const snumat_t *h, int *values) {
-  if (!h || !values) return set_error(SNU_ERR_BAD_INPUT, "null");
-  *values = leaf_workspace(h->P);
   return SNU_OK;
 }
 extern "C" int snumat_counts(const snumat_t *h, double *c4) {

@@ -35,6 +35,8 @@ struct snubatch {
   int max_n = 0;
   DensePlan<double> f64;
   DensePlan<float> f32;
+  FrontPlan<double> fr64;  // the members' leaf fronts, merged per height
+  FrontPlan<float> fr32;
   // solves (sized for the last right-hand-side pattern)
   std::vector<int> pattern;  // nrhs per matrix of the current solve plan
   long long vec_cap = 0;
@@ -90,6 +92,8 @@ void free_batch(snubatch *b) {
   free_graph(b->graph);
   free_dense_plan(b->f64);
   free_dense_plan(b->f32);
+  free_dense_plan(b->fr64.dense);
+  free_dense_plan(b->fr32.dense);
   free_gemv_plan(b->s64);
   free_gemv_plan(b->s32);
   for (void *p : b->allocs) (void)hipFree(p);
@@ -183,6 +187,7 @@ int batch_factorize(snubatch *b) {
   fs.anorm_out = b->d_anorm;
   fs.max_n = b->max_n;
   fs.wd = &b->wd;
+  fs.fronts = std::is_same<T, double>::value ? (const void *)&b->fr64 : (const void *)&b->fr32;
   fs.eps = b->mats[0]->opts.eps;
   const DensePlan<T> &plan = std::is_same<T, double>::value ? (const DensePlan<T> &)b->f64
                                                             : (const DensePlan<T> &)b->f32;
@@ -343,12 +348,8 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   WorkLists W;
   W.tptr.assign(1, 0);
   W.tile_ptr.assign(1, 0);
-  int wsz = 0, max_leaf = 0;
-  for (snumat *h : b->mats) wsz = std::max(wsz, h->wd.wsz), max_leaf = std::max(max_leaf, h->wd.max_leaf);
-  const int vsize = b->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
-  const int waves = waves_for(wsz, vsize, max_leaf, 16);
-  const int sep_wsz = wsz, sep_waves = waves_for(sep_wsz, vsize, 0, vsize == 4 ? 16 : 8);
-  if (waves < 1) return fail(set_error(SNU_ERR_OOM, "leaf workspace exceeds the LDS of one workgroup"));
+  int max_leaf = 0;
+  for (snumat *h : b->mats) max_leaf = std::max(max_leaf, h->wd.max_leaf);
   int cb = 0;
   std::vector<PrepDev> prep;
   std::vector<IrDev> ira;
@@ -362,10 +363,7 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
     }
     for (int r = 0; r < h->P.np; r++) {
       const int nbr = h->P.leaves[r].nbr;
-      for (int f = 0; f < nbr; f += sep_waves) W.wg_leaf.push_back(lbase + r), W.wg_first.push_back(f);
       for (int f = 0; f < nbr; f += 4) W.fwg_leaf.push_back(lbase + r), W.fwg_first.push_back(f);
-      if (h->h_leaves[r].nt > 0)
-        for (int f = 0; f < nbr; f += kTailRows) W.twg_leaf.push_back(lbase + r), W.twg_first.push_back(f);
     }
     const WorkLists &w = h->wl;
     W.trow_blk.insert(W.trow_blk.end(), w.trow_blk.begin(), w.trow_blk.end());
@@ -403,26 +401,16 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   }
   WorkDev &wd = b->wd;
   wd.nleaves = (int)leaves.size();
-  wd.nwg = (int)W.wg_leaf.size();
   wd.nfwg = (int)W.fwg_leaf.size();
-  wd.ntwg = (int)W.twg_leaf.size();
   wd.ntarget = (int)W.trow_glob.size();
   wd.ntile = (int)W.tile_blk.size();
-  wd.max_nt = 0;
-  for (snumat *h : b->mats) wd.max_nt = std::max(wd.max_nt, h->wd.max_nt);
   wd.cb_total = cb;
-  wd.wsz = wsz;
-  wd.waves = waves;
-  wd.sep_wsz = sep_wsz;
-  wd.sep_waves = sep_waves;
   wd.max_leaf = max_leaf;
   wd.max_lev = 0;
   for (snumat *h : b->mats) wd.max_lev = std::max(wd.max_lev, h->wd.max_lev);
   int rc;
-  if ((rc = bupload(b, &wd.leaves, leaves)) || (rc = bupload(b, &wd.wg_leaf, W.wg_leaf)) ||
-      (rc = bupload(b, &wd.wg_first, W.wg_first)) || (rc = bupload(b, &wd.fwg_leaf, W.fwg_leaf)) ||
+  if ((rc = bupload(b, &wd.leaves, leaves)) || (rc = bupload(b, &wd.fwg_leaf, W.fwg_leaf)) ||
       (rc = bupload(b, &wd.fwg_first, W.fwg_first)) || (rc = bupload(b, &wd.trow_blk, W.trow_blk)) ||
-      (rc = bupload(b, &wd.twg_leaf, W.twg_leaf)) || (rc = bupload(b, &wd.twg_first, W.twg_first)) ||
       (rc = bupload(b, &wd.trow_loc, W.trow_loc)) || (rc = bupload(b, &wd.trow_glob, W.trow_glob)) ||
       (rc = bupload(b, &wd.tptr, W.tptr)) || (rc = bupload(b, &wd.tsrc_leaf, W.tsrc_leaf)) ||
       (rc = bupload(b, &wd.tsrc_ri, W.tsrc_ri)) || (rc = bupload(b, &b->d_prep, prep)) ||
@@ -431,28 +419,25 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
       (rc = bupload(b, &wd.seg_ri, W.seg_ri)) ||
       (rc = bupload(b, &b->d_ir_a, ira)))
     return fail(rc);
-  // leaf-tail GEMM problems over the concatenated leaves
+  // the members' leaf fronts, merged per height (descriptors point into the
+  // members' own arrays)
   {
-    auto tails = [&](auto tag) -> int {
+    auto fronts = [&](auto tag) -> int {
       using T = decltype(tag);
-      std::vector<GemmProbT<T>> gp;
-      std::vector<GemmSegT<T>> gs;
-      int tiles = 0, r;
-      tail_gemm_list<T>(leaves, gp, gs, tiles);
-      GemmProbT<T> *dgp = nullptr;
-      GemmSegT<T> *dgs = nullptr;
-      if (!gp.empty() && ((r = bupload(b, &dgp, gp)) || (r = bupload(b, &dgs, gs)))) return r;
-      wd.tail_gp = dgp;
-      wd.tail_gs = dgs;
-      wd.tail_np = (int)gp.size();
-      wd.tail_tiles = tiles;
+      FrontPlan<T> &fp = std::is_same<T, double>::value ? (FrontPlan<T> &)b->fr64 : (FrontPlan<T> &)b->fr32;
+      std::vector<const FrontPlan<T> *> parts;
+      for (snumat *h : b->mats) parts.push_back(&vals<T>(h).fronts);
+      merge_fronts(parts, fp);
+      int r;
+      if ((r = upload_dense_plan(fp.dense)) || (r = bupload(b, &fp.d_small, fp.small)) ||
+          (r = bupload(b, &fp.d_large, fp.large)) || (r = bupload(b, &fp.d_work, fp.work)) ||
+          (r = bupload(b, &fp.d_ls, fp.ls)) || (r = bupload(b, &fp.d_tiles, fp.tiles)))
+        return r;
       return SNU_OK;
     };
-    if ((rc = b->prec == SNU_FP32 ? tails(float()) : tails(double()))) return fail(rc);
+    if ((rc = b->prec == SNU_FP32 ? fronts(float()) : fronts(double()))) return fail(rc);
   }
-  if ((rc = balloc(b, (void **)&wd.stall, sizeof(int))) ||
-      (rc = balloc(b, (void **)&wd.half_done, sizeof(int) * (leaves.size() + 1))) ||
-      (rc = balloc(b, (void **)&b->d_anorm, sizeof(unsigned long long) * count)) ||
+  if ((rc = balloc(b, (void **)&b->d_anorm, sizeof(unsigned long long) * count)) ||
       (rc = balloc(b, (void **)&b->d_active, sizeof(int) * count)) ||
       (rc = balloc(b, (void **)&b->d_vec, sizeof(VecDev) * count)) ||
       (rc = balloc(b, (void **)&b->d_perm, sizeof(PermDev) * 2 * count)) ||

@@ -63,6 +63,33 @@ struct GemvPlan {
   size_t cap_vp = 0, cap_vs = 0, cap_part = 0;  // device capacities (grow-only)
 };
 
+// Multifrontal leaf factorization of a launch set (one matrix, or a batch
+// merged): per height of the leaves' front trees, the fronts the LDS kernel
+// takes whole and the larger fronts' 64-column work items (front index into
+// `large`); the larger fronts' blocked partial LU as a keyed launch list
+// (key: height, step within it); the tiles of the boundary Schur blocks.
+struct FrontLevel {
+  int small0 = 0, nsmall = 0, max_f = 0;
+  int work0 = 0, nwork = 0;
+};
+template <typename T>
+struct FrontPlan {
+  std::vector<FrontDev> small, large;
+  std::vector<int2> work;
+  std::vector<FrontLevel> levels;
+  DensePlan<T> dense;
+  std::vector<LeafSchurDev> ls;
+  std::vector<int4> tiles;
+  FrontDev *d_small = nullptr, *d_large = nullptr;
+  int2 *d_work = nullptr;
+  LeafSchurDev *d_ls = nullptr;
+  int4 *d_tiles = nullptr;
+};
+inline long long front_key(int height, int step, int kind) {
+  return (long long)height * 100000000LL + step * 8LL + kind;
+}
+inline int front_key_height(long long key) { return (int)(key / 100000000LL); }
+
 // Value-typed device state of one matrix (FP64, or FP32 refined in FP64).
 template <typename T>
 struct Values {
@@ -70,6 +97,8 @@ struct Values {
   T *S = nullptr;      // boundary Schur blocks
   T *inv = nullptr;    // per separator: full L^-1 then U^-1 (s x s each)
   T *tmp = nullptr;    // products of the triangular inversion
+  T *farena = nullptr; // leaf fronts (each f x f; larger fronts then 2 x 64 x 64 of inverses)
+  FrontPlan<T> fronts;
   DensePlan<T> fact;
   // standalone solves
   T *bp = nullptr, *ys = nullptr, *cbuf = nullptr;
@@ -84,9 +113,7 @@ struct Values {
 
 // Host copies of the per-leaf work lists (kept for batching).
 struct WorkLists {
-  std::vector<int> wg_leaf, wg_first;    // leaf separator-row workgroups
   std::vector<int> fwg_leaf, fwg_first;  // forward boundary-row workgroups
-  std::vector<int> twg_leaf, twg_first;  // leaf tail-row workgroups (kTailRows rows)
   std::vector<int> trow_blk, trow_loc, trow_glob, tptr, tsrc_leaf, tsrc_ri;  // targets
   std::vector<int> tile_blk, tile_row0, tile_ptr, seg_leaf, seg_ri;          // assembly tiles
 };
@@ -95,23 +122,14 @@ struct WorkLists {
 struct WorkDev {
   LeafDev *leaves = nullptr;
   int nleaves = 0;
-  int *wg_leaf = nullptr, *wg_first = nullptr, nwg = 0;
   int *fwg_leaf = nullptr, *fwg_first = nullptr, nfwg = 0;
-  int *twg_leaf = nullptr, *twg_first = nullptr, ntwg = 0;
   int *trow_blk = nullptr, *trow_loc = nullptr, *trow_glob = nullptr, *tptr = nullptr;
   int *tsrc_leaf = nullptr, *tsrc_ri = nullptr, ntarget = 0;
   int *tile_blk = nullptr, *tile_row0 = nullptr, *tile_ptr = nullptr;
   int *seg_leaf = nullptr, *seg_ri = nullptr, ntile = 0;
-  void *tail_gp = nullptr, *tail_gs = nullptr;  // leaf-tail GEMM problems (value type of the set)
-  int tail_np = 0, tail_tiles = 0;
-  int max_nt = 0;  // largest leaf tail
   int cb_total = 0;
-  int wsz = 0, waves = 1;          // leaf rows (workspace: leaf + boundary columns)
-  int sep_wsz = 0, sep_waves = 1;  // separator rows (same workspace, no flags)
-  int max_leaf = 0;  // rows of the largest leaf (completion flags in LDS)
+  int max_leaf = 0;  // rows of the largest leaf
   int max_lev = 0;   // most level sets of a leaf (forward or backward)
-  int *stall = nullptr;  // leaf-row wait gave up (device flag)
-  int *half_done = nullptr;  // per leaf: its helper workgroup finished
 };
 
 // A factorization's launch sequence captured once into a HIP graph and
@@ -191,12 +209,13 @@ const Values<T> &vals(const snumat *h) {
 }
 
 int upload(snumat *h);
-// values of one leaf-row workspace (the widest leaf's rows plus its boundary
-// columns, rounded up to even), and whether one such workspace and the row
-// flags fit the LDS of a workgroup on the current device (upload refuses
-// plans that do not)
-int leaf_workspace(const Plan &P);
-bool leaf_fits_lds(const Plan &P, int prec);
+// The multifrontal plan of one uploaded matrix (value arrays allocated).
+template <typename T>
+int build_front_plan(snumat *h);
+// Merge the front plans of several matrices into out's host tables (per
+// height: their fronts side by side; dense launches merged by key).
+template <typename T>
+void merge_fronts(const std::vector<const FrontPlan<T> *> &in, FrontPlan<T> &out);
 // GEMM tile (64 or 128) for a C block of M x N, by precision; autotune_tiles
 // measures the table once per process (on the current device)
 template <typename T>
@@ -207,7 +226,6 @@ void free_device(snumat *h);
 size_t estimate_device_bytes(const snumat *h);
 int device_alloc(snumat *h, void **p, size_t bytes);
 float ev_ms(hipEvent_t a, hipEvent_t b);
-int waves_for(int wsz, int vsize, int max_leaf, int cap);
 
 template <typename T>
 int upload_list(T **p, const std::vector<T> &v);
@@ -218,11 +236,6 @@ template <typename T>
 int add_gemm(std::vector<GemmProbT<T>> &gp, std::vector<GemmSegT<T>> &gs, int &tiles,
              T *C, long long ldc, int M, int N, const std::vector<GemmSegT<T>> &segs,
              double &flops, int tb = 64);
-// One C -= A B problem per leaf with a dense tail: S^T -= tail_u^T tail_l^T
-// (column-major view of the row-major blocks).
-template <typename T>
-void tail_gemm_list(const std::vector<LeafDev> &leaves, std::vector<GemmProbT<T>> &gp,
-                    std::vector<GemmSegT<T>> &gs, int &tiles);
 
 // GEMV plan of one matrix's dense solves into the packed vectors bp / ys at
 // this matrix's segment (ld = n, nrhs columns); keys order fwd / bwd.
@@ -260,6 +273,7 @@ struct FactorSet {
   unsigned long long *anorm_out;
   int max_n;
   const WorkDev *wd;
+  const void *fronts;  // FrontPlan of the set's value type
   double eps;
 };
 // graph (optional, not with profile): capture the launches on the first

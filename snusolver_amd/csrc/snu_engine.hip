@@ -73,28 +73,6 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
-// waves per leaf workgroup: the LDS of one workgroup holds the leaf's row
-// completion flags and one workspace row (wsz values) per wave
-// waves of a leaf workgroup: as many workspaces as the LDS holds, at most
-// cap (leaf rows: 16, FP32 workspaces being half the size; separator rows:
-// 16 in FP32, 8 in FP64, whose kernel needs more than 128 registers per lane)
-int waves_for(int wsz, int vsize, int max_leaf, int cap) {
-  int fit = (device_lds_limit() - leaf_flag_bytes(max_leaf)) / std::max(1, wsz * vsize);
-  return std::min(cap, fit);
-}
-
-int leaf_workspace(const Plan &P) {
-  int w = 1;
-  for (const LeafPlan &L : P.leaves) w = std::max(w, L.leaf_size + L.nbc);
-  return (w + 1) & ~1;
-}
-bool leaf_fits_lds(const Plan &P, int prec) {
-  int ml = 0;
-  for (const LeafPlan &L : P.leaves) ml = std::max(ml, L.leaf_size);
-  const long long vs = prec == SNU_FP32 ? 4 : 8;
-  return leaf_flag_bytes(ml) + (long long)leaf_workspace(P) * vs <= device_lds_limit();
-}
-
 // ---------------------------------------------------------------------------
 // dense plan (structure only; pointers into the matrix's value arrays)
 // ---------------------------------------------------------------------------
@@ -123,19 +101,6 @@ int add_gemm(std::vector<GemmProbT<T>> &gp, std::vector<GemmSegT<T>> &gs, int &t
   gp.push_back(p);
   return 1;
 }
-template <typename T>
-void tail_gemm_list(const std::vector<LeafDev> &leaves, std::vector<GemmProbT<T>> &gp,
-                    std::vector<GemmSegT<T>> &gs, int &tiles) {
-  double flops = 0;
-  for (const LeafDev &L : leaves)
-    if (L.nt > 0 && L.nbr > 0 && L.nbc > 0)
-      add_gemm(gp, gs, tiles, (T *)L.S, L.nbc, L.nbc, L.nbr,
-               {GemmSegT<T>{(const T *)L.tail_u, (const T *)L.tail_l, L.nbc, L.nt, L.nt}}, flops, 64);
-}
-template void tail_gemm_list<double>(const std::vector<LeafDev> &, std::vector<GemmProbT<double>> &,
-                                     std::vector<GemmSegT<double>> &, int &);
-template void tail_gemm_list<float>(const std::vector<LeafDev> &, std::vector<GemmProbT<float>> &,
-                                    std::vector<GemmSegT<float>> &, int &);
 template int add_gemm<double>(std::vector<GemmProbT<double>> &, std::vector<GemmSegT<double>> &,
                               int &, double *, long long, int, int,
                               const std::vector<GemmSegT<double>> &, double &, int);
@@ -376,7 +341,7 @@ void build_dense_plan(snumat *h) {
         int s = P.block_size[i];
         if (s <= k) continue;
         int nb = std::min(64, s - k);
-        DiagProb dp;
+        DiagProb dp{};  // eps = 0: separator pivots are not clamped
         dp.D = R(i) + k + (long long)k * s;
         dp.ld = s;
         dp.nb = nb;
@@ -901,23 +866,6 @@ void merge_gemv(const std::vector<const GemvPlan<T> *> &in, GemvPlan<T> &out) {
 // device layout of one matrix
 // ---------------------------------------------------------------------------
 namespace {
-// The leaf-tail GEMM problems of one matrix (leaf descriptors complete).
-template <typename T>
-int upload_tail_gemm(snumat *h) {
-  std::vector<GemmProbT<T>> gp;
-  std::vector<GemmSegT<T>> gs;
-  int tiles = 0, rc;
-  tail_gemm_list<T>(h->h_leaves, gp, gs, tiles);
-  WorkDev &wd = h->wd;
-  GemmProbT<T> *dgp = nullptr;
-  GemmSegT<T> *dgs = nullptr;
-  if (!gp.empty() && ((rc = dupload(h, &dgp, gp)) || (rc = dupload(h, &dgs, gs)))) return rc;
-  wd.tail_gp = dgp;
-  wd.tail_gs = dgs;
-  wd.tail_np = (int)gp.size();
-  wd.tail_tiles = tiles;
-  return SNU_OK;
-}
 template <typename T>
 int alloc_values(snumat *h, long long S_total) {
   Values<T> &VV = vals<T>(h);
@@ -938,6 +886,230 @@ int alloc_values(snumat *h, long long S_total) {
 }
 }  // namespace
 
+// Multifrontal plan of one matrix's leaves (value arrays allocated): the
+// front value arena, the integer tables (owned slots and positions,
+// extend-add indices, the root fronts' boundary rows / columns), child and
+// root descriptors, per-height front tables, the larger fronts' blocked
+// partial LU (per 64-column step: diagonal block LU with the pivot clamp,
+// both triangular solves by the block inverses, the trailing update of the
+// whole remaining front) and the boundary Schur tiles.
+template <typename T>
+int build_front_plan(snumat *h) {
+  using GemmSeg = GemmSegT<T>;
+  Values<T> &VV = vals<T>(h);
+  const Plan &P = h->P;
+  const int np = P.np, fmax = front_small_max<T>();
+  const T eps = (T)h->opts.eps;
+  FrontPlan<T> &FP = VV.fronts;
+  free_dense_plan(FP.dense);
+  FP = FrontPlan<T>();
+  int rc;
+  // integer tables and front value offsets
+  std::vector<int> ints;
+  std::vector<size_t> o_slot(np), o_pos(np), o_rel(np);
+  std::vector<std::vector<long long>> foff(np);
+  std::vector<std::vector<std::pair<size_t, size_t>>> o_root(np);  // per root: arow, acol offsets
+  long long ftotal = 0;
+  int levels = 0;
+  for (int r = 0; r < np; r++) {
+    const LeafPlan &L = P.leaves[r];
+    o_slot[r] = ints.size();
+    ints.insert(ints.end(), L.fslot.begin(), L.fslot.end());
+    o_pos[r] = ints.size();
+    ints.insert(ints.end(), L.fpos.begin(), L.fpos.end());
+    o_rel[r] = ints.size();
+    ints.insert(ints.end(), L.frel.begin(), L.frel.end());
+    const int nf = (int)L.fk0.size();
+    foff[r].resize(nf);
+    for (int f = 0; f < nf; f++) {
+      foff[r][f] = ftotal;
+      ftotal += (long long)L.ff[f] * L.ff[f];
+      if (L.ff[f] > fmax) ftotal += 2 * 64 * 64;  // inverses of the diagonal blocks
+      if (L.fpar[f] >= 0) continue;
+      std::vector<int> arow(L.nbr, -1), acol(L.nbc, -1);
+      for (int q = L.fidx_ptr[f]; q < L.fidx_ptr[f + 1]; q++) {
+        if (L.fsrow[q] >= 0) arow[L.fsrow[q]] = q - L.fidx_ptr[f];
+        if (L.fscol[q] >= 0) acol[L.fscol[q]] = q - L.fidx_ptr[f];
+      }
+      const size_t oa = ints.size();
+      ints.insert(ints.end(), arow.begin(), arow.end());
+      const size_t oc = ints.size();
+      ints.insert(ints.end(), acol.begin(), acol.end());
+      o_root[r].push_back({oa, oc});
+    }
+    if (nf) levels = std::max(levels, L.fheight_max + 1);
+  }
+  int *d_ints = nullptr;
+  if ((rc = dupload(h, &d_ints, ints)) || (rc = dalloc(h, &VV.farena, (size_t)std::max(1LL, ftotal)))) return rc;
+  T *fa = VV.farena;
+  // children
+  std::vector<FrontChild> ch;
+  std::vector<std::vector<int>> ch0(np);
+  for (int r = 0; r < np; r++) {
+    const LeafPlan &L = P.leaves[r];
+    const int nf = (int)L.fk0.size();
+    ch0[r].resize(nf);
+    for (int f = 0; f < nf; f++) {
+      ch0[r][f] = (int)ch.size();
+      for (int q = L.fchild_ptr[f]; q < L.fchild_ptr[f + 1]; q++) {
+        const int c = L.fchild[q], fc = L.ff[c], nkc = L.fnk[c];
+        ch.push_back(FrontChild{fa + foff[r][c] + nkc + (long long)nkc * fc,
+                                d_ints + o_rel[r] + L.fidx_ptr[c], fc, fc - nkc});
+      }
+    }
+  }
+  FrontChild *d_ch = nullptr;
+  if ((rc = dupload(h, &d_ch, ch))) return rc;
+  // fronts by height
+  std::vector<std::vector<FrontDev>> sm(levels), lg(levels);
+  for (int r = 0; r < np; r++) {
+    const LeafPlan &L = P.leaves[r];
+    for (int f = 0; f < (int)L.fk0.size(); f++) {
+      FrontDev D;
+      D.F = fa + foff[r][f];
+      D.vals = VV.arena + P.leaf_off[r];
+      D.slot = d_ints + o_slot[r] + L.fslot_ptr[f];
+      D.pos = d_ints + o_pos[r] + L.fslot_ptr[f];
+      D.child = d_ch + ch0[r][f];
+      D.f = L.ff[f];
+      D.nk = L.fnk[f];
+      D.nslot = L.fslot_ptr[f + 1] - L.fslot_ptr[f];
+      D.nchild = L.fchild_ptr[f + 1] - L.fchild_ptr[f];
+      (D.f <= fmax ? sm : lg)[L.fheight[f]].push_back(D);
+    }
+  }
+  FP.levels.assign(levels, FrontLevel());
+  DensePlan<T> &DP = FP.dense;
+  for (int hh = 0; hh < levels; hh++) {
+    FrontLevel &lv = FP.levels[hh];
+    lv.small0 = (int)FP.small.size();
+    lv.nsmall = (int)sm[hh].size();
+    for (const FrontDev &D : sm[hh]) lv.max_f = std::max(lv.max_f, D.f);
+    FP.small.insert(FP.small.end(), sm[hh].begin(), sm[hh].end());
+    lv.work0 = (int)FP.work.size();
+    const int lbase = (int)FP.large.size();
+    int kmax = 0;
+    for (size_t q = 0; q < lg[hh].size(); q++) {
+      const FrontDev &D = lg[hh][q];
+      for (int c0 = 0; c0 < D.f; c0 += 64) FP.work.push_back(make_int2(lbase + (int)q, c0));
+      kmax = std::max(kmax, D.nk);
+    }
+    lv.nwork = (int)FP.work.size() - lv.work0;
+    FP.large.insert(FP.large.end(), lg[hh].begin(), lg[hh].end());
+    // blocked partial LU of this height's larger fronts, grouped per step
+    for (int k = 0; k < kmax; k += 64) {
+      const int step = k / 64;
+      Launch d{0, (int)DP.dp.size(), 0, 0, 0, front_key(hh, step, 0)};
+      Launch tr{1, (int)DP.gp.size(), 0, 0, 0, front_key(hh, step, 1)};
+      GemmBatch<T> tu;
+      for (const FrontDev &D : lg[hh]) {
+        if (D.nk <= k) continue;
+        const int f = D.f, nb = std::min(64, D.nk - k), m = f - k - nb;
+        T *F = (T *)D.F;
+        T *Li = F + (long long)f * f, *Ui = Li + 64 * 64;
+        DiagProbT<T> dp{};
+        dp.D = F + k + (long long)k * f;
+        dp.ld = f;
+        dp.nb = nb;
+        dp.Linv = Li;
+        dp.Uinv = Ui;
+        dp.ldi = 64;
+        dp.eps = eps;
+        DP.dp.push_back(dp);
+        d.np++;
+        // L21 <- L21 U11^-1 (rows below), U12 <- L11^-1 U12 (columns right)
+        T *X1 = F + (k + nb) + (long long)k * f;
+        tr.np += add_gemm(DP.gp, DP.gs, tr.tiles, X1, f, m, nb, {GemmSeg{X1, Ui, f, 64, nb}}, tr.flops);
+        T *X = F + k + (long long)(k + nb) * f;
+        tr.np += add_gemm(DP.gp, DP.gs, tr.tiles, X, f, nb, m, {GemmSeg{Li, X, 64, f, nb}}, tr.flops);
+        // trailing update of the whole remaining front
+        tu.add(F + (k + nb) + (long long)(k + nb) * f, f, m, m, {GemmSeg{X1, X, f, f, nb}});
+      }
+      if (d.np) DP.launches.push_back(d);
+      if (tr.np) DP.launches.push_back(tr);
+      tu.flush(DP.gp, DP.gs, DP.launches, front_key(hh, step, 3));
+    }
+  }
+  if ((rc = upload_dense_plan(DP))) return rc;
+  // boundary Schur blocks from the root fronts
+  std::vector<FrontRoot> roots;
+  for (int r = 0; r < np; r++) {
+    const LeafPlan &L = P.leaves[r];
+    const int root0 = (int)roots.size();
+    int k = 0;
+    for (int f = 0; f < (int)L.fk0.size(); f++) {
+      if (L.fpar[f] >= 0) continue;
+      const int ff = L.ff[f], nk = L.fnk[f];
+      roots.push_back(FrontRoot{fa + foff[r][f] + nk + (long long)nk * ff, d_ints + o_root[r][k].first,
+                                d_ints + o_root[r][k].second, ff});
+      k++;
+    }
+    if (L.nbr > 0 && L.nbc > 0) {
+      const int li = (int)FP.ls.size();
+      FP.ls.push_back(LeafSchurDev{h->h_leaves[r].S, nullptr, L.nbr, L.nbc, (int)roots.size() - root0});
+      FP.ls.back().root = (const FrontRoot *)(uintptr_t)root0;  // index until the table is uploaded
+      for (int r0 = 0; r0 < L.nbr; r0 += 64)
+        for (int c0 = 0; c0 < L.nbc; c0 += 64) FP.tiles.push_back(make_int4(li, r0, c0, 0));
+    }
+  }
+  FrontRoot *d_roots = nullptr;
+  if ((rc = dupload(h, &d_roots, roots))) return rc;
+  for (LeafSchurDev &l : FP.ls) l.root = d_roots + (uintptr_t)l.root;
+  if ((rc = dupload(h, &FP.d_small, FP.small)) || (rc = dupload(h, &FP.d_large, FP.large)) ||
+      (rc = dupload(h, &FP.d_work, FP.work)) || (rc = dupload(h, &FP.d_ls, FP.ls)) ||
+      (rc = dupload(h, &FP.d_tiles, FP.tiles)))
+    return rc;
+  return SNU_OK;
+}
+
+template <typename T>
+void merge_fronts(const std::vector<const FrontPlan<T> *> &in, FrontPlan<T> &out) {
+  out.small.clear();
+  out.large.clear();
+  out.work.clear();
+  out.ls.clear();
+  out.tiles.clear();
+  size_t levels = 0;
+  for (const FrontPlan<T> *p : in) levels = std::max(levels, p->levels.size());
+  out.levels.assign(levels, FrontLevel());
+  for (size_t hh = 0; hh < levels; hh++) {
+    FrontLevel &lv = out.levels[hh];
+    lv.small0 = (int)out.small.size();
+    lv.work0 = (int)out.work.size();
+    for (const FrontPlan<T> *p : in) {
+      if (hh >= p->levels.size()) continue;
+      const FrontLevel &pl = p->levels[hh];
+      out.small.insert(out.small.end(), p->small.begin() + pl.small0, p->small.begin() + pl.small0 + pl.nsmall);
+      lv.max_f = std::max(lv.max_f, pl.max_f);
+      // this matrix's larger fronts of the height, in the order its work items name them
+      int first = -1, last = -1;
+      for (int q = pl.work0; q < pl.work0 + pl.nwork; q++) {
+        if (first < 0) first = p->work[q].x;
+        last = p->work[q].x;
+      }
+      if (first < 0) continue;
+      const int base = (int)out.large.size();
+      out.large.insert(out.large.end(), p->large.begin() + first, p->large.begin() + last + 1);
+      for (int q = pl.work0; q < pl.work0 + pl.nwork; q++)
+        out.work.push_back(make_int2(base + p->work[q].x - first, p->work[q].y));
+    }
+    lv.nsmall = (int)out.small.size() - lv.small0;
+    lv.nwork = (int)out.work.size() - lv.work0;
+  }
+  std::vector<const DensePlan<T> *> dp;
+  for (const FrontPlan<T> *p : in) dp.push_back(&p->dense);
+  merge_dense(dp, out.dense);
+  for (const FrontPlan<T> *p : in) {
+    const int base = (int)out.ls.size();
+    out.ls.insert(out.ls.end(), p->ls.begin(), p->ls.end());
+    for (int4 t : p->tiles) out.tiles.push_back(make_int4(base + t.x, t.y, t.z, t.w));
+  }
+}
+template void merge_fronts<double>(const std::vector<const FrontPlan<double> *> &, FrontPlan<double> &);
+template void merge_fronts<float>(const std::vector<const FrontPlan<float> *> &, FrontPlan<float> &);
+template int build_front_plan<double>(snumat *);
+template int build_front_plan<float>(snumat *);
+
 int upload(snumat *h) {
   Plan &P = h->P;
   int rc;
@@ -947,10 +1119,8 @@ int upload(snumat *h) {
   const int np = P.np;
   // ---- leaf arrays (concatenated; descriptors hold pointers into them) ----
   std::vector<int> rowptr, diag, levptr, levrows, ulevptr, ulevrows;
-  std::vector<int> brow, bcol_glob, bcol_blk, bcol_loc, wcol, ftask, btask, ubnd, sep_tail;
-  const int vsz = h->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
-  int max_nt = 0;
-  struct Off { size_t rp, idx, lev, levr, ulev, ulevr, brow, bcol, task, row; long long S, tu, tl, tp; };
+  std::vector<int> brow, bcol_glob, bcol_blk, bcol_loc, wcol, ftask, btask, ubnd;
+  struct Off { size_t rp, idx, lev, levr, ulev, ulevr, brow, bcol, task, row; long long S; };
   std::vector<Off> off(np);
   long long S_total = 0;
   h->h_leaves.assign(np, LeafDev());
@@ -971,27 +1141,6 @@ int upload(snumat *h) {
     o.row = ubnd.size();
     o.S = S_total;
     S_total += (long long)L.nbr * L.nbc;
-    // dense tail: the rows after the leaf's split (its own top separator),
-    // when they fit the LDS of the tail kernels
-    int nt = L.split_b > 0 && L.nbr > 0 ? L.leaf_size - L.split_b : 0;
-    if (nt > kMaxLeafTail || (long long)128 * nt * vsz > device_lds_limit() ||
-        (long long)nt * (nt + 1) / 2 * vsz > device_lds_limit())
-      nt = 0;
-    D.t0 = L.leaf_size - nt;
-    D.nt = nt;
-    max_nt = std::max(max_nt, nt);
-    o.tu = S_total;
-    S_total += (long long)nt * L.nbc;
-    o.tl = S_total;
-    S_total += (long long)L.nbr * nt;
-    o.tp = S_total;
-    S_total += (long long)nt * (nt + 1) / 2;
-    for (int ri = 0; ri < L.nbr; ri++) {
-      const int i = L.brow[ri];
-      int q = L.lu_rowptr[i];
-      while (q < L.lu_rowptr[i + 1] && L.lu_colidx[q] < D.t0) q++;
-      sep_tail.push_back(q);
-    }
     rowptr.insert(rowptr.end(), L.lu_rowptr.begin(), L.lu_rowptr.end());
     diag.insert(diag.end(), L.lu_diag.begin(), L.lu_diag.end());
     diag.push_back(0);  // diag shares rowptr's offsets (core_n + 1 entries)
@@ -1034,8 +1183,6 @@ int upload(snumat *h) {
     D.nbc = L.nbc;
     D.nlev = (int)L.lev_ptr.size() - 1;
     D.nulev = (int)L.ulev_ptr.size() - 1;
-    D.split_a = L.split_a;
-    D.split_b = L.split_b;
     D.gstart = P.block_start[L.leaf];
     D.cb_off = cb;
     D.vec = 0;
@@ -1044,47 +1191,16 @@ int upload(snumat *h) {
   }
   h->max_leaf_w = maxw;
   h->wd.cb_total = cb;
-  h->wd.wsz = (maxw + 1) & ~1;
   h->wd.max_leaf = 0;
   for (int r = 0; r < np; r++) h->wd.max_leaf = std::max(h->wd.max_leaf, P.leaves[r].leaf_size);
   h->wd.max_lev = 0;
   for (int r = 0; r < np; r++)
     h->wd.max_lev = std::max(h->wd.max_lev, (int)std::max(P.leaves[r].lev_ptr.size(), P.leaves[r].ulev_ptr.size()) - 1);
-  const int vsize = h->prec == SNU_FP32 ? (int)sizeof(float) : (int)sizeof(double);
-  h->wd.waves = waves_for(h->wd.wsz, vsize, h->wd.max_leaf, 16);
-  {
-    // more leaves than CUs: narrower workgroups that keep every leaf
-    // resident at once (k per CU) when they still have 6 or more waves
-    // (measured on 512 leaves: 7 resident waves beat 15 waves in two
-    // rounds, 4 resident waves lose to 7 in two rounds)
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        cus > 0 && np > cus) {
-      const int k = (np + cus - 1) / cus;
-      const int per = ((device_lds_limit() - leaf_flag_bytes(h->wd.max_leaf)) / k -
-                       leaf_flag_bytes(h->wd.max_leaf)) / std::max(1, h->wd.wsz * vsize);
-      if (per >= 6 && per < h->wd.waves) h->wd.waves = per;
-    }
-  }
-  h->wd.sep_wsz = h->wd.wsz;
-  h->wd.sep_waves = waves_for(h->wd.sep_wsz, vsize, 0, vsize == 4 ? 16 : 8);
-  if (h->wd.waves < 1)
-    return set_error(SNU_ERR_OOM, "leaf workspace of " + std::to_string(maxw) +
-                                      " values (the widest leaf's rows plus its boundary columns) exceeds the " +
-                                      std::to_string(device_lds_limit()) +
-                                      " bytes of LDS of one workgroup: use more leaves (np)");
-  // work lists: leaf separator-row workgroups, forward boundary workgroups,
-  // and boundary targets (separator rows receiving leaf contributions, each
-  // with its sources in ascending rank order)
+  // work lists: forward boundary workgroups and boundary targets (separator
+  // rows receiving leaf contributions, each with its sources in ascending
+  // rank order)
   WorkLists &W = h->wl;
   W = WorkLists();
-  for (int r = 0; r < np; r++)
-    for (int f = 0; f < P.leaves[r].nbr; f += h->wd.sep_waves) W.wg_leaf.push_back(r), W.wg_first.push_back(f);
-  h->wd.max_nt = max_nt;
-  for (int r = 0; r < np; r++)
-    if (h->h_leaves[r].nt > 0)
-      for (int f = 0; f < P.leaves[r].nbr; f += kTailRows) W.twg_leaf.push_back(r), W.twg_first.push_back(f);
   for (int r = 0; r < np; r++)
     for (int f = 0; f < P.leaves[r].nbr; f += 4) W.fwg_leaf.push_back(r), W.fwg_first.push_back(f);
   struct Src { int R, leaf, ri; };
@@ -1161,7 +1277,7 @@ int upload(snumat *h) {
   h->arena_len = P.leaf_total + P.dense_total;
   // ---- structure uploads ----
   int *d_rowptr, *d_diag, *d_wcol, *d_levptr, *d_levrows, *d_ulevptr, *d_ulevrows;
-  int *d_brow, *d_bcol_glob, *d_bcol_blk, *d_bcol_loc, *d_ftask, *d_btask, *d_ubnd, *d_sep_tail;
+  int *d_brow, *d_bcol_glob, *d_bcol_blk, *d_bcol_loc, *d_ftask, *d_btask, *d_ubnd;
   if ((rc = dalloc(h, &h->d_aval, (size_t)P.nnz))) return rc;
   if ((rc = dupload(h, &h->d_adest, std::vector<long long>(P.a_dest.begin(), P.a_dest.end())))) return rc;
   if ((rc = dupload(h, &d_rowptr, rowptr)) || (rc = dupload(h, &d_diag, diag)) ||
@@ -1171,18 +1287,14 @@ int upload(snumat *h) {
       (rc = dupload(h, &d_bcol_glob, bcol_glob)) || (rc = dupload(h, &d_bcol_blk, bcol_blk)) ||
       (rc = dupload(h, &d_bcol_loc, bcol_loc)) || (rc = dupload(h, &h->d_order, P.order)) ||
       (rc = dupload(h, &d_ftask, ftask)) || (rc = dupload(h, &d_btask, btask)) ||
-      (rc = dupload(h, &d_ubnd, ubnd)) || (rc = dupload(h, &d_sep_tail, sep_tail)))
+      (rc = dupload(h, &d_ubnd, ubnd)))
     return rc;
   WorkDev &wd = h->wd;
   wd.nleaves = np;
-  wd.nwg = (int)W.wg_leaf.size();
   wd.nfwg = (int)W.fwg_leaf.size();
-  wd.ntwg = (int)W.twg_leaf.size();
   wd.ntarget = (int)W.trow_glob.size();
   wd.ntile = (int)W.tile_blk.size();
-  if ((rc = dupload(h, &wd.wg_leaf, W.wg_leaf)) || (rc = dupload(h, &wd.wg_first, W.wg_first)) ||
-      (rc = dupload(h, &wd.twg_leaf, W.twg_leaf)) || (rc = dupload(h, &wd.twg_first, W.twg_first)) ||
-      (rc = dupload(h, &wd.fwg_leaf, W.fwg_leaf)) || (rc = dupload(h, &wd.fwg_first, W.fwg_first)) ||
+  if ((rc = dupload(h, &wd.fwg_leaf, W.fwg_leaf)) || (rc = dupload(h, &wd.fwg_first, W.fwg_first)) ||
       (rc = dupload(h, &wd.trow_blk, W.trow_blk)) || (rc = dupload(h, &wd.trow_loc, W.trow_loc)) ||
       (rc = dupload(h, &wd.trow_glob, W.trow_glob)) || (rc = dupload(h, &wd.tptr, W.tptr)) ||
       (rc = dupload(h, &wd.tsrc_leaf, W.tsrc_leaf)) || (rc = dupload(h, &wd.tsrc_ri, W.tsrc_ri)) ||
@@ -1222,11 +1334,9 @@ int upload(snumat *h) {
     D.ftask = d_ftask + o.task;
     D.btask = d_btask + o.task;
     D.ubnd = d_ubnd + o.row;
-    D.sep_tail = d_sep_tail + o.brow;
     D.dd = h->d_dd;
   }
-  if ((rc = dalloc(h, &wd.stall, 1)) || (rc = dalloc(h, &wd.half_done, np)) ||
-      (rc = dalloc(h, &h->d_vec, 1)) || (rc = dalloc(h, &h->d_perm, 2)) ||
+  if ((rc = dalloc(h, &h->d_vec, 1)) || (rc = dalloc(h, &h->d_perm, 2)) ||
       (rc = dalloc(h, &h->d_prep, 1)) || (rc = dalloc(h, &h->d_ir, 1)) ||
       (rc = dalloc(h, &h->d_active, 1)) || (rc = dalloc(h, &h->d_norms, 1)))
     return rc;
@@ -1235,24 +1345,14 @@ int upload(snumat *h) {
       return rc;
     rc = alloc_values<float>(h, S_total);
     if (!rc) {
-      for (int r = 0; r < np; r++) {
-        h->h_leaves[r].S = h->v32.S + off[r].S;
-        h->h_leaves[r].tail_u = h->v32.S + off[r].tu;
-        h->h_leaves[r].tail_l = h->v32.S + off[r].tl;
-        h->h_leaves[r].tail_p = h->v32.S + off[r].tp;
-      }
-      rc = upload_tail_gemm<float>(h);
+      for (int r = 0; r < np; r++) h->h_leaves[r].S = h->v32.S + off[r].S;
+      rc = build_front_plan<float>(h);
     }
   } else {
     rc = alloc_values<double>(h, S_total);
     if (!rc) {
-      for (int r = 0; r < np; r++) {
-        h->h_leaves[r].S = h->v64.S + off[r].S;
-        h->h_leaves[r].tail_u = h->v64.S + off[r].tu;
-        h->h_leaves[r].tail_l = h->v64.S + off[r].tl;
-        h->h_leaves[r].tail_p = h->v64.S + off[r].tp;
-      }
-      rc = upload_tail_gemm<double>(h);
+      for (int r = 0; r < np; r++) h->h_leaves[r].S = h->v64.S + off[r].S;
+      rc = build_front_plan<double>(h);
     }
   }
   if (rc) return rc;
@@ -1274,12 +1374,19 @@ int upload(snumat *h) {
 size_t estimate_device_bytes(const snumat *h) {
   const Plan &P = h->P;
   const double vs = h->prec == SNU_FP32 ? 4.0 : 8.0;
-  double S_total = 0, idx = 0, inv = 0;
+  double S_total = 0, idx = 0, inv = 0, fronts = 0;
+  const int fmax = h->prec == SNU_FP32 ? front_small_max<float>() : front_small_max<double>();
   for (const LeafPlan &L : P.leaves) {
     S_total += (double)L.nbr * L.nbc;
     idx += 2.0 * (L.core_n + 1) + L.lu_nnz + L.lev_ptr.size() + L.lev_rows.size() + L.ulev_ptr.size() +
            L.ulev_rows.size() + L.brow.size() + 3.0 * L.bcol.size() + 2.0 * L.nbr / 4 + 4;
+    // fronts: values (larger ones with their diagonal-block inverses), slot /
+    // position / extend-add tables, root row / column maps, descriptors
+    fronts += (double)L.fentries;
+    for (int f : L.ff) fronts += f > fmax ? 2.0 * 64 * 64 : 0.0;
+    idx += 2.0 * L.fslot.size() + L.frel.size() + (double)(L.nbr + L.nbc) + 16.0 * L.fk0.size();
   }
+  S_total += fronts;
   for (int p = 1; p < P.np; p++) inv += 2.0 * P.block_size[p] * P.block_size[p];
   double bytes = vs * ((double)P.leaf_total + (double)P.dense_total + S_total + inv +
                        (double)inversion_tmp_size(P));
@@ -1300,6 +1407,8 @@ void free_device(snumat *h) {
   free_gemv_plan(h->v32.sp);
   free_dense_plan(h->v64.fact);
   free_dense_plan(h->v32.fact);
+  free_dense_plan(h->v64.fronts.dense);
+  free_dense_plan(h->v32.fronts.dense);
   for (void *p : h->allocs) (void)hipFree(p);
   h->allocs.clear();
   for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
@@ -1349,15 +1458,23 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
   launch_prepare<T>(fs.prep, fs.nmat, fs.max_len, fs.max_nnz, st);
   if (fs.ir) launch_rowabs_max(fs.ir, fs.nmat, fs.max_n, fs.anorm_out, st);
   if (ev) HIPCHK(hipEventRecord((*ev)[1], st));
-  launch_zero_i32(wd.stall, 1, st);
-  launch_leaf_rows<T>(wd.leaves, wd.nleaves, wd.wsz, wd.waves, wd.max_leaf, fs.eps, wd.stall,
-                      wd.half_done, st);
+  // leaf fronts, height by height: whole fronts in LDS; larger fronts
+  // assembled, factored by the dense launches of their height, written back
+  const FrontPlan<T> &fp = *(const FrontPlan<T> *)fs.fronts;
+  size_t li = 0;
+  const size_t nl = fp.dense.launches.size();
+  for (int hh = 0; hh < (int)fp.levels.size(); hh++) {
+    const FrontLevel &lv = fp.levels[hh];
+    launch_front_small<T>(fp.d_small + lv.small0, lv.nsmall, lv.max_f, fs.eps, st);
+    launch_front_assemble<T>(fp.d_large, fp.d_work + lv.work0, lv.nwork, st);
+    for (; li < nl && front_key_height(fp.dense.launches[li].key) == hh; li++)
+      run_launch<T>(fp.dense.launches[li], fp.dense.d_gp, fp.dense.d_gs, fp.dense.d_dp, nullptr, nullptr,
+                    nullptr, st);
+    launch_front_writeback<T>(fp.d_large, fp.d_work + lv.work0, lv.nwork, st);
+  }
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[2], st));
-  launch_leaf_sep<T>(wd.leaves, wd.wg_leaf, wd.wg_first, wd.nwg, wd.sep_wsz, wd.sep_waves, st);
-  launch_leaf_tail<T>(wd.leaves, wd.nleaves, wd.twg_leaf, wd.twg_first, wd.ntwg, kTailRows,
-                      (const GemmProbT<T> *)wd.tail_gp, wd.tail_np, (const GemmSegT<T> *)wd.tail_gs,
-                      wd.tail_tiles, wd.max_nt, st);
+  launch_front_schur<T>(fp.d_ls, fp.d_tiles, (int)fp.tiles.size(), st);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[3], st));
   launch_assemble_schur<T>(wd.ntile, wd.tile_blk, wd.tile_row0, wd.tile_ptr, wd.seg_leaf, wd.seg_ri,
@@ -1419,10 +1536,7 @@ int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
   } else if ((rc = enqueue_factor<T>(fs, plan, st, ev, profile, pev))) {
     return rc;
   }
-  int stalled = 0;
-  HIPCHK(hipMemcpyAsync(&stalled, wd.stall, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  if (stalled) return set_error(SNU_ERR_HIP, "leaf factorization: a row's wait for its pivot rows timed out");
   if (profile && pev && gemm_ms) {
     HIPCHK(hipStreamSynchronize(st));
     gemm_ms[0] = gemm_ms[1] = gemm_ms[2] = 0;

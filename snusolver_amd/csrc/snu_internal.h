@@ -30,8 +30,18 @@ struct LeafPlan {
   std::vector<int> bcol;    // core cols of the boundary cols
   std::vector<int> lev_ptr, lev_rows;    // factor / forward level sets
   std::vector<int> ulev_ptr, ulev_rows;  // backward level sets
-  int split_a = 0, split_b = 0;          // rows [a, b) factor beside [0, a)
   double flops = 0;         // structural leaf flops (LU + Schur)
+  // multifrontal form of the leaf factorization (build_fronts), fronts in
+  // ascending pivot order (children before parents):
+  std::vector<int> fk0, fnk, ff;         // first pivot, pivots, front order
+  std::vector<int> fpar, fheight;        // parent front (-1: root), height
+  std::vector<int> fidx_ptr, fidx;       // B of each front: core indices beyond K
+  std::vector<int> frel;                 // per B entry: local index in the parent
+  std::vector<int> fslot_ptr, fslot, fpos;  // owned LU slots, front position (col-major)
+  std::vector<int> fchild_ptr, fchild;   // children, ascending
+  std::vector<int> fsrow, fscol;         // root fronts: per B entry, boundary row / col (-1)
+  long long fentries = 0;                // sum of front order squared
+  int fheight_max = 0;
 };
 
 struct Plan {
@@ -78,6 +88,9 @@ struct Plan {
 };
 
 int analyze(const snu_csr_t *A, const snu_opts_t *opts, Plan &P);
+// Multifrontal structure of a leaf from its core pattern and LU pattern
+// (boundary rows / cols set).
+int build_fronts(LeafPlan &L);
 
 int order_grid(int nx, int ny, int nz, int np, int *order, int *sizes);
 // method 0: the smaller of the multilevel and the level-set separator at

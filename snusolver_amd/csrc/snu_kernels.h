@@ -32,20 +32,43 @@ struct LeafDev {
   const int *bcol_glob, *bcol_blk, *bcol_loc;  // boundary columns: permuted index, block, offset
   void *vals;                            // leaf LU values (value type of the launch)
   void *S;                               // boundary Schur block, nbr x nbc row-major
-  void *tail_u;                          // nt x nbc: tail rows' U entries on boundary columns
-  void *tail_l;                          // nbr x nt: separator rows on the tail columns
-  void *tail_p;                          // packed upper triangle of the tail block (U)
-  const int *sep_tail;                   // per boundary row: first L entry on a tail column
   void *dense;                           // the matrix's dense separator arena
   const DenseDev *dd;                    // the matrix's separator tables
   int core_n, leaf_size, nbr, nbc, nlev, nulev;
-  int split_a, split_b;                  // rows [split_a, split_b) depend on none below split_a
-  int t0, nt;                            // dense tail rows [t0, leaf_size), nt of them (0: none)
   int gstart;                            // permuted index of the first leaf row
   int cb_off;                            // rows of the rhs contribution buffer
   int vec;                               // the matrix's slot in the launch's VecDev table
   int pad;
 };
+// Multifrontal leaf factorization.  A front is an f x f column-major matrix
+// whose first nk rows / columns are its pivots; after its partial LU the
+// trailing (f - nk) x (f - nk) block holds its update for the parent.
+struct FrontChild {
+  const void *U;     // the child's update block (its front at (nk, nk)), ld
+  const int *rel;    // per entry of the child's B: index in this front (ascending)
+  int ld, nb;        // the child's front order, |B| of the child
+};
+struct FrontDev {
+  void *F;                  // the front's values (f x f, column-major)
+  void *vals;               // the leaf's LU values (owned slots are read, then written)
+  const int *slot, *pos;    // owned LU slots and their positions in F (ascending positions)
+  const FrontChild *child;  // children in ascending order (their updates are added in it)
+  int f, nk, nslot, nchild;
+};
+// Boundary Schur block of one leaf from the updates of its root fronts: S
+// (nbr x nbc, row-major) = sum over roots of U[arow[r], acol[c]] (entries
+// whose row or column the root lacks, -1, add nothing).
+struct FrontRoot {
+  const void *U;
+  const int *arow, *acol;   // per S row / column: index in the root's B (-1)
+  int ld;
+};
+struct LeafSchurDev {
+  void *S;
+  const FrontRoot *root;
+  int nbr, nbc, nroot;
+};
+
 // Segment of a packed right-hand-side buffer: columns of length n at
 // base + c ld, c < nrhs.
 struct VecDev {
@@ -101,6 +124,7 @@ struct DiagProbT {
   int nb;
   T *Linv, *Uinv;  // nb x nb diagonal blocks of the full inverses
   long long ldi;
+  T eps;           // pivots of magnitude below eps become +-eps (0: none)
 };
 // y (M x nrhs) = / -= sum_seg A_seg (M x K) x_seg (K x nrhs); tri: 0 dense,
 // 1 lower triangular single segment (columns k <= row), 2 upper (k >= row).
@@ -129,13 +153,6 @@ using GemvSeg = GemvSegT<double>;
 using GemvProb = GemvProbT<double>;
 
 int device_lds_limit();
-// Leaves whose own top separator (the rows after the leaf's split) has at
-// most this many rows finish their separator rows on it as dense row solves.
-constexpr int kMaxLeafTail = 160;
-// separator rows per workgroup of the leaf tail kernel (two per wave): the
-// packed tail triangle in LDS is loaded once per workgroup, so a workgroup
-// takes many rows
-constexpr int kTailRows = 32;
 
 template <typename T>
 void launch_fill(T *p, long long n, hipStream_t st);
@@ -143,31 +160,9 @@ void launch_fill(T *p, long long n, hipStream_t st);
 template <typename T>
 void launch_prepare(const PrepDev *tab, int nmat, long long max_len, long long max_nnz,
                     hipStream_t st);
-// max_leaf: the largest leaf of the launch (its rows' completion flags
-// share the LDS with the waves' workspaces)
-// stall: set to 1 if a row gave up waiting for a pivot row (never expected;
-// the factors are then invalid and the caller reports an error)
-// half_done: nleaves ints (cleared here), the helper workgroups' completion
-template <typename T>
-void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, int max_leaf,
-                      double eps, int *stall, int *half_done, hipStream_t st);
-int leaf_flag_bytes(int max_leaf);
 void launch_zero_i32(int *p, int n, hipStream_t st);
 // dst = src, bytes a multiple of 16
 void launch_copy(const void *src, void *dst, long long bytes, hipStream_t st);
-// separator rows: L_SL row and Schur row per wave (workspace: leaf and
-// boundary columns)
-// Dense leaf tails (leaves with nt > 0): gathers the tail rows' U entries,
-// finishes the separator rows' tail multipliers (wg lists of the
-// separator-row launch) and their boundary values (tgp: one GEMM problem per
-// leaf, S -= multipliers x tail U on boundary columns).
-template <typename T>
-void launch_leaf_tail(const LeafDev *leaves, int nleaves, const int *wg_leaf, const int *wg_first,
-                      int nwg, int waves, const GemmProbT<T> *tgp, int ntgp,
-                      const GemmSegT<T> *tgs, int ttiles, int max_nt, hipStream_t st);
-template <typename T>
-void launch_leaf_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
-                     int wsz, int waves, hipStream_t st);
 // boundary Schur rows into the dense blocks, per tile of 64 separator rows
 template <typename T>
 void launch_assemble_schur(int ntile, const int *tile_blk, const int *tile_row0,
@@ -223,6 +218,26 @@ void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp
 template <typename T>
 void launch_leaf_solve(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp, int max_leaf,
                        int max_lev, int max_rhs, bool bwd, hipStream_t st);
+
+// ---- multifrontal leaf factorization ----
+// Fronts held whole in LDS (f <= max_f): assembly, partial LU with the
+// pivot clamp, write-back of the owned slots, update block to global memory.
+template <typename T>
+void launch_front_small(const FrontDev *fr, int nfront, int max_f, double eps, hipStream_t st);
+// Largest front order the LDS kernel takes for this value type.
+template <typename T>
+int front_small_max();
+// Larger fronts, in global memory: assembly (zero, owned slots, children's
+// updates) and write-back of the owned slots after the dense partial LU, one
+// workgroup per (front, 64-column chunk) work item.
+template <typename T>
+void launch_front_assemble(const FrontDev *fr, const int2 *work, int nwork, hipStream_t st);
+template <typename T>
+void launch_front_writeback(const FrontDev *fr, const int2 *work, int nwork, hipStream_t st);
+// Boundary Schur blocks from the root fronts, one workgroup per 64 x 64 S
+// tile (leaf, first row, first column).
+template <typename T>
+void launch_front_schur(const LeafSchurDev *ls, const int4 *tiles, int ntile, hipStream_t st);
 
 // ---- FP64 iterative refinement (one matrix per table entry) ----
 void launch_residual(const IrDev *tab, int nmat, int max_n, hipStream_t st);

@@ -77,424 +77,6 @@ void launch_fill(T *p, long long n, hipStream_t st) {
 }
 
 
-// ---------------------------------------------------------------------------
-// leaf factorization
-// ---------------------------------------------------------------------------
-// Factor one core row i with one wave.  w is the wave's LDS workspace,
-// indexed by workspace column (leaf columns, then the leaf's boundary
-// columns), all zero on entry and on exit.
-//   LEAF_ROW: L part ends at diag[i]; the diagonal is clamped to +-eps
-//             (kernel_sparse.cu:742-749); pivots update all their U entries
-//             (leaf and boundary columns).
-//   otherwise a separator row: every entry is an L entry, and the updates
-//             that land on boundary columns form the row's Schur
-//             contribution -sum_k L(r,k) U(k,c) (kernel_sparse.cu:953-971).
-// done (leaf rows only): per-row completion flags in LDS.  A pivot row is
-// used only after its flag is seen (then an acquire fence orders the loads
-// of its values after the owner's release), so rows of one leaf run
-// concurrently on the waves of the workgroup, each waiting only for the
-// pivots it has reached; the row's own flag is raised after its values are
-// stored.
-template <typename T, bool LEAF_ROW, int NCH, int NST>
-__device__ __forceinline__ void factor_row(int i, int pend, const int *__restrict__ rowptr,
-                           const int *__restrict__ diag,
-                           const int *__restrict__ wcol,
-                           T *__restrict__ val, T *w, int lane,
-                           T eps, volatile unsigned char *done, int *stall) {
-#pragma clang fp contract(off)
-  // Pivot rows are streamed NST - 1 pivots ahead: while pivot t is applied,
-  // the first NCH chunks (64 entries each) of the U rows of the next pivots
-  // are in flight into a ring of NST register sets (chunks past a
-  // row's end are not loaded; lanes past it read a clamped entry that is
-  // never used).  Longer U rows are loaded inside the update.  A pivot's U
-  // entries name distinct workspace columns, so its update gathers every
-  // workspace value it touches (one LDS round trip, overlapping the
-  // division) before it writes any of them back.
-  const int p0 = rowptr[i], p1 = rowptr[i + 1];
-  const int pd = LEAF_ROW ? diag[i] : pend;  // separator rows: pivots before pend only
-  for (int p = p0 + lane; p < p1; p += 64) w[wcol[p]] = val[p];
-  wave_lds_fence();
-  int cs[NST][NCH] = {};
-  T us[NST][NCH] = {};
-  // always inlined: a call would pass the register arrays through memory
-  auto fetch = [&](int s, int e, int(&c)[NCH], T(&u)[NCH]) __attribute__((always_inline)) {
-    const int n = (e - s + 63) >> 6;
-#pragma unroll
-    for (int j = 0; j < NCH; j++)
-      if (j < n) {
-        const int q = min(s + lane + 64 * j, e - 1);
-        c[j] = wcol[q];
-        u[j] = val[q];
-      }
-  };
-  auto apply = [&](int kt, int s, int e, T pt, const int(&c)[NCH], const T(&u)[NCH])
-                   __attribute__((always_inline)) {
-    const int n = (e - s + 63) >> 6;
-    const T wk = w[kt];
-    T wv[NCH];
-#pragma unroll
-    for (int j = 0; j < NCH; j++) wv[j] = (j < n && s + lane + 64 * j < e) ? w[c[j]] : T(0);
-    const T l = wk / pt;
-#pragma unroll
-    for (int j = 0; j < NCH; j++)
-      if (j < n && s + lane + 64 * j < e) {
-        T prod = l * u[j];
-        w[c[j]] = wv[j] - prod;
-      }
-    for (int q = s + lane + 64 * NCH; q < e; q += 64) {
-      int c2 = wcol[q];
-      T prod = l * val[q];
-      w[c2] = w[c2] - prod;
-    }
-    if (lane == 0) w[kt] = l;
-    wave_lds_fence();
-  };
-  for (int pb = p0; pb < pd;) {
-    int cnt = min(64, pd - pb);
-    int k = 0, ks = 1, ke = 1;
-    T piv = T(1);
-    if (lane < cnt) k = wcol[pb + lane];
-    if (LEAF_ROW) {
-      // the leading pivots of this batch whose rows are complete (at least
-      // one: the first pivot is waited for).  The wait is bounded: past
-      // kSpinLimit polls the row proceeds and the stall is reported, so the
-      // kernel always ends.
-      constexpr int kSpinLimit = 1 << 26;
-      for (int spin = 0;; spin++) {
-        const bool ready = lane >= cnt || done[k] != 0;
-        const unsigned long long notready = __ballot(!ready);
-        const int f = notready ? __builtin_ctzll(notready) : cnt;
-        if (f > 0) {
-          cnt = f;
-          break;
-        }
-        if (spin == kSpinLimit) {
-          if (lane == 0) *stall = 1;
-          cnt = 1;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    }
-    if (lane < cnt) {
-      int kd = diag[k];
-      ks = kd + 1;
-      ke = rowptr[k + 1];
-      piv = val[kd];
-    }
-    // pivot t of the batch is held in register set t mod NST; the loads of
-    // pivot t + NST - 1 are issued before pivot t is applied (the set
-    // indices are compile-time constants once the inner loop is unrolled)
-#pragma unroll
-    for (int j = 0; j < NST - 1; j++)
-      if (j < cnt) fetch(rl(ks, j), rl(ke, j), cs[j], us[j]);
-    for (int t = 0; t < cnt; t += NST) {
-#pragma unroll
-      for (int j = 0; j < NST; j++) {
-        const int tt = t + j;
-        if (tt >= cnt) break;
-        if (tt + NST - 1 < cnt)
-          fetch(rl(ks, tt + NST - 1), rl(ke, tt + NST - 1), cs[(j + NST - 1) % NST], us[(j + NST - 1) % NST]);
-        apply(rl(k, tt), rl(ks, tt), rl(ke, tt), rld(piv, tt), cs[j], us[j]);
-      }
-    }
-    pb += cnt;
-  }
-  if (LEAF_ROW) {
-    if (lane == 0) {
-      T d = w[i];
-      if (d < eps && d > -eps) d = d < 0 ? -eps : eps;
-      w[i] = d;
-    }
-    wave_lds_fence();
-  }
-  // separator rows stored up to pend; the workspace keeps the rest
-  for (int p = p0 + lane; p < (LEAF_ROW ? p1 : pend); p += 64) {
-    int c = wcol[p];
-    val[p] = w[c];
-    w[c] = T(0);
-  }
-  wave_lds_fence();
-  if (LEAF_ROW) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    if (lane == 0) done[i] = 1;
-  }
-}
-
-// With split set, two workgroups per leaf, 2L (helper) and 2L + 1 (main);
-// otherwise one, which factors every row (used when the leaves alone fill
-// the GPU: a main workgroup waiting for its helper would hold a CU).  A
-// leaf's rows
-// split as [0, sa) [sa, sb) [sb, ls): the middle range depends on no row
-// below sa (the two halves of the leaf's own dissection), so the helper
-// factors it while the main workgroup factors [0, sa); the main one then
-// waits for the helper's completion flag and factors [sb, ls), which may use
-// every earlier row.  The waiting workgroup has the higher index, so the
-// one it waits for was dispatched before it; the wait is bounded.  Leaves
-// without a split (sa == sb) run on the main workgroup alone.
-// Inside a workgroup, wave w takes rows w, w + W, ... of its ranges in
-// ascending order and waits (LDS completion flags) only for the pivot rows
-// it reaches, so the rows of a dense chain overlap; the lowest unfinished
-// row depends only on finished rows and its wave is working on it.
-// LDS: completion flags (fl bytes), then one workspace row (wsz values) per
-// wave.
-extern __shared__ __align__(16) char lds_raw[];
-template <typename T>
-__global__ __launch_bounds__(1024) void k_leaf_rows(const LeafDev *__restrict__ leaves,
-                                                   int wsz, int fl, T eps, int *stall,
-                                                   int *half_done, int split) {
-  const int leaf = split ? blockIdx.x >> 1 : blockIdx.x, role = split ? blockIdx.x & 1 : 1;
-  const LeafDev L = leaves[leaf];
-  const int sa = split ? L.split_a : 0, sb = split ? L.split_b : 0;
-  volatile unsigned char *done = (volatile unsigned char *)lds_raw;
-  T *lds = (T *)(lds_raw + fl);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int W = blockDim.x >> 6;
-  if (role == 0 && sa == sb) return;  // no helper work
-  for (int r = threadIdx.x; r < L.leaf_size; r += blockDim.x) done[r] = 0;
-  T *w = lds + (size_t)wave * wsz;
-  for (int c = lane; c < L.leaf_size + L.nbc; c += 64) w[c] = T(0);
-  __syncthreads();
-  const int *rp = L.rowptr, *dg = L.diag, *wc = L.wcol;
-  T *v = (T *)L.vals;
-  if (role == 0) {
-    for (int r = sa + wave; r < sb; r += W)
-      factor_row<T, true, 4, 4>(r, -1, rp, dg, wc, v, w, lane, eps, done, stall);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __threadfence();
-      __hip_atomic_store(half_done + leaf, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return;
-  }
-  for (int r = wave; r < sa; r += W)
-    factor_row<T, true, 4, 4>(r, -1, rp, dg, wc, v, w, lane, eps, done, stall);
-  if (sa < sb) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      constexpr int kSpinLimit = 1 << 26;
-      for (int spin = 0;; spin++) {
-        if (__hip_atomic_load(half_done + leaf, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)) break;
-        if (spin == kSpinLimit) {
-          *stall = 1;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int r = sa + threadIdx.x; r < sb; r += blockDim.x) done[r] = 1;
-    __syncthreads();
-  }
-  for (int r = sb + wave; r < L.leaf_size; r += W)
-    factor_row<T, true, 4, 4>(r, -1, rp, dg, wc, v, w, lane, eps, done, stall);
-}
-// LDS bytes of the completion flags for leaves of up to max_leaf rows
-int leaf_flag_bytes(int max_leaf) { return (max_leaf + 15) & ~15; }
-template <typename T>
-void launch_leaf_rows(const LeafDev *leaves, int nleaves, int wsz, int waves, int max_leaf,
-                      double eps, int *stall, int *half_done, hipStream_t st) {
-  if (nleaves <= 0) return;
-  const int fl = leaf_flag_bytes(max_leaf);
-  size_t lds = (size_t)fl + (size_t)wsz * waves * sizeof(T);
-  (void)hipFuncSetAttribute((const void *)k_leaf_rows<T>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  // split when both workgroups of every leaf are resident together
-  int dev = 0, cus = 0, per_cu = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_leaf_rows<T>,
-                                                     64 * waves, lds);
-  const int split = 2LL * nleaves <= (long long)cus * per_cu ? 1 : 0;
-  if (split) launch_zero_i32(half_done, nleaves, st);
-  hipLaunchKernelGGL(k_leaf_rows<T>, dim3(split ? 2 * nleaves : nleaves), dim3(64 * waves), lds,
-                     st, leaves, wsz, fl, (T)eps, stall, half_done, split);
-}
-
-// Separator rows of the leaves: each wave factors one boundary row (its L_SL
-// row) and writes its Schur contribution row into S.  (Splitting the two
-// halves into separate passes with smaller workspaces measured slower: the
-// cost is per pivot step, and the pass over the pivots is then made twice.)
-template <typename T>
-__global__ __launch_bounds__(sizeof(T) == 4 ? 1024 : 512) void k_leaf_sep(const LeafDev *__restrict__ leaves,
-                                                  const int *__restrict__ wg_leaf,
-                                                  const int *__restrict__ wg_first, int wsz) {
-  T *lds = (T *)lds_raw;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const LeafDev L = leaves[wg_leaf[blockIdx.x]];
-  const int ri = wg_first[blockIdx.x] + wave;
-  if (ri >= L.nbr) return;
-  T *w = lds + (size_t)wave * wsz;
-  const int ls = L.leaf_size;
-  for (int c = lane; c < ls + L.nbc; c += 64) w[c] = T(0);
-  wave_lds_fence();
-  // with a dense tail (nt > 0) only the pivots left of it are applied here;
-  // the row's tail values go to tail_l and its boundary values, still
-  // missing the tail pivots' updates, to S (k_leaf_tail_l / _s finish both)
-  const int i = L.brow[ri];
-  const int pend = L.nt ? L.sep_tail[ri] : L.rowptr[i + 1];
-  // FP32: 16 waves fit the LDS; 12 prefetched chunks keep the registers
-  // within the 128 per lane that 16 waves leave.  FP64: the first 8 chunks
-  // of the U rows of the next 3 pivots in flight (8 x 4 measured 7.6 ms on
-  // the 64^3 separator rows against 10.0 for 16 x 2, 7.8 for 4 x 8 and
-  // 8.9 for 2 x 16)
-  factor_row<T, false, sizeof(T) == 4 ? 12 : 8, sizeof(T) == 4 ? 2 : 4>(i, pend, L.rowptr, L.diag, L.wcol, (T *)L.vals, w, lane, T(0),
-                              nullptr, nullptr);
-  if (L.nt) {
-    T *trow = (T *)L.tail_l + (long long)ri * L.nt;
-    for (int c = lane; c < L.nt; c += 64) {
-      trow[c] = w[L.t0 + c];
-      w[L.t0 + c] = T(0);
-    }
-  }
-  T *Srow = (T *)L.S + (long long)ri * L.nbc;
-  for (int j = lane; j < L.nbc; j += 64) {
-    Srow[j] = w[ls + j];
-    w[ls + j] = T(0);
-  }
-}
-template <typename T>
-void launch_leaf_sep(const LeafDev *leaves, const int *wg_leaf, const int *wg_first, int nwg,
-                     int wsz, int waves, hipStream_t st) {
-  if (nwg <= 0) return;
-  size_t lds = (size_t)wsz * waves * sizeof(T);
-  (void)hipFuncSetAttribute((const void *)k_leaf_sep<T>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_leaf_sep<T>, dim3(nwg), dim3(64 * waves), lds, st, leaves, wg_leaf,
-                     wg_first, wsz);
-}
-
-// ---------------------------------------------------------------------------
-// leaf tails: the last nt rows of a leaf (its own top separator, dense after
-// the elimination below it) finish the separator rows as dense row solves
-// ---------------------------------------------------------------------------
-// Packed upper triangle of the nt x nt tail block: row r, column c >= r.
-__device__ __forceinline__ int tail_pk(int r, int c, int nt) {
-  return r * nt - ((r * (r - 1)) >> 1) + (c - r);
-}
-// From the tail rows' U entries: tail_u (nt x nbc, row-major) <- those on
-// boundary columns, tail_p (packed upper triangle) <- those on tail columns,
-// zeros elsewhere.  One workgroup per leaf.
-template <typename T>
-__global__ __launch_bounds__(256) void k_leaf_tail_gather(const LeafDev *__restrict__ leaves) {
-  const LeafDev L = leaves[blockIdx.x];
-  if (!L.nt) return;
-  T *tu = (T *)L.tail_u, *tp = (T *)L.tail_p;
-  const int nt = L.nt;
-  const long long n = (long long)nt * L.nbc, npk = (long long)nt * (nt + 1) / 2;
-  for (long long e = threadIdx.x; e < n; e += blockDim.x) tu[e] = T(0);
-  for (long long e = threadIdx.x; e < npk; e += blockDim.x) tp[e] = T(0);
-  __syncthreads();
-  const int ls = L.leaf_size, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const T *v = (const T *)L.vals;
-  for (int r = wave; r < nt; r += blockDim.x >> 6) {
-    const int i = L.t0 + r;
-    for (int p = L.diag[i] + lane; p < L.rowptr[i + 1]; p += 64) {
-      const int c = L.wcol[p];
-      if (c >= ls) tu[(long long)r * L.nbc + (c - ls)] = v[p];
-      else tp[tail_pk(r, c - L.t0, nt)] = v[p];
-    }
-  }
-}
-// Separator rows on the tail: each wave takes two rows of tail_l (their
-// workspace values on the tail columns) and applies the tail pivots in
-// ascending order exactly as the sparse row elimination does (l = w_k /
-// U_kk, then w_c -= l U_kc for c > k; no FMA contraction), with the tail's
-// upper triangle packed in LDS; the multipliers replace the row in tail_l
-// and are stored into the row's L entries on tail columns.  Lane j holds
-// columns j, j + 64, j + 128.
-template <typename T>
-__global__ __launch_bounds__(1024) void k_leaf_tail_l(const LeafDev *__restrict__ leaves,
-                                                     const int *__restrict__ wg_leaf,
-                                                     const int *__restrict__ wg_first, int group) {
-#pragma clang fp contract(off)
-  T *ut = (T *)lds_raw;
-  const LeafDev L = leaves[wg_leaf[blockIdx.x]];
-  const int nt = L.nt;
-  if (!nt) return;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int npk = nt * (nt + 1) / 2;
-  const T *tp = (const T *)L.tail_p;
-  for (int e = threadIdx.x; e < npk; e += blockDim.x) ut[e] = tp[e];
-  __syncthreads();
-  // two rows per wave (two independent dependency chains interleave)
-  // rows [first, first + group) of the separator-row group
-  const int first = wg_first[blockIdx.x], end = min(first + group, L.nbr);
-  const int ra = first + 2 * wave, rb = ra + 1;
-  if (ra >= end) return;
-  const bool has_b = rb < end;
-  T *trow_a = (T *)L.tail_l + (long long)ra * nt;
-  T *trow_b = (T *)L.tail_l + (long long)(has_b ? rb : ra) * nt;
-  T xa[3], xb[3];
-#pragma unroll
-  for (int s = 0; s < 3; s++) {
-    const bool in = lane + 64 * s < nt;
-    xa[s] = in ? trow_a[lane + 64 * s] : T(0);
-    xb[s] = in ? trow_b[lane + 64 * s] : T(0);
-  }
-#pragma unroll
-  for (int s = 0; s < 3; s++) {
-    const int kend = min(nt, 64 * (s + 1));
-    for (int k = 64 * s; k < kend; k++) {
-      // every LDS operand of the step first (addresses clamped to the
-      // diagonal for lanes the step does not update), one wait
-      const int base = tail_pk(k, k, nt);
-      T u[3];
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        const int c = lane + 64 * q;
-        u[q] = ut[base + ((c > k && c < nt) ? c - k : 0)];
-      }
-      const T d = ut[base];
-      const T la = rld(xa[s], k - 64 * s) / d;
-      const T lb = rld(xb[s], k - 64 * s) / d;
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        const int c = lane + 64 * q;
-        if (c > k && c < nt) {
-          T pa = la * u[q], pb = lb * u[q];
-          xa[q] = xa[q] - pa;
-          xb[q] = xb[q] - pb;
-        }
-      }
-      if (lane == k - 64 * s) xa[s] = la, xb[s] = lb;
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < 3; s++)
-    if (lane + 64 * s < nt) {
-      trow_a[lane + 64 * s] = xa[s];
-      if (has_b) trow_b[lane + 64 * s] = xb[s];
-    }
-  // the rows' L entries on tail columns (pattern order, columns ascending)
-  T *vw = (T *)L.vals;
-  for (int h = 0; h < (has_b ? 2 : 1); h++) {
-    const int ri = h ? rb : ra;
-    const T *trow = h ? trow_b : trow_a;
-    const int i = L.brow[ri];
-    for (int p = L.sep_tail[ri] + lane; p < L.rowptr[i + 1]; p += 64) vw[p] = trow[L.wcol[p] - L.t0];
-  }
-}
-template <typename T>
-void launch_leaf_tail(const LeafDev *leaves, int nleaves, const int *wg_leaf, const int *wg_first,
-                      int nwg, int waves, const GemmProbT<T> *tgp, int ntgp,
-                      const GemmSegT<T> *tgs, int ttiles, int max_nt, hipStream_t st) {
-  if (max_nt <= 0 || nleaves <= 0) return;
-  hipLaunchKernelGGL(k_leaf_tail_gather<T>, dim3(nleaves), dim3(256), 0, st, leaves);
-  const size_t lds_l = (size_t)max_nt * (max_nt + 1) / 2 * sizeof(T);
-  (void)hipFuncSetAttribute((const void *)k_leaf_tail_l<T>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_l);
-  if (nwg > 0)  // the tail-row groups of `waves` rows, two rows per wave
-    hipLaunchKernelGGL(k_leaf_tail_l<T>, dim3(nwg), dim3(64 * ((waves + 1) / 2)), lds_l, st, leaves,
-                       wg_leaf, wg_first, waves);
-  // S -= tail multipliers x tail U on boundary columns: one grouped GEMM,
-  // as S^T (nbc x nbr) -= tail_u^T (nbc x nt) tail_l^T (nt x nbr) in the
-  // column-major terms of the kernel
-  launch_gemm(tgp, ntgp, tgs, ttiles, 0, st);
-}
-
 // Slot of separator entry (bi, lr) x (bj, lc) in the dense arena.
 __device__ __forceinline__ long long sep_index(const DenseDev &d, int bi,
                                                int lr, int bj, int lc) {
@@ -1224,7 +806,7 @@ __device__ __forceinline__ T lane_value(T v, int lane) {
 }
 template <typename T, int J>
 __device__ __forceinline__ void diag_step(T (&x)[8], T (&xl)[8], T (&xu)[8], int r, int g,
-                                          T (*lb)[64], T (*yb)[64]) {
+                                          T (*lb)[64], T (*yb)[64], T eps) {
   constexpr int b = J & 1;
   if (g == (J & 7)) {
     lb[b][r] = x[J >> 3];
@@ -1241,8 +823,11 @@ __device__ __forceinline__ void diag_step(T (&x)[8], T (&xl)[8], T (&xu)[8], int
   // keep the scheduler from interleaving steps (register pressure)
   __builtin_amdgcn_sched_barrier(0);
   const bool below = r > J;
-  // lb[b][J]: column J at row J, the pivot U_JJ
-  const T m = (below ? lb[b][r] : yb[b][r]) / lb[b][J];
+  // lb[b][J]: column J at row J, the pivot U_JJ (clamped to +-eps when
+  // smaller in magnitude: the leaf pivots' rule; eps = 0 leaves it as is)
+  T piv = lb[b][J];
+  if (piv < eps && piv > -eps) piv = piv < T(0) ? -eps : eps;
+  const T m = (below ? lb[b][r] : yb[b][r]) / piv;
   // multipliers zeroed on the lanes an update does not apply to (x - 0 u is
   // x exactly)
   const T mb = below ? m : T(0), ma = below ? T(0) : m;
@@ -1259,7 +844,7 @@ __device__ __forceinline__ void diag_step(T (&x)[8], T (&xl)[8], T (&xu)[8], int
       const T nx = x[q] - mb * uq[q];
       const T ny = xu[q] - ma * uq[q];
       const T nl = xl[q] - mb * xq[q];
-      x[q] = right ? nx : (at && below ? m : x[q]);
+      x[q] = right ? nx : (at ? (below ? m : (r == J ? piv : x[q])) : x[q]);
       xu[q] = right ? ny : (at && !below ? m : xu[q]);
       xl[q] = right ? xl[q] : nl;
     }
@@ -1273,8 +858,8 @@ __device__ __forceinline__ void diag_step(T (&x)[8], T (&xl)[8], T (&xu)[8], int
 template <typename T, int... J>
 __device__ __forceinline__ void diag_steps(std::integer_sequence<int, J...>, T (&x)[8],
                                            T (&xl)[8], T (&xu)[8], int r, int g, T (*lb)[64],
-                                           T (*yb)[64]) {
-  (diag_step<T, J>(x, xl, xu, r, g, lb, yb), ...);
+                                           T (*yb)[64], T eps) {
+  (diag_step<T, J>(x, xl, xu, r, g, lb, yb, eps), ...);
 }
 template <typename T>
 __global__ __launch_bounds__(512) void k_diag_lu(const DiagProbT<T> *__restrict__ probs) {
@@ -1291,7 +876,7 @@ __global__ __launch_bounds__(512) void k_diag_lu(const DiagProbT<T> *__restrict_
   }
   // all 64 steps run: past nb the padding is an identity block, whose steps
   // change nothing (zero multipliers)
-  diag_steps<T>(std::make_integer_sequence<int, 64>{}, x, xl, xu, r, g, lb, yb);
+  diag_steps<T>(std::make_integer_sequence<int, 64>{}, x, xl, xu, r, g, lb, yb, P.eps);
   // the descriptor is read again (through a pointer the compiler cannot
   // equate with the first), so no store address stays live across the steps
   const DiagProbT<T> *pq = probs;
@@ -1552,6 +1137,7 @@ void launch_leaf_bwd(const LeafDev *leaves, int nleaves, const VecDev *vd, T *bp
 // leaf columns.  Columns go in batches of cmax.
 // LDS: [level offsets | tasks at task_off | cmax x ls values at xs_off].
 // Threads per leaf: the waves take a level's rows in turn.
+extern __shared__ __align__(16) char lds_raw[];
 constexpr int kLeafSolveThreads = 1024;
 template <typename T, bool BWD>
 __global__ __launch_bounds__(kLeafSolveThreads) void k_leaf_solve(const LeafDev *__restrict__ leaves,
@@ -1877,13 +1463,6 @@ int device_lds_limit() {
 #define SNU_INSTANTIATE(T)                                                                    \
   template void launch_fill<T>(T *, long long, hipStream_t);                                  \
   template void launch_prepare<T>(const PrepDev *, int, long long, long long, hipStream_t);   \
-  template void launch_leaf_rows<T>(const LeafDev *, int, int, int, int, double, int *, int *,\
-                                    hipStream_t);                                              \
-  template void launch_leaf_tail<T>(const LeafDev *, int, const int *, const int *, int, int,  \
-                                    const GemmProbT<T> *, int, const GemmSegT<T> *, int, int,   \
-                                    hipStream_t);                                              \
-  template void launch_leaf_sep<T>(const LeafDev *, const int *, const int *, int, int, int,  \
-                                   hipStream_t);                                              \
   template void launch_assemble_schur<T>(int, const int *, const int *, const int *,          \
                                          const int *, const int *, const LeafDev *,           \
                                          hipStream_t);                                        \

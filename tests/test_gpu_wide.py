@@ -7,7 +7,8 @@ full-size 64^3 case and a 512-leaf case exercise the production launch
 shapes (128 x 128 Schur tiles, the narrow resident leaf workgroups).
 
 Bars as in test_gpu_parity: x within 1e-9 relative (max norm) of the oracle,
-relative backward error <= 1e-12; leaf LU values bit-exact; stage-1 separator
+relative backward error <= 1e-12; leaf LU values within 1e-12 (normwise per
+leaf); stage-1 separator
 blocks (A_SS plus the leaf Schur contributions) within 1e-13 of the oracle's
 (the dense tail of a leaf is summed on MFMA) with the same zero pattern.
 """
@@ -20,7 +21,7 @@ import time
 import numpy as np
 import pytest
 
-from test_gpu_parity import _batch_members
+from test_gpu_parity import LEAF_TOL, _batch_members, leaf_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -182,8 +183,8 @@ def test_serve_stream_vs_oracle(S, oracle_mod, precision):
 
 def test_c2_full_size_vs_oracle(S, oracle_mod):
     """64^3 7-point, 128 leaves, geometric ND (the benchmark's workload):
-    every leaf's LU bit-exact, x against the oracle; the factorization runs
-    the 128 x 128 Schur launches."""
+    every leaf's LU against the oracle's, x against the oracle; the
+    factorization runs the 128 x 128 Schur launches."""
     A, grid, np_ = S.config_matrix("c2")
     order, sizes = S.order_grid(*grid, np_)
     _, b = S.gen_rhs(A, 1, 1)
@@ -195,7 +196,7 @@ def test_c2_full_size_vs_oracle(S, oracle_mod):
     with S.SnuMat(A, np_, order=order, sizes=sizes, device=0) as M:
         x = M.factorize().solve(b)
         split = M.gemm_split()
-        bad = [r for r in range(np_) if not np.array_equal(M.leaf_values(r), O.leaf_values(r))]
+        bad = [r for r in range(np_) if leaf_rel(M.leaf_values(r), O.leaf_values(r)) > LEAF_TOL]
     O.close()
     print(f"  128x128 Schur launches {split['t128_launches']:.0f}, 64x64 {split['t64_launches']:.0f}")
     assert not bad, bad
@@ -209,9 +210,8 @@ def test_c2_full_size_vs_oracle(S, oracle_mod):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_many_leaves_27pt_vs_oracle(S, oracle_mod, precision):
     """27-point anisotropic operator (the precision workload's recipe) on a
-    48^3 grid with 512 leaves, more leaves than CUs: the leaf rows run in the
-    narrower resident workgroups.  Leaf LU bit-exact (FP64), x against the
-    oracle in FP64 and FP32 + refinement."""
+    48^3 grid with 512 leaves, more leaves than CUs.  Leaf LU against the
+    oracle's (FP64), x against the oracle in FP64 and FP32 + refinement."""
     A = S.gen_grid(48, 48, 48, 27, 1, 0.0, 0)
     order, sizes = S.order_grid(48, 48, 48, 512)
     _, b = S.gen_rhs(A, 1, 1)
@@ -223,9 +223,9 @@ def test_many_leaves_27pt_vs_oracle(S, oracle_mod, precision):
         info = M.info()
         bad = []
         if precision == "fp64":
-            bad = [r for r in range(512) if not np.array_equal(M.leaf_values(r), O.leaf_values(r))]
+            bad = [r for r in range(512) if leaf_rel(M.leaf_values(r), O.leaf_values(r)) > LEAF_TOL]
     O.close()
-    print(f"  leaf-row waves per workgroup {info['waves']}")
+    print(f"  leaf fronts {info['leaf_fronts']}")
     assert not bad, bad
     rel = _rel(x, xo)
     berr = S.backward_error(A, x, b)
@@ -237,8 +237,8 @@ def test_stage1_blocks_vs_oracle(S, oracle_mod):
     """After the leaves and the boundary Schur assembly (stage 1) every dense
     separator block equals the oracle's assembled block (A_SS entries, then
     each leaf's contribution in ascending rank) within 1e-13 relative, with
-    the same zero pattern.  Leaves without a dense tail give bit-equal rows;
-    a tail's part is summed on MFMA in another order."""
+    the same zero pattern (the leaves' fronts sum the contributions in
+    another order than the oracle's row-by-row elimination)."""
     cases = [((20, 20, 20), 7, 0, 6.1, 16), ((33, 17, 1), 9, 0, 8.1, 8), ((10, 9, 8), 27, 1, 0.0, 8),
              ((24, 24, 24), 7, 0, 6.1, 32)]
     for grid, st, rec, dg, np_ in cases:
@@ -346,20 +346,22 @@ def test_driver_writes_x_vs_oracle(S, oracle_mod, precision):
             assert S.backward_error(A, x, b) <= BERR_TOL
 
 
-def test_wide_leaves_refused_or_refined(S):
-    """A plan whose widest leaf workspace exceeds the LDS is refused by
-    construct with a clear error; the one-shot solve (the reference's
-    solve()) doubles the leaves until it fits and solves; x against SuperLU
-    (scipy) and the backward error."""
+def test_wide_leaf_factorizes(S):
+    """One leaf of 48,400 rows (np = 1): its fronts reach well beyond the
+    LDS of a workgroup and go through the global-memory front path; the
+    handle and the one-shot solve (the reference's solve()) both solve; x
+    against SuperLU (scipy) and the backward error."""
     import ctypes as C
 
     import scipy.sparse.linalg as spla
     A = S.gen_grid(220, 220, 1, 5, 0, 4.1, 9)
     _, b = S.gen_rhs(A, 1, 1)
     b = b[:, 0]
-    with pytest.raises(S.SnuError) as e:
-        S.SnuMat(A, 1, grid=(220, 220, 1), device=0)
-    assert e.value.code == -7 and "LDS" in str(e.value) and "np" in str(e.value)
+    xs = spla.spsolve(A.to_scipy().tocsc(), b)
+    with S.SnuMat(A, 1, grid=(220, 220, 1), device=0) as M:
+        x = M.factorize().solve(b)
+    assert _rel(x, xs) <= REL_TOL
+    assert S.backward_error(A, x, b) <= BERR_TOL
     o = S._Opts()
     S.lib().snu_default_opts(C.byref(o))
     o.np = 1
@@ -368,7 +370,6 @@ def test_wide_leaves_refused_or_refined(S):
     x = np.zeros(A.n)
     a = A.c()
     S._check(S.lib().snu_solve_csr(C.byref(a), S._dp(b), S._dp(x), C.byref(o)))
-    xs = spla.spsolve(A.to_scipy().tocsc(), b)
     assert _rel(x, xs) <= REL_TOL
     assert S.backward_error(A, x, b) <= BERR_TOL
 

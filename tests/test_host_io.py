@@ -112,18 +112,19 @@ def test_drivers_usage(name):
     assert r.returncode == 1 and "Usage" in r.stderr
 
 
-def test_leaf_workspace_shrinks_with_np(S):
-    """The leaf-row workspace (widest leaf's rows plus its boundary columns)
-    is what the LDS must hold; doubling the leaves shrinks it, which is how
-    the one-shot solve fits large matrices."""
+def test_wide_leaf_fronts(S):
+    """A single leaf of 40,000 rows (2D 5-point grid, np = 1) is factored as
+    fronts: the pivots are partitioned into contiguous ranges, the largest
+    front is of the order of the widest separator (not of the leaf), and
+    more leaves give smaller fronts."""
     A = S.gen_grid(200, 200, 1, 5, 0, 4.1, 0)
-    import ctypes as C
-    ws = []
-    for np_ in (1, 4, 16):
+    biggest = []
+    for np_ in (1, 4):
         with S.SnuMat(A, np_, grid=(200, 200, 1), device=-1) as M:
-            v = C.c_int()
-            S._check(S.lib().snumat_leaf_workspace(M._h, C.byref(v)))
-            ls = M.info()["max_leaf_size"]
-            assert v.value >= ls and v.value % 2 == 0
-            ws.append(v.value)
-    assert ws[0] >= 40000 and ws[0] > ws[1] > ws[2]
+            f = M.leaf_fronts(0)
+            ls = M.leaf_sizes(0)[0]
+            assert f["k0"][0] == 0 and f["k0"][-1] + f["nk"][-1] == ls
+            assert np.all(f["k0"][1:] == f["k0"][:-1] + f["nk"][:-1])
+            biggest.append(int(f["f"].max()))
+            assert biggest[-1] <= 3 * 200
+    assert biggest[0] >= biggest[1]
