@@ -23,9 +23,12 @@ namespace snu {
 
 int64_t Plan::sep_index(int bi, int lr, int bj, int lc) const {
   if (bi == bj) return offR[bi] + lr + (int64_t)lc * block_size[bi];
-  if (bi > bj)  // row block deeper: U panel of bi
-    return offR[bi] + lr + (int64_t)(block_size[bi] + anc_off(bi, bj) + lc) * block_size[bi];
-  return offL[bj] + (anc_off(bj, bi) + lr) + (int64_t)lc * anc_sum[bj];
+  if (bi > bj) {  // row block deeper: a stored column of the U panel of bi
+    const int c = umap[bi][anc_off(bi, bj) + lc];
+    return c < 0 ? -1 : offR[bi] + lr + (int64_t)(block_size[bi] + c) * block_size[bi];
+  }
+  const int r = lmap[bj][anc_off(bj, bi) + lr];  // a stored row of the L panel of bj
+  return r < 0 ? -1 : offL[bj] + r + (int64_t)lc * nL[bj];
 }
 
 namespace {
@@ -421,43 +424,15 @@ int analyze(const snu_csr_t *A, const snu_opts_t *opts, Plan &P) {
   });
   for (int r = 0; r < np; r++)
     if (rcs[r]) return set_error(rcs[r], "structurally zero diagonal in leaf of rank " + std::to_string(r));
-  // ---- dense single-copy layout ----
   P.anc_sum.assign(nb + 1, 0);
-  P.offR.assign(nb + 1, 0);
-  P.offL.assign(nb + 1, 0);
-  int64_t off = 0;
   for (int p = 1; p < np; p++) {
     int a = 0;
     for (int q = p / 2; q; q /= 2) a += P.block_size[q];
     P.anc_sum[p] = a;
-    int64_t s = P.block_size[p];
-    P.offR[p] = off;
-    off += s * (s + a);
-    P.offL[p] = off;
-    off += (int64_t)a * s;
   }
-  P.dense_total = off;
   P.leaf_off.assign(np + 1, 0);
   for (int r = 0; r < np; r++) P.leaf_off[r + 1] = P.leaf_off[r] + P.leaves[r].lu_nnz;
   P.leaf_total = P.leaf_off[np];
-  // ---- destination of every entry of A ----
-  {
-    std::vector<int64_t> dest_q(nnz, -1);
-    for (int i = nb; i >= 1; i--) {
-      for (int q = P.mstart[i]; q < P.mstart[i] + P.msize[i]; q++) {
-        if (i >= np) {
-          const LeafPlan &L = P.leaves[np + np - 1 - i];
-          dest_q[q] = P.leaf_off[L.rank] + L.core_map[q - L.coo_off];
-        } else {
-          int bi = block[P.coo_r[q]], bj = block[P.coo_c[q]];
-          dest_q[q] = P.leaf_total + P.sep_index(bi, P.coo_r[q] - P.block_start[bi], bj,
-                                                 P.coo_c[q] - P.block_start[bj]);
-        }
-      }
-    }
-    P.a_dest.assign(nnz, 0);
-    for (int k = 0; k < nnz; k++) P.a_dest[k] = dest_q[P.perm_map[k]];
-  }
   // ---- structural nonzeros of the separator panels ----
   // Sources: A's separator-separator entries, every leaf's boundary Schur
   // block (its boundary rows x boundary cols), and the Schur update of each
@@ -528,12 +503,71 @@ int analyze(const snu_csr_t *A, const snu_opts_t *opts, Plan &P) {
       }
     }
   }
+  // ---- dense single-copy layout, tile-compressed panels ----
+  // Per separator p: R_p = [D_p | Up_p] (s x (s + nU_p), ld s) and Lp_p
+  // (nL_p x s, ld nL_p), where Up_p keeps, of each ancestor's columns, the
+  // 64-column tiles (aligned to that ancestor's first column) holding a
+  // structural nonzero, and Lp_p likewise its ancestors' 64-row tiles; every
+  // structurally nonzero entry lies in a kept tile, and the tiles are those
+  // of the dense factor, update and solve launches, so a kept tile is one
+  // contiguous run of the compressed panel.
+  P.nL.assign(nb + 1, 0);
+  P.nU.assign(nb + 1, 0);
+  P.lmap.assign(nb + 1, std::vector<int>());
+  P.umap.assign(nb + 1, std::vector<int>());
+  P.offR.assign(nb + 1, 0);
+  P.offL.assign(nb + 1, 0);
+  int64_t off = 0;
+  for (int p = 1; p < np; p++) {
+    const int a = P.anc_sum[p];
+    std::vector<int> &lm = P.lmap[p], &um = P.umap[p];
+    lm.assign(a, -1);
+    um.assign(a, -1);
+    int nl = 0, nu = 0;
+    for (int q = p / 2; q; q /= 2) {
+      const int o = P.anc_off(p, q), sq = P.block_size[q];
+      for (int t = 0; t < sq; t += 64) {
+        const int x0 = o + t, x1 = o + std::min(sq, t + 64);
+        if (P.anyL(p, x0, x1))
+          for (int x = x0; x < x1; x++) lm[x] = nl++;
+        if (P.anyU(p, x0, x1))
+          for (int x = x0; x < x1; x++) um[x] = nu++;
+      }
+    }
+    P.nL[p] = nl;
+    P.nU[p] = nu;
+    int64_t s = P.block_size[p];
+    P.offR[p] = off;
+    off += s * (s + nu);
+    P.offL[p] = off;
+    off += (int64_t)nl * s;
+  }
+  P.dense_total = off;
+  // ---- destination of every entry of A ----
+  {
+    std::vector<int64_t> dest_q(nnz, -1);
+    for (int i = nb; i >= 1; i--) {
+      for (int q = P.mstart[i]; q < P.mstart[i] + P.msize[i]; q++) {
+        if (i >= np) {
+          const LeafPlan &L = P.leaves[np + np - 1 - i];
+          dest_q[q] = P.leaf_off[L.rank] + L.core_map[q - L.coo_off];
+        } else {
+          int bi = block[P.coo_r[q]], bj = block[P.coo_c[q]];
+          const int64_t d = P.sep_index(bi, P.coo_r[q] - P.block_start[bi], bj, P.coo_c[q] - P.block_start[bj]);
+          if (d < 0) return set_error(SNU_ERR_STRUCTURE, "separator entry outside the kept panel tiles (internal error)");
+          dest_q[q] = P.leaf_total + d;
+        }
+      }
+    }
+    P.a_dest.assign(nnz, 0);
+    for (int k = 0; k < nnz; k++) P.a_dest[k] = dest_q[P.perm_map[k]];
+  }
   // ---- algorithmic counts ----
   double fd = 0, bsolve = 0, fl = 0, bleaf = 0;
   for (int p = 1; p < np; p++) {
     double s = P.block_size[p], a = P.anc_sum[p];
     fd += 2.0 / 3.0 * s * s * s + 2.0 * s * s * a + 2.0 * s * a * a;
-    bsolve += 8.0 * (s * s + 2.0 * s * a);
+    bsolve += 8.0 * (s * s + s * ((double)P.nL[p] + P.nU[p]));  // inverses' triangles + stored panels
   }
   for (const LeafPlan &L : P.leaves) {
     fl += L.flops;

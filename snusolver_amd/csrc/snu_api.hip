@@ -762,20 +762,45 @@ extern "C" int snumat_sep_block(const snumat_t *h, int i, int j, double *out) {
   if (x != y) return set_error(SNU_ERR_BAD_INPUT, "blocks not on one root path");
   const int si = P.block_size[i], sj = P.block_size[j];
   if (!si || !sj) return SNU_OK;
-  // every block is a strided view of the arena: column c of block (i, j)
-  // holds rows 0..si-1 contiguously at sep_index(i, 0, j, c), and the
-  // columns are equally spaced, so one 2-D copy reads exactly the block
-  const long long c0 = P.leaf_total + P.sep_index(i, 0, j, 0);
-  const long long ld = sj > 1 ? P.sep_index(i, 0, j, 1) - P.sep_index(i, 0, j, 0) : si;
+  // Rows (U panel) or columns (L panel) of the block that fall in panel tiles
+  // not stored are zero.  The stored ones come in runs, each a strided view
+  // of the arena: one 2-D copy per run.
+  std::fill(out, out + (size_t)si * sj, 0.0);
   HIPCHK(hipSetDevice(h->device));
-  if (h->prec == SNU_FP32) {
-    std::vector<float> tmp((size_t)si * sj);
-    HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * si, h->v32.arena + c0, sizeof(float) * ld,
-                       sizeof(float) * si, sj, hipMemcpyDeviceToHost));
-    for (size_t q = 0; q < tmp.size(); q++) out[q] = tmp[q];
-  } else {
-    HIPCHK(hipMemcpy2D(out, sizeof(double) * si, h->v64.arena + c0, sizeof(double) * ld,
-                       sizeof(double) * si, sj, hipMemcpyDeviceToHost));
+  auto copy = [&](long long src, long long ld, int rows, int cols, double *dst) -> int {
+    if (h->prec == SNU_FP32) {
+      std::vector<float> tmp((size_t)rows * cols);
+      HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * rows, h->v32.arena + P.leaf_total + src, sizeof(float) * ld,
+                         sizeof(float) * rows, cols, hipMemcpyDeviceToHost));
+      for (int c = 0; c < cols; c++)
+        for (int r = 0; r < rows; r++) dst[r + (size_t)c * si] = tmp[r + (size_t)c * rows];
+    } else {
+      HIPCHK(hipMemcpy2D(dst, sizeof(double) * si, h->v64.arena + P.leaf_total + src, sizeof(double) * ld,
+                         sizeof(double) * rows, cols, hipMemcpyDeviceToHost));
+    }
+    return SNU_OK;
+  };
+  int rc;
+  if (i == j) return copy(P.offR[i], si, si, si, out);
+  if (i > j) {  // columns of the U panel of i: runs of stored columns
+    const int o = P.anc_off(i, j);
+    for (int c = 0; c < sj;) {
+      if (P.umap[i][o + c] < 0) { c++; continue; }
+      int e = c + 1;
+      while (e < sj && P.umap[i][o + e] == P.umap[i][o + c] + (e - c)) e++;
+      if ((rc = copy(P.sep_index(i, 0, j, c), si, si, e - c, out + (size_t)c * si))) return rc;
+      c = e;
+    }
+    return SNU_OK;
+  }
+  // rows of the L panel of j: runs of stored rows
+  const int o = P.anc_off(j, i);
+  for (int r = 0; r < si;) {
+    if (P.lmap[j][o + r] < 0) { r++; continue; }
+    int e = r + 1;
+    while (e < si && P.lmap[j][o + e] == P.lmap[j][o + r] + (e - r)) e++;
+    if ((rc = copy(P.sep_index(i, r, j, 0), P.nL[j], e - r, sj, out + r))) return rc;
+    r = e;
   }
   return SNU_OK;
 }

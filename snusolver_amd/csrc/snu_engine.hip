@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -459,20 +460,16 @@ void build_dense_plan(snumat *h) {
     }
     // panels: Lp <- Lp U^-1 per W-column strip of the result (U^-1 is upper
     // triangular: strip [n0, n0 + W) needs rows [0, n0 + W) of it), Up <-
-    // L^-1 Up per W-row strip (rows [0, m0 + W) of Up), over the
-    // structurally nonzero row / column runs, into scratch; then copied back.
+    // L^-1 Up per W-row strip (rows [0, m0 + W) of Up), over the stored
+    // (tile-compressed) rows / columns, into scratch; then copied back.
     // W = 128 (128 x 128 tiles) when the autotuner prefers that tile for the
     // level's panels and they hold enough such tiles to fill the device
     {
       long long big = 0;
       for (int i = lo; i <= hi; i++) {
-        const int s = P.block_size[i], a = P.anc_sum[i];
-        for (auto rr : nz_runs(a, [&](int x0, int x1) { return P.anyL(i, x0, x1); }))
-          if (tile_for(prec, rr.second - rr.first, s) == 128)
-            big += (long long)((rr.second - rr.first + 127) / 128) * ((s + 127) / 128);
-        for (auto cr : nz_runs(a, [&](int x0, int x1) { return P.anyU(i, x0, x1); }))
-          if (tile_for(prec, s, cr.second - cr.first) == 128)
-            big += (long long)((s + 127) / 128) * ((cr.second - cr.first + 127) / 128);
+        const int s = P.block_size[i], nl = P.nL[i], nu = P.nU[i];
+        if (nl && tile_for(prec, nl, s) == 128) big += (long long)((nl + 127) / 128) * ((s + 127) / 128);
+        if (nu && tile_for(prec, s, nu) == 128) big += (long long)((s + 127) / 128) * ((nu + 127) / 128);
       }
       const int W = big >= kMin128Tiles ? 128 : 64;
       Launch pm{W == 128 ? 10 : 1, (int)h_gp.size(), 0, 0, 0, fkey(height, 60000000LL)};
@@ -481,24 +478,22 @@ void build_dense_plan(snumat *h) {
       double cfl = 0;
       long long toff = 0;
       for (int i = lo; i <= hi; i++) {
-        const int s = P.block_size[i], a = P.anc_sum[i];
-        for (auto rr : nz_runs(a, [&](int x0, int x1) { return P.anyL(i, x0, x1); })) {
-          const int m = rr.second - rr.first;
+        const int s = P.block_size[i], m = P.nL[i], nc = P.nU[i];
+        if (m && s) {
           T *dst = VV.tmp + toff;
           toff += (long long)m * s;
-          T *src = Lp(i) + rr.first;
+          T *src = Lp(i);
           for (int n0 = 0; n0 < s; n0 += W) {
             const int nt = std::min(W, s - n0), kk = n0 + nt;
             pm.np += add_gemm(h_gp, h_gs, pm.tiles, dst + (long long)n0 * m, m, m, nt,
-                              {GemmSeg{src, Ui(i) + (long long)n0 * s, a, s, kk}}, pm.flops, W);
+                              {GemmSeg{src, Ui(i) + (long long)n0 * s, m, s, kk}}, pm.flops, W);
           }
-          cl.np += add_gemm(cp, h_gs, cl.tiles, src, a, m, s, {GemmSeg{dst, dst, m, m, 1}}, cfl);
+          cl.np += add_gemm(cp, h_gs, cl.tiles, src, m, m, s, {GemmSeg{dst, dst, m, m, 1}}, cfl);
         }
-        for (auto cr : nz_runs(a, [&](int x0, int x1) { return P.anyU(i, x0, x1); })) {
-          const int nc = cr.second - cr.first;
+        if (nc && s) {
           T *dst = VV.tmp + toff;
           toff += (long long)s * nc;
-          T *src = R(i) + (long long)(s + cr.first) * s;
+          T *src = R(i) + (long long)s * s;
           for (int m0 = 0; m0 < s; m0 += W) {
             const int mt = std::min(W, s - m0), kk = m0 + mt;
             pm.np += add_gemm(h_gp, h_gs, pm.tiles, dst + m0, s, mt, nc,
@@ -515,53 +510,80 @@ void build_dense_plan(snumat *h) {
       }
     }
     // Schur updates of every ancestor, K-concatenated over the level's
-    // separators below it (one grouped launch per level).  Emitted per 64 x 64
-    // target tile, with only the contributors whose L-panel rows and U-panel
-    // columns under that tile hold structural nonzeros (skipped products are
-    // exactly zero).
+    // separators below it (one grouped launch per level).  Emitted per
+    // target tile (64 rows / columns aligned to the first index of the block
+    // they belong to, the tiles of the compressed storage), with only the
+    // contributors that store both the L-panel rows and the U-panel columns
+    // of that tile (the other products are exactly zero).
     GemmBatch<T> sc;
     for (int p = 1; p < lo; p++) {
       int lp = 31 - __builtin_clz(p);
       int first = p << (l - lp), last = ((p + 1) << (l - lp)) - 1;
-      int sp = P.block_size[p], ap = P.anc_sum[p];
+      const int sp = P.block_size[p];
+      // the tiles of p itself, then the stored tiles of p's ancestors (in
+      // ancestor-space order; x: index in p's ancestor space), as
+      // {position in the target, width, x}; x = -1 - t for p's own tile t
+      std::vector<std::array<int, 3>> own, anL, anU;
+      for (int t = 0; t < sp; t += 64) own.push_back({t, std::min(64, sp - t), -1 - t});
+      for (int q = p / 2; q; q /= 2) {
+        const int o = P.anc_off(p, q), sq = P.block_size[q];
+        for (int t = 0; t < sq; t += 64) {
+          const int w = std::min(64, sq - t);
+          if (P.lmap[p][o + t] >= 0) anL.push_back({P.lmap[p][o + t], w, o + t});
+          if (P.umap[p][o + t] >= 0) anU.push_back({sp + P.umap[p][o + t], w, o + t});
+        }
+      }
       for (int pass = 0; pass < 2; pass++) {  // 0: R_p = [D_p | Up_p], 1: Lp_p
-        const int M = pass == 0 ? sp : ap, N = pass == 0 ? sp + ap : sp;
-        const int tm = (M + 63) / 64, tn = (N + 63) / 64;
-        // contributor set of every 64 x 64 target tile
+        std::vector<std::array<int, 3>> rows = pass == 0 ? own : anL, cols = own;
+        if (pass == 0) cols.insert(cols.end(), anU.begin(), anU.end());
+        const int tm = (int)rows.size(), tn = (int)cols.size();
+        if (!tm || !tn) continue;
+        T *Cb = pass == 0 ? R(p) : Lp(p);
+        const int ldc = pass == 0 ? sp : P.nL[p];
+        // index of a tile in contributor i's ancestor space
+        auto ix = [&](int i, const std::array<int, 3> &t) {
+          return P.anc_off(i, p) + (t[2] < 0 ? -1 - t[2] : sp + t[2]);
+        };
+        // contributor set of every target tile
         std::vector<std::vector<int>> who((size_t)tm * tn);
         for (int a = 0; a < tm; a++)
-          for (int b = 0; b < tn; b++) {
-            const int m0 = 64 * a, n0 = 64 * b, mt = std::min(64, M - m0), nt = std::min(64, N - n0);
-            for (int i = first; i <= last; i++) {
-              const int off = P.anc_off(i, p);
-              const int r0 = off + (pass == 0 ? 0 : sp) + m0, c0 = off + n0;
-              if (P.anyL(i, r0, r0 + mt) && P.anyU(i, c0, c0 + nt)) who[(size_t)a * tn + b].push_back(i);
-            }
-          }
-        // one problem per 64 x 64 target tile with its contributor set, or
-        // per 128 x 128 block when its four tiles share one set (the tile
-        // kernel is then chosen per size bucket), emitted in strips of 8
-        // tile rows, two tile columns at a time: consecutive tiles (which
-        // one XCD runs together) share A rows (same tile row and
-        // contributor) and B columns (same tile column), so their operands
-        // meet in that XCD's L2
-        auto segs_of = [&](const std::vector<int> &w, int m0, int n0) {
+          for (int b = 0; b < tn; b++)
+            for (int i = first; i <= last; i++)
+              if (P.lmap[i][ix(i, rows[a])] >= 0 && P.umap[i][ix(i, cols[b])] >= 0)
+                who[(size_t)a * tn + b].push_back(i);
+        auto segs_of = [&](const std::vector<int> &w, int a, int b) {
           std::vector<GemmSeg> segs;
           for (int i : w) {
-            const int si = P.block_size[i], ai = P.anc_sum[i], off = P.anc_off(i, p);
-            const int r0 = off + (pass == 0 ? 0 : sp) + m0, c0 = off + n0;
-            segs.push_back(GemmSeg{Lp(i) + r0, R(i) + (long long)(si + c0) * si, ai, si, si});
+            const int si = P.block_size[i];
+            segs.push_back(GemmSeg{Lp(i) + P.lmap[i][ix(i, rows[a])],
+                                   R(i) + (long long)(si + P.umap[i][ix(i, cols[b])]) * si, P.nL[i], si, si});
           }
           return segs;
         };
         auto emit = [&](int a, int b, int ta, int tb) {
           const std::vector<int> &w = who[(size_t)a * tn + b];
-          const int m0 = 64 * a, n0 = 64 * b;
-          const int mt = std::min(64 * ta, M - m0), nt = std::min(64 * tb, N - n0);
-          T *C = pass == 0 ? R(p) + m0 + (long long)n0 * sp : Lp(p) + m0 + (long long)n0 * ap;
-          sc.add(C, pass == 0 ? sp : ap, mt, nt, segs_of(w, m0, n0));
+          int mt = 0, nt = 0;
+          for (int x = 0; x < ta; x++) mt += rows[a + x][1];
+          for (int y = 0; y < tb; y++) nt += cols[b + y][1];
+          sc.add(Cb + rows[a][0] + (long long)cols[b][0] * ldc, ldc, mt, nt, segs_of(w, a, b));
         };
         auto nonempty = [&](int a, int b) { return a < tm && b < tn && !who[(size_t)a * tn + b].empty(); };
+        // tiles a, a + 1 (rows) are consecutive in every contributor's L
+        // panel, b, b + 1 (columns) in its U panel
+        auto merge_ok = [&](int a, int b) {
+          for (int i : who[(size_t)a * tn + b])
+            if (P.lmap[i][ix(i, rows[a + 1])] != P.lmap[i][ix(i, rows[a])] + rows[a][1] ||
+                P.umap[i][ix(i, cols[b + 1])] != P.umap[i][ix(i, cols[b])] + cols[b][1])
+              return false;
+          return true;
+        };
+        // one problem per target tile with its contributor set, or per 2 x 2
+        // tiles when the four share one set and are contiguous in every
+        // contributor's panels (the tile kernel is then chosen per size
+        // bucket), emitted in strips of 8 tile rows, two tile columns at a
+        // time: consecutive tiles (which one XCD runs together) share A rows
+        // (same tile row and contributor) and B columns (same tile column),
+        // so their operands meet in that XCD's L2
         constexpr int kStrip = 8;
         for (int a0 = 0; a0 < tm; a0 += kStrip)
           for (int b = 0; b < tn; b += 2)
@@ -569,7 +591,7 @@ void build_dense_plan(snumat *h) {
               const std::vector<int> &w = who[(size_t)a * tn + b];
               if (nonempty(a, b) && nonempty(a + 1, b) && nonempty(a, b + 1) && nonempty(a + 1, b + 1) &&
                   who[(size_t)(a + 1) * tn + b] == w && who[(size_t)a * tn + b + 1] == w &&
-                  who[(size_t)(a + 1) * tn + b + 1] == w) {
+                  who[(size_t)(a + 1) * tn + b + 1] == w && merge_ok(a, b)) {
                 emit(a, b, 2, 2);
                 continue;
               }
@@ -593,13 +615,7 @@ long long inversion_tmp_size(const Plan &P) {
     int smax = 0;
     for (int i = lo; i <= hi; i++) smax = std::max(smax, P.block_size[i]);
     long long panels = 0;
-    for (int i = lo; i <= hi; i++) {
-      const int s = P.block_size[i], a = P.anc_sum[i];
-      for (auto rr : nz_runs(a, [&](int x0, int x1) { return P.anyL(i, x0, x1); }))
-        panels += (long long)(rr.second - rr.first) * s;
-      for (auto cr : nz_runs(a, [&](int x0, int x1) { return P.anyU(i, x0, x1); }))
-        panels += (long long)(cr.second - cr.first) * s;
-    }
+    for (int i = lo; i <= hi; i++) panels += (long long)(P.nL[i] + P.nU[i]) * P.block_size[i];
     best = std::max(best, panels);
     for (int hh = 64; hh < smax; hh *= 2) {
       long long t = 0;
@@ -654,9 +670,9 @@ void build_solve_plan(const snumat *h, T *bp, T *ys, int nrhs, GemvPlan<T> &S) {
         int mt = std::min(64, sp - m0);
         std::vector<GemvSeg> segs;
         for (int i = first; i <= last; i++) {
-          int r0 = P.anc_off(i, p) + m0;
-          if (!P.anyL(i, r0, r0 + mt)) continue;
-          segs.push_back(GemvSeg{Lp(i) + r0, yseg(i), P.anc_sum[i], n, P.block_size[i]});
+          const int r = P.lmap[i][P.anc_off(i, p) + m0];  // the tile's stored rows (or -1)
+          if (r < 0) continue;
+          segs.push_back(GemvSeg{Lp(i) + r, yseg(i), P.nL[i], n, P.block_size[i]});
         }
         g.add(S.vs, bseg(p) + m0, n, mt, 0, nrhs, segs);
       }
@@ -672,8 +688,10 @@ void build_solve_plan(const snumat *h, T *bp, T *ys, int nrhs, GemvPlan<T> &S) {
       std::vector<GemvSeg> segs;
       for (int a = i / 2; a >= 1; a /= 2) {
         int off = P.anc_off(i, a);
-        for (auto cr : nz_runs(P.block_size[a], [&](int x0, int x1) { return P.anyU(i, off + x0, off + x1); }))
-          segs.push_back(GemvSeg{R(i) + (long long)(s + off + cr.first) * s, bseg(a) + cr.first, s, n,
+        // stored column tiles of a, merged into runs (consecutive stored
+        // tiles of one ancestor are consecutive columns of the panel)
+        for (auto cr : nz_runs(P.block_size[a], [&](int x0, int x1) { return P.umap[i][off + x0] >= 0; }))
+          segs.push_back(GemvSeg{R(i) + (long long)(s + P.umap[i][off + cr.first]) * s, bseg(a) + cr.first, s, n,
                                  cr.second - cr.first});
       }
       g.add(S.vs, yseg(i), n, s, 0, nrhs, segs);
@@ -1302,18 +1320,30 @@ int upload(snumat *h) {
       (rc = dupload(h, &wd.tile_ptr, W.tile_ptr)) || (rc = dupload(h, &wd.seg_leaf, W.seg_leaf)) ||
       (rc = dupload(h, &wd.seg_ri, W.seg_ri)))
     return rc;
-  long long *d_offR, *d_offL;
-  int *d_bsize, *d_anc_sum, *d_aoff;
+  long long *d_offR, *d_offL, *d_mapoff;
+  int *d_bsize, *d_nL, *d_lmap, *d_umap, *d_aoff;
   std::vector<long long> offR(P.offR.begin(), P.offR.end()), offL(P.offL.begin(), P.offL.end());
+  std::vector<long long> mapoff(P.num_block + 1, 0);
+  std::vector<int> lmap, umap;
+  for (int p = 1; p < np; p++) {
+    mapoff[p] = (long long)lmap.size();
+    lmap.insert(lmap.end(), P.lmap[p].begin(), P.lmap[p].end());
+    umap.insert(umap.end(), P.umap[p].begin(), P.umap[p].end());
+  }
+  if (lmap.empty()) lmap.push_back(-1), umap.push_back(-1);
   if ((rc = dupload(h, &d_offR, offR)) || (rc = dupload(h, &d_offL, offL)) ||
-      (rc = dupload(h, &d_bsize, P.block_size)) || (rc = dupload(h, &d_anc_sum, P.anc_sum)) ||
+      (rc = dupload(h, &d_mapoff, mapoff)) || (rc = dupload(h, &d_bsize, P.block_size)) ||
+      (rc = dupload(h, &d_nL, P.nL)) || (rc = dupload(h, &d_lmap, lmap)) || (rc = dupload(h, &d_umap, umap)) ||
       (rc = dupload(h, &d_aoff, aoff)))
     return rc;
   DenseDev dd;
   dd.offR = d_offR;
   dd.offL = d_offL;
+  dd.mapoff = d_mapoff;
   dd.bsize = d_bsize;
-  dd.anc_sum = d_anc_sum;
+  dd.nL = d_nL;
+  dd.lmap = d_lmap;
+  dd.umap = d_umap;
   dd.aoff = d_aoff;
   dd.nk = P.nk;
   if ((rc = dupload(h, &h->d_dd, std::vector<DenseDev>{dd}))) return rc;

@@ -55,9 +55,15 @@ struct Plan {
   std::vector<int> mstart, msize; // merge region of each block
   std::vector<LeafPlan> leaves;   // by rank
   // dense single-copy separator storage:
-  //   R_p = [D_p | Up_p], s_p x (s_p + A_p), ld s_p (ancestors bottom-up)
-  //   Lp_p, A_p x s_p, ld A_p
+  //   R_p = [D_p | Up_p], s_p x (s_p + nU_p), ld s_p (ancestors bottom-up)
+  //   Lp_p, nL_p x s_p, ld nL_p
+  // with the panels tile-compressed: of each ancestor q, the 64-row (Lp) /
+  // 64-column (Up) tiles aligned to q's first index that hold a structural
+  // nonzero; lmap_p / umap_p give the stored row / column of each ancestor-
+  // space index (ancestors bottom-up, A_p entries; -1: not stored)
   std::vector<int> anc_sum;          // A_p
+  std::vector<int> nL, nU;           // stored rows of Lp_p, stored columns of Up_p
+  std::vector<std::vector<int>> lmap, umap;
   std::vector<int64_t> offR, offL;   // per separator, in the dense arena
   int64_t dense_total = 0;
   std::vector<int64_t> leaf_off;     // per rank, in the leaf value arena
@@ -75,7 +81,8 @@ struct Plan {
     for (int a = p / 2; a > q; a /= 2) s += block_size[a];
     return s;
   }
-  // value-arena index of separator entry (bi, lr) x (bj, lc)
+  // value-arena index of separator entry (bi, lr) x (bj, lc), -1 when it
+  // lies in a panel tile that is not stored (structurally zero)
   int64_t sep_index(int bi, int lr, int bj, int lc) const;
   // Structural nonzeros of the dense separator panels: for separator p,
   // nzL[p][r] / nzU[p][c] flag the rows of Lp_p / columns of Up_p (ancestors

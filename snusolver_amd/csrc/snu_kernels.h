@@ -9,13 +9,16 @@
 
 namespace snu {
 
-// Single-copy separator storage:
-//   R_p = [D_p | Up_p] : s_p x (s_p + A_p), ld s_p, ancestors bottom-up
-//   Lp_p              : A_p x s_p, ld A_p
+// Single-copy separator storage, panels tile-compressed:
+//   R_p = [D_p | Up_p] : s_p x (s_p + nU_p), ld s_p
+//   Lp_p              : nL_p x s_p, ld nL_p
+// lmap / umap (from mapoff[p], A_p entries, ancestors bottom-up): stored row
+// of Lp_p / stored column of Up_p of each ancestor index, -1 if not stored.
 struct DenseDev {
-  const long long *offR, *offL;  // per heap block
-  const int *bsize, *anc_sum;    // per heap block
-  const int *aoff;               // (block * nk + distance) -> ancestor offset
+  const long long *offR, *offL, *mapoff;  // per heap block
+  const int *bsize, *nL;                  // per heap block
+  const int *lmap, *umap;
+  const int *aoff;                        // (block * nk + distance) -> ancestor offset
   int nk;
 };
 

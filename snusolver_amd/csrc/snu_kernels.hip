@@ -77,18 +77,19 @@ void launch_fill(T *p, long long n, hipStream_t st) {
 }
 
 
-// Slot of separator entry (bi, lr) x (bj, lc) in the dense arena.
+// Slot of separator entry (bi, lr) x (bj, lc) in the dense arena, -1 when
+// it lies in a panel tile that is not stored.
 __device__ __forceinline__ long long sep_index(const DenseDev &d, int bi,
                                                int lr, int bj, int lc) {
   if (bi == bj) return d.offR[bi] + lr + (long long)lc * d.bsize[bi];
   if (bi > bj) {  // row block deeper: U panel of bi
     int dist = (31 - __clz(bi)) - (31 - __clz(bj));
-    int off = d.aoff[bi * d.nk + dist];
-    return d.offR[bi] + lr + (long long)(d.bsize[bi] + off + lc) * d.bsize[bi];
+    int c = d.umap[d.mapoff[bi] + d.aoff[bi * d.nk + dist] + lc];
+    return c < 0 ? -1 : d.offR[bi] + lr + (long long)(d.bsize[bi] + c) * d.bsize[bi];
   }
   int dist = (31 - __clz(bj)) - (31 - __clz(bi));
-  int off = d.aoff[bj * d.nk + dist];
-  return d.offL[bj] + off + lr + (long long)lc * d.anc_sum[bj];
+  int r = d.lmap[d.mapoff[bj] + d.aoff[bj * d.nk + dist] + lr];
+  return r < 0 ? -1 : d.offL[bj] + r + (long long)lc * d.nL[bj];
 }
 
 // One workgroup per assembly tile: 64 consecutive rows of one separator
@@ -128,21 +129,23 @@ __global__ __launch_bounds__(64 * kAsmWaves) void k_assemble_schur(
       // of a lane are read before any is written (one round trip)
       const int nc = min(64, L.nbc - c0);
       constexpr int NJ = 64 / kAsmWaves;
+      // (every structurally nonzero entry of S lies in a stored tile; an
+      // index outside them, which the analysis rules out, is skipped)
       T v[NJ], old[NJ];
       long long idx[NJ];
 #pragma unroll
       for (int j = 0; j < NJ; j++) {
         const int cc = wave + kAsmWaves * j;
         v[j] = cc < nc ? st[lane][cc] : T(0);
-        idx[j] = 0;
+        idx[j] = -1;
         if (v[j] != T(0)) {
           idx[j] = sep_index(dd, bi, row0 + lane, L.bcol_blk[c0 + cc], L.bcol_loc[c0 + cc]);
-          old[j] = dense[idx[j]];
+          if (idx[j] >= 0) old[j] = dense[idx[j]];
         }
       }
 #pragma unroll
       for (int j = 0; j < NJ; j++)
-        if (v[j] != T(0)) dense[idx[j]] = old[j] + v[j];
+        if (idx[j] >= 0) dense[idx[j]] = old[j] + v[j];
       __syncthreads();
     }
   }

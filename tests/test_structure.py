@@ -124,15 +124,20 @@ def test_dense_layout_accounting(S):
     A = S.gen_grid(16, 16, 16, 7, 0, 6.1, 0)
     M = S.SnuMat(A, 16, grid=(16, 16, 16), device=-1)
     bs, sz = M.blocks()
-    total = 0
-    for p in range(1, 16):  # sum over separators of s^2 + 2 s A
+    total = diag = 0
+    for p in range(1, 16):  # uncompressed: sum over separators of s^2 + 2 s A
         a = 0
         q = p // 2
         while q:
             a += sz[q - 1]
             q //= 2
         total += sz[p - 1] ** 2 + 2 * sz[p - 1] * a
+        diag += sz[p - 1] ** 2
     info = M.info()
-    assert info["dense_entries"] == total
+    # the panels keep only the 64-wide tiles of each ancestor holding a
+    # structural nonzero: between the diagonal blocks alone and the full
+    # panels, and strictly below the latter on a 3-D grid (far separators
+    # touch few of the ancestors' tiles)
+    assert diag < info["dense_entries"] < total
     assert info["leaf_lu_nnz"] == sum(M.leaf_sizes(r)[3] for r in range(16))
     M.close()
