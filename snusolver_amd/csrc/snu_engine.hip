@@ -291,6 +291,9 @@ std::vector<std::pair<int, int>> nz_runs(int len, F any) {
   return out;
 }
 
+// Outer panel width of the blocked LU of the separators' diagonal blocks.
+constexpr int kPanel = 512;
+
 // Launch keys: tree height (1 = parents of the leaves) major, then the step
 // inside the level.  Matrices of different depths align at equal height.
 inline long long fkey(int height, long long step) { return (long long)height * 100000000LL + step; }
@@ -371,16 +374,25 @@ void build_dense_plan(snumat *h) {
                           {GemmSeg{L11i, X, s, s, nb}}, tr.flops);
       }
       if (tr.np) out.push_back(tr);
-      // trailing update of D
+      // trailing update of D, outer-blocked by kPanel columns: inside a
+      // panel [K0, K1) a step updates only the L-shaped rest of the panel
+      // (its columns below the step, its rows right of the panel); the last
+      // step of the panel applies the panel's K1 - K0 rank-64 updates to the
+      // trailing block D[K1:, K1:] as one product with K = K1 - K0 (a K = 64
+      // update reads and writes the whole trailing block per step)
       GemmBatch<T> tu;
+      const int K0 = k - k % kPanel;
       for (int i = lo; i <= hi; i++) {
-        int s = P.block_size[i];
+        const int s = P.block_size[i];
         if (s <= k) continue;
-        int nb = std::min(64, s - k);
-        T *Ccol = R(i) + k + (long long)(k + nb) * s;  // U12 rows
-        const T *L21 = R(i) + (k + nb) + (long long)k * s;
-        tu.add(R(i) + (k + nb) + (long long)(k + nb) * s, s, s - k - nb, s - k - nb,
-               {GemmSeg{L21, Ccol, s, s, nb}});
+        const int nb = std::min(64, s - k), k2 = k + nb, K1 = std::min(s, K0 + kPanel);
+        auto D = [&](int r, int c) { return R(i) + r + (long long)c * s; };
+        if (k2 < K1) {
+          tu.add(D(k2, k2), s, s - k2, K1 - k2, {GemmSeg{D(k2, k), D(k, k2), s, s, nb}});
+          tu.add(D(k2, K1), s, K1 - k2, s - K1, {GemmSeg{D(k2, k), D(k, K1), s, s, nb}});
+        } else {
+          tu.add(D(K1, K1), s, s - K1, s - K1, {GemmSeg{D(K1, K0), D(K0, K1), s, s, K1 - K0}});
+        }
       }
       tu.flush(h_gp, h_gs, out, fkey(height, ks + 3));
     }
@@ -1040,8 +1052,17 @@ int build_front_plan(snumat *h) {
         tr.np += add_gemm(DP.gp, DP.gs, tr.tiles, X1, f, m, nb, {GemmSeg{X1, Ui, f, 64, nb}}, tr.flops);
         T *X = F + k + (long long)(k + nb) * f;
         tr.np += add_gemm(DP.gp, DP.gs, tr.tiles, X, f, nb, m, {GemmSeg{Li, X, 64, f, nb}}, tr.flops);
-        // trailing update of the whole remaining front
-        tu.add(F + (k + nb) + (long long)(k + nb) * f, f, m, m, {GemmSeg{X1, X, f, f, nb}});
+        // trailing update of the remaining front, outer-blocked by kPanel
+        // pivots as for the separators (the last step of a panel applies
+        // its rank-64 updates to F[K1:, K1:] at once)
+        const int k2 = k + nb, K0 = k - k % kPanel, K1 = std::min(D.nk, K0 + kPanel);
+        auto at = [&](int r, int c) { return F + r + (long long)c * f; };
+        if (k2 < K1) {
+          tu.add(at(k2, k2), f, f - k2, K1 - k2, {GemmSeg{X1, X, f, f, nb}});
+          tu.add(at(k2, K1), f, K1 - k2, f - K1, {GemmSeg{X1, at(k, K1), f, f, nb}});
+        } else {
+          tu.add(at(K1, K1), f, f - K1, f - K1, {GemmSeg{at(K1, K0), at(K0, K1), f, f, K1 - K0}});
+        }
       }
       if (d.np) DP.launches.push_back(d);
       if (tr.np) DP.launches.push_back(tr);

@@ -213,8 +213,17 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
   constexpr bool F64 = std::is_same<T, double>::value;
   constexpr int BM = 32 * BMB, BN = 32 * BNB;
   constexpr int NA = BM * KB / 256, NB = BN * KB / 256;
-  __shared__ T As[2][KB][BM + 1];
-  __shared__ T Bs[2][KB][BN + 1];
+  // LDS tiles k-contiguous per row of A / column of B.  A lane reads GR
+  // consecutive k of its row per access: FP32 128 x 128 tile 4 (16 bytes),
+  // FP32 64 x 64 2, FP64 64 x 64 2 (16 bytes), FP64 128 x 128 1 (more
+  // operands in registers would cost the second wave per SIMD).  Rows are
+  // padded by GR elements: the row stride is then an odd multiple of the
+  // access width modulo 32 banks, so the accesses of consecutive rows that
+  // the LDS serves together (8 of 16 bytes, 16 of 8) hit disjoint banks.
+  constexpr int GR = F64 ? (BMB * BNB >= 16 ? 1 : 2) : (BMB * BNB >= 16 ? 4 : 2);
+  constexpr int KP = KB + GR;
+  __shared__ __align__(16) T As[2][BM][KP];
+  __shared__ __align__(16) T Bs[2][BN][KP];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   int lo = 0, hi = nprob - 1;
   if (nprob == (int)gridDim.x) lo = hi = tile;  // one tile per problem
@@ -251,24 +260,24 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
   // operands, so they stay in flight across the MFMA block: a clamped row of
   // A (column of B) only feeds a row (column) of the tile that is never
   // stored; the K tail is zeroed in LDS after the store.  Element i of a
-  // thread's A chunk is row m0 + tid % BM, column kk0 + tid / BM + i (256 /
-  // BM); of its B chunk row kk0 + tid % KB, column n0 + tid / KB + i (256 /
-  // KB): on full chunks (no K tail) and full tiles (no N edge), the uniform
-  // case, the addresses are one base plus multiples of one stride.
+  // thread's A chunk is row m0 + tid % BM, column kk0 + NA (tid / BM) + i
+  // (NA consecutive k: one vector store into LDS); of its B chunk row
+  // kk0 + tid % KB, column n0 + tid / KB + i (256 / KB): on full chunks (no
+  // K tail) and full tiles (no N edge), the uniform case, the addresses are
+  // one base plus multiples of one stride.
   static_assert(256 % BM == 0 && 256 % KB == 0, "thread-to-element map");
-  constexpr int ASTEP = 256 / BM, BSTEP = 256 / KB;
+  constexpr int BSTEP = 256 / KB;
   const int am = tid % BM, ak = tid / BM, bk = tid % KB, bn = tid / KB;
   auto load_into = [&](int sgi, int kk0, T (&xa)[NA], T (&xb)[NB]) {
     const GemmSegT<T> S = segs[P.seg0 + sgi];
     const auto pa = gptr(S.A) + min(m0 + am, P.M - 1);
     if (kk0 + KB <= S.K) {
-      const auto p = pa + (long long)(kk0 + ak) * S.lda;
-      const long long st = (long long)ASTEP * S.lda;
+      const auto p = pa + (long long)(kk0 + ak * NA) * S.lda;
 #pragma unroll
-      for (int i = 0; i < NA; i++) xa[i] = p[i * st];
+      for (int i = 0; i < NA; i++) xa[i] = p[i * S.lda];
     } else {
 #pragma unroll
-      for (int i = 0; i < NA; i++) xa[i] = pa[(long long)min(kk0 + ak + i * ASTEP, S.K - 1) * S.lda];
+      for (int i = 0; i < NA; i++) xa[i] = pa[(long long)min(kk0 + ak * NA + i, S.K - 1) * S.lda];
     }
     const auto pb = gptr(S.B) + min(kk0 + bk, S.K - 1);
     if (n0 + BN <= P.N) {
@@ -282,44 +291,25 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
     }
   };
   auto load = [&](int sgi, int kk0) { load_into(sgi, kk0, ra, rb); };
+  // vectors of GR consecutive k
+  typedef T vec_t __attribute__((ext_vector_type(GR)));
+  static_assert(NA % GR == 0 && (KB / 4) % GR == 0, "vector LDS accesses");
   auto store_from = [&](int buf, int kvalid, const T (&xa)[NA], const T (&xb)[NB]) {
 #pragma unroll
-    for (int i = 0; i < NA; i++) {
-      int e = tid + 256 * i;
-      As[buf][e / BM][e % BM] = xa[i];
+    for (int v = 0; v < NA / GR; v++) {
+      vec_t w;
+#pragma unroll
+      for (int j = 0; j < GR; j++) w[j] = xa[v * GR + j];
+      *(vec_t *)&As[buf][am][ak * NA + v * GR] = w;
     }
 #pragma unroll
-    for (int i = 0; i < NB; i++) {
-      int e = tid + 256 * i;
-      Bs[buf][e % KB][e / KB] = xb[i];
-    }
-    if (kvalid < KB) {  // uniform: last chunk of a segment whose K % KB != 0
+    for (int i = 0; i < NB; i++) Bs[buf][bn + i * BSTEP][bk] = xb[i];
+    if (kvalid < KB && bk >= kvalid) {  // last chunk of a segment whose K % KB != 0
 #pragma unroll
-      for (int i = 0; i < NB; i++) {
-        int e = tid + 256 * i;
-        if (e % KB >= kvalid) Bs[buf][e % KB][e / KB] = T(0);
-      }
+      for (int i = 0; i < NB; i++) Bs[buf][bn + i * BSTEP][bk] = T(0);
     }
   };
-  auto store = [&](int buf, int kvalid) {
-#pragma unroll
-    for (int i = 0; i < NA; i++) {
-      int e = tid + 256 * i;
-      As[buf][e / BM][e % BM] = ra[i];
-    }
-#pragma unroll
-    for (int i = 0; i < NB; i++) {
-      int e = tid + 256 * i;
-      Bs[buf][e % KB][e / KB] = rb[i];
-    }
-    if (kvalid < KB) {  // uniform: last chunk of a segment whose K % KB != 0
-#pragma unroll
-      for (int i = 0; i < NB; i++) {
-        int e = tid + 256 * i;
-        if (e % KB >= kvalid) Bs[buf][e % KB][e / KB] = T(0);
-      }
-    }
-  };
+  auto store = [&](int buf, int kvalid) { store_from(buf, kvalid, ra, rb); };
   auto advance = [&](int &sgi, int &kk0) {
     kk0 += KB;
     while (sgi < P.nseg && kk0 >= segs[P.seg0 + sgi].K) {
@@ -327,44 +317,37 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
       kk0 = 0;
     }
   };
-  // one K chunk from LDS buffer buf: all LDS operands first (one wait),
-  // then the MFMA block
-  // small wave tiles: every LDS operand of the chunk first (one wait);
-  // large ones (16 blocks per k step): operands per k step, which keeps the
-  // register count at two workgroups per CU
-  constexpr int KQL = BMB * BNB >= 16 ? 1 : KB / 4;
+  // One K chunk from LDS buffer buf.  The chunk's k indices are permuted
+  // between the MFMA steps (A and B alike, so the sum is the same): at step
+  // kq, lanes 16 k' .. 16 k' + 15 hold k = (KB / 4) k' + kq, so a lane's
+  // operands for GR consecutive steps are GR consecutive values of its row
+  // of A (column of B): one LDS read per row block per GR steps.
+  constexpr int KQ = KB / 4;
   auto compute = [&](int buf) {
-      T bvk[KB / 4][BNB], avk[KB / 4][BMB];
 #pragma unroll
-      for (int kq = 0; kq < KQL; kq++) {
+    for (int g0 = 0; g0 < KQ; g0 += GR) {
+      vec_t bv[BNB], av[BMB];
 #pragma unroll
-        for (int b = 0; b < BNB; b++) bvk[kq][b] = Bs[buf][4 * kq + lk][wn + b * 16 + l16];
+      for (int b = 0; b < BNB; b++) bv[b] = *(const vec_t *)&Bs[buf][wn + b * 16 + l16][KQ * lk + g0];
 #pragma unroll
-        for (int a = 0; a < BMB; a++) avk[kq][a] = As[buf][4 * kq + lk][wm + a * 16 + l16];
-      }
+      for (int a = 0; a < BMB; a++) av[a] = *(const vec_t *)&As[buf][wm + a * 16 + l16][KQ * lk + g0];
 #pragma unroll
-      for (int kq = 0; kq < KB / 4; kq++) {
-        if (kq >= KQL) {
-#pragma unroll
-          for (int b = 0; b < BNB; b++) bvk[kq][b] = Bs[buf][4 * kq + lk][wn + b * 16 + l16];
-#pragma unroll
-          for (int a = 0; a < BMB; a++) avk[kq][a] = As[buf][4 * kq + lk][wm + a * 16 + l16];
-        }
-        const T *bv = bvk[kq], *av = avk[kq];
+      for (int j = 0; j < GR; j++) {
         if constexpr (F64) {
 #pragma unroll
           for (int a = 0; a < BMB; a++)
 #pragma unroll
             for (int b = 0; b < BNB; b++)
-              accd[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b], av[a], accd[a][b], 0, 0, 0);
+              accd[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b][j], av[a][j], accd[a][b], 0, 0, 0);
         } else {
 #pragma unroll
           for (int a = 0; a < BMB; a++)
 #pragma unroll
             for (int b = 0; b < BNB; b++)
-              accf[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[b], av[a], accf[a][b], 0, 0, 0);
+              accf[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[b][j], av[a][j], accf[a][b], 0, 0, 0);
         }
       }
+    }
   };
   // element (a, b, r) of this lane's accumulators: its place in C and value
   auto elem = [&](int a, int b, int r, int &m, int &n, T &v) __attribute__((always_inline)) {
@@ -514,20 +497,28 @@ void launch_copy2d(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs
   if (total_tiles <= 0 || nprob <= 0) return;
   hipLaunchKernelGGL(k_copy2d<T>, dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
 }
+// K chunk of the production launches: 32 in FP32 (the 128 x 128 tile
+// 110 -> 122 TF/s at 4096^3, the 64 x 64 tile even with 16), 16 in FP64
+// (32 costs the FP64 tiles their second workgroup per CU)
+template <typename T>
+constexpr int gemm_kb() {
+  return sizeof(T) == 4 ? 32 : 16;
+}
 template <typename T>
 void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                  int total_tiles, int mode, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
   // loads one chunk ahead (two ahead measured slower in both precisions
   // once the accumulators stay in VGPRs)
+  constexpr int KB = gemm_kb<T>();
   if (mode == 1)
-    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+    hipLaunchKernelGGL((k_gemm<T, 1, 2, 2, KB, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
                        segs);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+    hipLaunchKernelGGL((k_gemm<T, 2, 2, 2, KB, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
                        segs);
   else
-    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+    hipLaunchKernelGGL((k_gemm<T, 0, 2, 2, KB, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
                        segs);
 }
 // 128 x 128 tiles, mode 0 (C -= AB), 1 (C = AB) or 2 (C = -AB); loads one
@@ -537,14 +528,15 @@ template <typename T>
 void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                     int total_tiles, int mode, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
+  constexpr int KB = gemm_kb<T>();
   if (mode == 1)
-    hipLaunchKernelGGL((k_gemm<T, 1, 4, 4, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+    hipLaunchKernelGGL((k_gemm<T, 1, 4, 4, KB, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
                        segs);
   else if (mode == 2)
-    hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+    hipLaunchKernelGGL((k_gemm<T, 2, 4, 4, KB, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
                        segs);
   else
-    hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, 16, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
+    hipLaunchKernelGGL((k_gemm<T, 0, 4, 4, KB, 1>), dim3(total_tiles), dim3(256), 0, st, probs, nprob,
                        segs);
 }
 // C -= AB variants for tuning: tile 64 or 128, K chunk 16 or 32, register
