@@ -27,7 +27,11 @@ Two further legs (skippable) report the other forms of the path:
      sides in 32..80; 8 2D 5-point grids, sides in 256..1024), a Poisson
      stream of solve requests with 2^k columns (k uniform in 0..6), 5% of
      requests new factorizations admitted under a 200 GB budget; p50 / p99
-     latency and solves/s at the stated rate.
+     latency and solves/s at the stated rate; then the same 16 matrices
+     under a budget of half their device bytes ("under_pressure"): the
+     server parks idle least-recently-used matrices and admits a parked one
+     again when a request for it arrives; end-to-end latency from arrival
+     (re-admission included), re-admissions and evictions.
 """
 import argparse
 import json
@@ -386,7 +390,62 @@ def serve_leg(S, np, rate, nreq, budget=200e9):
                 "solves_per_s": round(st["requests"] / wall, 2), "columns_per_s": round(cols / wall, 2),
                 "new_factorizations": len(adds), "evictions": int(st["evictions"]),
                 "worst_backward_error_sampled": worst, "wall_s": round(wall, 2)})
+    out["under_pressure"] = serve_pressure(S, mix, rate, min(nreq, 150))
     return out
+
+
+def serve_pressure(S, mix, rate, nreq):
+    """The serving mix under a budget of half its device bytes, requests
+    uniform over all 16 matrices at the given Poisson rate: a request for a
+    parked matrix waits for its admission (upload + factorization) on the
+    submitting thread.  Latency is end to end: from the request's arrival
+    to its answer."""
+    import ctypes
+    import numpy as np
+    need = []
+    for A, g, np_ in mix:
+        with S.SnuMat(A, np_, grid=g, device=-1, precision="fp32") as M:
+            nb = ctypes.c_size_t()
+            S._check(S.lib().snumat_bytes_needed(M._h, ctypes.byref(nb)))
+            need.append(nb.value)
+    budget = int(0.5 * sum(need))
+    print("serving leg: under a binding budget", file=sys.stderr, flush=True)
+    srv = S.Server(budget_bytes=budget, slots=2, max_cols=256)
+    ids = [srv.add(A, np_, grid=g, precision="fp32") for A, g, np_ in mix]
+    srv.stats(reset=True)
+    adm0 = srv.stats()
+    arr = np.random.default_rng(6)
+    gaps = arr.exponential(1.0 / rate, nreq)
+    tickets = []
+    start = time.perf_counter()
+    due = start
+    for q in range(nreq):
+        due += gaps[q]
+        dt = due - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        key = int(arr.integers(len(mix)))
+        b = arr.standard_normal((mix[key][0].n, 1 << int(arr.integers(0, 7))))
+        t = srv.submit(ids[key], b)
+        tickets.append((key, b, t, time.perf_counter() - due))  # arrival -> queued
+    lat, worst, cols = [], 0.0, 0
+    for key, b, t, queued in tickets:
+        x, ms = srv.wait(t)
+        lat.append(1e3 * queued + ms)
+        cols += b.shape[1]
+        if t % 10 == 0:
+            worst = max(worst, S.backward_error(mix[key][0], x, b))
+    wall = time.perf_counter() - start
+    st = srv.stats()
+    srv.close()
+    lat = np.sort(np.array(lat))
+    return {"budget_gb": round(budget / 1e9, 2), "resident_set_gb": round(sum(need) / 1e9, 2),
+            "rate_per_s": round(rate, 2), "requests": nreq, "columns": cols,
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(float(lat.mean()), 3), "solves_per_s": round(nreq / wall, 2),
+            "readmissions": int(st["readmissions"] - adm0["readmissions"]),
+            "evictions": int(st["evictions"] - adm0["evictions"]),
+            "worst_backward_error_sampled": worst, "wall_s": round(wall, 2)}
 
 
 def batch_leg(S, np, count=512):
@@ -411,6 +470,21 @@ def batch_leg(S, np, count=512):
             batched = (time.perf_counter() - t0) / reps
             bt = B.timings()
             fl, sl = B.launches()
+            # the same batch with the factorization replayed from a HIP graph
+            # (captured on the first factorize after switching it on)
+            B.use_graph(True)
+            B.factorize()
+            B.solve(bs)
+            S.device_sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                B.factorize()
+                xg = B.solve(bs)
+            S.device_sync()
+            graphed = (time.perf_counter() - t0) / reps
+            gt = B.timings()
+            B.use_graph(False)
+        same = all(np.array_equal(x, y) for x, y in zip(xs, xg))
         worst = max(S.backward_error(A, x, b) for (A, _, _), x, b in zip(mem, xs, bs))
         t0 = time.perf_counter()
         for M, b in zip(mats, bs):
@@ -423,6 +497,8 @@ def batch_leg(S, np, count=512):
                      "matrices_per_s": round(count / batched, 2), "speedup_vs_one_by_one": round(single / batched, 2),
                      "batch_factor_ms": round(bt["factor_ms"], 3), "batch_solve_ms": round(bt["solve_ms"], 3),
                      "dense_factor_launches": fl, "dense_solve_launches": sl,
+                     "graph_factorize_solve_s": round(graphed, 4), "graph_factor_ms": round(gt["factor_ms"], 3),
+                     "graph_results_equal_eager": bool(same),
                      "worst_backward_error": worst}
     return out
 

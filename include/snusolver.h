@@ -109,6 +109,13 @@ int snumat_bytes_needed(const snumat_t *h, size_t *bytes);
  * A's values (same pattern as at construct).  No reference counterpart. */
 int snumat_attach(snumat_t *h, int device, const double *values);
 
+/* Release every device allocation of a handle, keeping its symbolic plan:
+ * the handle is then host-only, as constructed with device < 0, and can be
+ * attached again (snumat_attach) without a new analysis.  An FP32 handle's
+ * FP64 twin is destroyed.  No reference counterpart (serving under a memory
+ * budget parks evicted matrices this way). */
+int snumat_detach(snumat_t *h);
+
 /* Numeric factorization on the GPU from the resident copy of A's values. */
 int snumat_factorize(snumat_t *h);
 

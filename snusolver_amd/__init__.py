@@ -587,7 +587,8 @@ class Server:
         st = (C.c_double * 16)()
         _check(lib().snuserve_stats(self._h, st, 1 if reset else 0))
         keys = ["requests", "columns", "packed_solves", "p50_ms", "p99_ms", "mean_ms", "max_ms",
-                "evictions", "admissions", "resident", "bytes_reserved", "wall_ms", "slot_busy_ms"]
+                "evictions", "admissions", "resident", "bytes_reserved", "wall_ms", "slot_busy_ms",
+                "readmissions", "parked"]
         return {k: float(st[i]) for i, k in enumerate(keys)}
 
     def close(self):

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <map>
 #include <mutex>
 #include <string>
@@ -247,6 +248,35 @@ extern "C" int snumat_attach(snumat_t *h, int device, const double *values) {
     h->device = -1;
   }
   return rc;
+}
+
+// Release every device allocation of a handle and return it to the
+// host-only state of a construct with device < 0 (the symbolic plan, the
+// options and the graph and profiling switches kept), ready for
+// snumat_attach.  The FP64 twin of an FP32 handle is destroyed.
+extern "C" int snumat_detach(snumat_t *h) {
+  if (!h) return set_error(SNU_ERR_BAD_INPUT, "null handle");
+  if (!h->on_device) return SNU_OK;
+  if (h->fb64) {
+    snumat_destroy(h->fb64);
+    h->fb64 = nullptr;
+  }
+  HIPCHK(hipSetDevice(h->device));
+  if (h->stream) HIPCHK(hipStreamSynchronize(h->stream));
+  free_device(h);
+  Plan P = std::move(h->P);
+  snu_opts_t o = h->opts;
+  const int prec = h->prec;
+  const bool prof = h->profile, graph = h->use_graph;
+  h->~snumat();
+  new (h) snumat();
+  h->P = std::move(P);
+  h->opts = o;
+  h->opts.device = -1;
+  h->prec = prec;
+  h->profile = prof;
+  h->use_graph = graph;
+  return SNU_OK;
 }
 
 extern "C" int snumat_update_values(snumat_t *h, const double *values) {

@@ -12,6 +12,10 @@
 // New factorizations (snuserve_add) are analysed on the host, admitted
 // against the budget by evicting least-recently-used idle matrices, then
 // uploaded and factorized on the caller's thread while slots keep serving.
+// An evicted matrix is parked, not forgotten: its device memory is released
+// but its symbolic plan and a host copy of its values stay, and the next
+// request for it admits it again (upload and factorization, no analysis) on
+// the submitting thread before it is queued.
 //
 // Built from the public C-ABI only (snumat_*, snubatch_*).
 #include <algorithm>
@@ -51,7 +55,9 @@ struct Resident {
   int n = 0;
   size_t bytes = 0;
   double last_use = 0;
-  int busy = 0;  // pending or in-flight requests
+  int busy = 0;               // pending or in-flight requests
+  bool parked = false;        // evicted: host-only handle, no device memory
+  std::vector<double> vals;   // A's values (for admitting a parked matrix again)
 };
 
 }  // namespace
@@ -63,7 +69,7 @@ struct snuserve {
   std::condition_variable cv_work, cv_done;
   std::shared_mutex set_mu;      // shared: a packed solve; exclusive: eviction
   std::mutex add_mu;             // one admission at a time
-  std::map<int, Resident> res;
+  std::map<int, Resident> res;  // resident and parked matrices
   int next_id = 1;
   unsigned long long generation = 0;
   std::deque<long long> pending;
@@ -73,7 +79,7 @@ struct snuserve {
   bool stop = false;
   // statistics
   std::vector<double> lat;
-  double cols = 0, packs = 0, evictions = 0, admissions = 0, busy_ms = 0;
+  double cols = 0, packs = 0, evictions = 0, admissions = 0, readmissions = 0, busy_ms = 0;
   double t_first = 0, t_last = 0;
 };
 
@@ -111,7 +117,8 @@ void worker(snuserve *s) {
         if (gen != s->generation) {
           ids.clear();
           mats.clear();
-          for (auto &kv : s->res) ids.push_back(kv.first), mats.push_back(kv.second.h);
+          for (auto &kv : s->res)
+            if (!kv.second.parked) ids.push_back(kv.first), mats.push_back(kv.second.h);
           gen = s->generation;
           if (batch) snubatch_destroy(batch);
           batch = nullptr;
@@ -224,8 +231,46 @@ extern "C" int snuserve_create(int device, size_t budget_bytes, int slots, int m
   return SNU_OK;
 }
 
-// Admission: host analysis, eviction of idle least-recently-used matrices
-// until the new one fits the budget, upload, factorization.
+namespace {
+
+// Room for `need` more bytes: park idle least-recently-used resident
+// matrices until they fit, waiting for busy ones to finish when every
+// resident matrix is busy; the bytes are then reserved.  Called with add_mu
+// held (one admission at a time).
+void make_room(snuserve *s, size_t need) {
+  for (;;) {
+    {
+      std::unique_lock<std::shared_mutex> ex(s->set_mu);  // no packed solve in flight
+      std::lock_guard<std::mutex> lk(s->mu);
+      while (s->used + need > s->budget) {
+        int best = -1;
+        double t = 0;
+        for (auto &kv : s->res)
+          if (!kv.second.parked && kv.second.busy == 0 && (best < 0 || kv.second.last_use < t))
+            best = kv.first, t = kv.second.last_use;
+        if (best < 0) break;
+        Resident &v = s->res[best];
+        (void)snumat_detach(v.h);
+        v.parked = true;
+        s->used -= v.bytes;
+        s->evictions += 1;
+        s->generation++;
+      }
+      if (s->used + need <= s->budget) {
+        s->used += need;
+        return;
+      }
+    }
+    // every remaining matrix is busy: wait for a request to finish
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait_for(lk, std::chrono::milliseconds(5));
+  }
+}
+
+}  // namespace
+
+// Admission: host analysis, room made under the budget, upload,
+// factorization.
 extern "C" int snuserve_add(snuserve_t *s, const snu_csr_t *A, const snu_opts_t *opts, int *id) {
   if (!s || !A || !opts || !id) return snu::set_error(SNU_ERR_BAD_INPUT, "null");
   std::lock_guard<std::mutex> admit(s->add_mu);
@@ -243,33 +288,7 @@ extern "C" int snuserve_add(snuserve_t *s, const snu_csr_t *A, const snu_opts_t 
     return snu::set_error(SNU_ERR_OOM, "matrix needs " + std::to_string(need) + " bytes, budget " +
                                            std::to_string(s->budget));
   }
-  for (;;) {
-    std::vector<snumat_t *> victims;
-    {
-      std::unique_lock<std::shared_mutex> ex(s->set_mu);  // no packed solve in flight
-      std::lock_guard<std::mutex> lk(s->mu);
-      while (s->used + need > s->budget) {
-        int best = -1;
-        double t = 0;
-        for (auto &kv : s->res)
-          if (kv.second.busy == 0 && (best < 0 || kv.second.last_use < t)) best = kv.first, t = kv.second.last_use;
-        if (best < 0) break;
-        victims.push_back(s->res[best].h);
-        s->used -= s->res[best].bytes;
-        s->res.erase(best);
-        s->evictions += 1;
-        s->generation++;
-      }
-      for (snumat_t *v : victims) snumat_destroy(v);
-      if (s->used + need <= s->budget) {
-        s->used += need;  // reserved
-        break;
-      }
-    }
-    // every remaining matrix is busy: wait for a request to finish
-    std::unique_lock<std::mutex> lk(s->mu);
-    s->cv_done.wait_for(lk, std::chrono::milliseconds(5));
-  }
+  make_room(s, need);
   rc = snumat_attach(h, s->device, A->data);
   if (!rc) rc = snumat_factorize(h);
   std::lock_guard<std::mutex> lk(s->mu);
@@ -283,13 +302,49 @@ extern "C" int snuserve_add(snuserve_t *s, const snu_csr_t *A, const snu_opts_t 
   r.n = A->n;
   r.bytes = need;
   r.last_use = now_ms();
+  r.vals.assign(A->data, A->data + A->nnz);
   *id = s->next_id++;
-  s->res[*id] = r;
+  s->res[*id] = std::move(r);
   s->prec = opts->precision;
   s->generation++;
   s->admissions += 1;
   return SNU_OK;
 }
+
+namespace {
+// A request for a parked matrix: admit it again (room, upload of the kept
+// values, factorization).  Returns with the matrix resident or an error.
+int readmit(snuserve *s, int id) {
+  std::lock_guard<std::mutex> admit(s->add_mu);
+  snumat_t *h = nullptr;
+  const double *vals = nullptr;
+  size_t need = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->res.find(id);
+    if (it == s->res.end()) return snu::set_error(SNU_ERR_BAD_INPUT, "unknown matrix id");
+    if (!it->second.parked) return SNU_OK;  // admitted again by another request
+    h = it->second.h;
+    vals = it->second.vals.data();
+    need = it->second.bytes;
+  }
+  make_room(s, need);
+  int rc = snumat_attach(h, s->device, vals);
+  if (!rc) rc = snumat_factorize(h);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (rc) {
+    (void)snumat_detach(h);
+    s->used -= need;
+    return rc;
+  }
+  Resident &r = s->res[id];
+  r.parked = false;
+  r.last_use = now_ms();
+  s->generation++;
+  s->readmissions += 1;
+  return SNU_OK;
+}
+}  // namespace
 
 extern "C" int snuserve_remove(snuserve_t *s, int id) {
   if (!s) return snu::set_error(SNU_ERR_BAD_INPUT, "null");
@@ -299,7 +354,7 @@ extern "C" int snuserve_remove(snuserve_t *s, int id) {
   if (it == s->res.end()) return snu::set_error(SNU_ERR_BAD_INPUT, "unknown matrix id");
   if (it->second.busy) return snu::set_error(SNU_ERR_BAD_INPUT, "matrix has requests in flight");
   snumat_destroy(it->second.h);
-  s->used -= it->second.bytes;
+  if (!it->second.parked) s->used -= it->second.bytes;
   s->res.erase(it);
   s->generation++;
   return SNU_OK;
@@ -308,13 +363,23 @@ extern "C" int snuserve_remove(snuserve_t *s, int id) {
 extern "C" int snuserve_submit(snuserve_t *s, int id, const double *b, double *x, int nrhs, int ld,
                                long long *ticket) {
   if (!s || !b || !x || nrhs < 1 || !ticket) return snu::set_error(SNU_ERR_BAD_INPUT, "bad request");
+  for (;;) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      auto it = s->res.find(id);
+      if (it == s->res.end()) return snu::set_error(SNU_ERR_BAD_INPUT, "unknown matrix id");
+      if (ld < it->second.n) return snu::set_error(SNU_ERR_BAD_INPUT, "ld < n");
+      if (nrhs > s->max_cols) return snu::set_error(SNU_ERR_BAD_INPUT, "nrhs above the server's column cap");
+      if (!it->second.parked) {
+        it->second.busy++;  // from here on it cannot be parked
+        break;
+      }
+    }
+    int rc = readmit(s, id);
+    if (rc) return rc;
+  }
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->res.find(id);
-    if (it == s->res.end()) return snu::set_error(SNU_ERR_BAD_INPUT, "unknown or evicted matrix id");
-    if (ld < it->second.n) return snu::set_error(SNU_ERR_BAD_INPUT, "ld < n");
-    if (nrhs > s->max_cols) return snu::set_error(SNU_ERR_BAD_INPUT, "nrhs above the server's column cap");
-    it->second.busy++;
     Request r;
     r.id = id;
     r.b = b;
@@ -348,7 +413,8 @@ extern "C" int snuserve_wait(snuserve_t *s, long long ticket, double *latency_ms
 // st[3] p50 latency ms, st[4] p99 ms, st[5] mean ms, st[6] max ms,
 // st[7] evictions, st[8] admissions, st[9] resident matrices, st[10] bytes
 // reserved, st[11] wall ms from the first submit to the last answer,
-// st[12] summed slot busy ms.  reset != 0 clears the latency record.
+// st[12] summed slot busy ms, st[13] admissions of parked matrices,
+// st[14] parked matrices.  reset != 0 clears the latency record.
 extern "C" int snuserve_stats(snuserve_t *s, double *st16, int reset) {
   if (!s || !st16) return snu::set_error(SNU_ERR_BAD_INPUT, "null");
   std::lock_guard<std::mutex> lk(s->mu);
@@ -371,10 +437,14 @@ extern "C" int snuserve_stats(snuserve_t *s, double *st16, int reset) {
   st16[6] = l.empty() ? 0 : l.back();
   st16[7] = s->evictions;
   st16[8] = s->admissions;
-  st16[9] = (double)s->res.size();
+  int resident = 0;
+  for (auto &kv : s->res) resident += !kv.second.parked;
+  st16[9] = resident;
   st16[10] = (double)s->used;
   st16[11] = s->t_last > s->t_first ? s->t_last - s->t_first : 0;
   st16[12] = s->busy_ms;
+  st16[13] = s->readmissions;
+  st16[14] = (double)s->res.size() - resident;
   if (reset) {
     s->lat.clear();
     s->cols = s->packs = s->busy_ms = 0;

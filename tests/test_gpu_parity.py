@@ -334,8 +334,9 @@ def test_serve_alone_vs_mix(S, precision):
 
 
 def test_serve_admission_evicts_lru(S):
-    """Admission under a budget evicts the least recently used idle matrix;
-    requests against an evicted id are refused."""
+    """Admission under a budget parks the least recently used idle matrix;
+    a request for a parked matrix admits it again (parking the then least
+    recently used one) and is answered."""
     mats = []
     for i, side in enumerate((48, 50, 52)):
         A = S.gen_grid(side, side, 1, 5, 0, 4.1, i)
@@ -353,11 +354,17 @@ def test_serve_admission_evicts_lru(S):
         x, _ = srv.wait(srv.submit(a, np.ones(mats[0][0].n)))  # a is now the most recent
         c = srv.add(mats[2][0], 4, order=mats[2][1][0], sizes=mats[2][1][1])
         st = srv.stats()
-        assert st["evictions"] == 1 and st["resident"] == 2
-        with pytest.raises(S.SnuError):
-            srv.submit(b, np.ones(mats[1][0].n))
+        assert st["evictions"] == 1 and st["resident"] == 2 and st["parked"] == 1
         y, _ = srv.wait(srv.submit(c, np.ones(mats[2][0].n)))
         assert S.backward_error(mats[2][0], y, np.ones(mats[2][0].n)) <= BERR_TOL
+        # b was parked; a is now the least recently used and is parked for it
+        z, _ = srv.wait(srv.submit(b, np.ones(mats[1][0].n)))
+        assert S.backward_error(mats[1][0], z, np.ones(mats[1][0].n)) <= BERR_TOL
+        st = srv.stats()
+        assert st["evictions"] == 2 and st["readmissions"] == 1
+        assert st["resident"] == 2 and st["parked"] == 1 and st["bytes_reserved"] <= budget
+        with pytest.raises(S.SnuError):
+            srv.submit(99, np.ones(mats[1][0].n))  # never admitted
 
 
 def C_size():

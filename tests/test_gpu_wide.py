@@ -10,7 +10,7 @@ Bars as in test_gpu_parity: x within 1e-9 relative (max norm) of the oracle,
 relative backward error <= 1e-12; leaf LU values within 1e-12 (normwise per
 leaf); stage-1 separator
 blocks (A_SS plus the leaf Schur contributions) within 1e-13 of the oracle's
-(the dense tail of a leaf is summed on MFMA) with the same zero pattern.
+(the leaf fronts' updates are summed on MFMA) with the same zero pattern.
 """
 import os
 import subprocess
@@ -179,6 +179,44 @@ def test_serve_stream_vs_oracle(S, oracle_mod, precision):
     for q, (mid, b, x) in enumerate(answers):
         A, np_, (o, z) = known[mid]
         _vs_oracle(S, oracle_mod, A, np_, o, z, x, b, f"request {q} (matrix {mid}, {b.shape[1]} cols)")
+
+
+def _bytes_needed(S, A, np_, order, sizes, precision):
+    import ctypes
+    with S.SnuMat(A, np_, order=order, sizes=sizes, device=-1, precision=precision) as M:
+        nb = ctypes.c_size_t()
+        S._check(S.lib().snumat_bytes_needed(M._h, ctypes.byref(nb)))
+        return nb.value
+
+
+def test_serve_under_memory_pressure_vs_oracle(S, oracle_mod):
+    """A working set larger than the budget: eight matrices, a budget that
+    holds about 40% of their bytes, requests for all of them; the server
+    parks idle least-recently-used matrices to admit others and admits a
+    parked matrix again when a request for it arrives.  Evictions and
+    re-admissions must happen; every answer against the oracle."""
+    mix = _serve_mix(S, np.random.default_rng(7))
+    need = [_bytes_needed(S, A, np_, o, z, "fp32") for A, np_, (o, z) in mix]
+    budget = max(int(0.4 * sum(need)), max(need) + 1)
+    arr = np.random.default_rng(8)
+    with S.Server(budget_bytes=budget, slots=2, max_cols=256) as srv:
+        ids = [srv.add(A, np_, order=o, sizes=z, precision="fp32") for A, np_, (o, z) in mix]
+        tickets = []
+        for q in range(40):
+            key = int(arr.integers(len(mix)))
+            b = arr.standard_normal((mix[key][0].n, 1 << int(arr.integers(0, 4))))
+            tickets.append((key, b, srv.submit(ids[key], b)))
+        answers = [(key, b, srv.wait(t)[0]) for key, b, t in tickets]
+        st = srv.stats()
+        print(f"budget {budget / 1e6:.1f} MB of {sum(need) / 1e6:.1f} MB: {st['admissions']:.0f} admissions, "
+              f"{st['evictions']:.0f} evictions, {st['readmissions']:.0f} re-admissions, "
+              f"{st['requests']:.0f} requests, p99 {st['p99_ms']:.2f} ms", flush=True)
+        assert st["requests"] == len(tickets)
+        assert st["evictions"] >= 3 and st["readmissions"] >= 3
+        assert st["bytes_reserved"] <= budget
+    for q, (key, b, x) in enumerate(answers):
+        A, np_, (o, z) = mix[key]
+        _vs_oracle(S, oracle_mod, A, np_, o, z, x, b, f"request {q} (matrix {key}, {b.shape[1]} cols)")
 
 
 def test_c2_full_size_vs_oracle(S, oracle_mod):
