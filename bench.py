@@ -390,16 +390,16 @@ def serve_leg(S, np, rate, nreq, budget=200e9):
                 "solves_per_s": round(st["requests"] / wall, 2), "columns_per_s": round(cols / wall, 2),
                 "new_factorizations": len(adds), "evictions": int(st["evictions"]),
                 "worst_backward_error_sampled": worst, "wall_s": round(wall, 2)})
-    out["under_pressure"] = serve_pressure(S, mix, rate, min(nreq, 150))
+    out["under_pressure"] = serve_pressure(S, mix, 150)
     return out
 
 
-def serve_pressure(S, mix, rate, nreq):
+def serve_pressure(S, mix, nreq):
     """The serving mix under a budget of half its device bytes, requests
-    uniform over all 16 matrices at the given Poisson rate: a request for a
-    parked matrix waits for its admission (upload + factorization) on the
-    submitting thread.  Latency is end to end: from the request's arrival
-    to its answer."""
+    uniform over all 16 matrices: a request for a parked matrix waits for its
+    admission (upload + factorization) on the submitting thread.  Offered at
+    half the capacity a closed-loop burst measures under the same budget.
+    Latency is end to end: from the request's arrival to its answer."""
     import ctypes
     import numpy as np
     need = []
@@ -412,9 +412,21 @@ def serve_pressure(S, mix, rate, nreq):
     print("serving leg: under a binding budget", file=sys.stderr, flush=True)
     srv = S.Server(budget_bytes=budget, slots=2, max_cols=256)
     ids = [srv.add(A, np_, grid=g, precision="fp32") for A, g, np_ in mix]
+    arr = np.random.default_rng(6)
+
+    def draw():
+        key = int(arr.integers(len(mix)))
+        return key, arr.standard_normal((mix[key][0].n, 1 << int(arr.integers(0, 7))))
+
+    t0 = time.perf_counter()
+    burst = [draw() for _ in range(32)]
+    for t in [srv.submit(ids[key], b) for key, b in burst]:
+        srv.wait(t)
+    cap = len(burst) / (time.perf_counter() - t0)
+    rate = 0.5 * cap
+    nreq = max(20, min(nreq, int(40 * rate)))
     srv.stats(reset=True)
     adm0 = srv.stats()
-    arr = np.random.default_rng(6)
     gaps = arr.exponential(1.0 / rate, nreq)
     tickets = []
     start = time.perf_counter()
@@ -424,8 +436,7 @@ def serve_pressure(S, mix, rate, nreq):
         dt = due - time.perf_counter()
         if dt > 0:
             time.sleep(dt)
-        key = int(arr.integers(len(mix)))
-        b = arr.standard_normal((mix[key][0].n, 1 << int(arr.integers(0, 7))))
+        key, b = draw()
         t = srv.submit(ids[key], b)
         tickets.append((key, b, t, time.perf_counter() - due))  # arrival -> queued
     lat, worst, cols = [], 0.0, 0
@@ -440,7 +451,7 @@ def serve_pressure(S, mix, rate, nreq):
     srv.close()
     lat = np.sort(np.array(lat))
     return {"budget_gb": round(budget / 1e9, 2), "resident_set_gb": round(sum(need) / 1e9, 2),
-            "rate_per_s": round(rate, 2), "requests": nreq, "columns": cols,
+            "capacity_req_per_s": round(cap, 2), "rate_per_s": round(rate, 2), "requests": nreq, "columns": cols,
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_ms": round(float(lat.mean()), 3), "solves_per_s": round(nreq / wall, 2),
             "readmissions": int(st["readmissions"] - adm0["readmissions"]),
