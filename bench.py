@@ -133,7 +133,8 @@ def main():
 
     # dominant kernel: the grouped FP64 MFMA GEMM (C -= sum A B) instantiation
     # (64 x 64 or 128 x 128 tiles) with the larger share of kernel time,
-    # timed per launch with HIP events on the library's stream
+    # timed per launch with HIP events on the library's stream (every launch
+    # of it in a factorization: the leaf fronts' and the dense levels')
     M.set_profiling(True)
     M.factorize()
     M.set_profiling(False)

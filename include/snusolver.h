@@ -282,10 +282,10 @@ int snumat_sep_block(const snumat_t *h, int i, int j, double *out);
  * final backward error (max over columns), t[15] 1 when that solve fell back
  * to FP64 factors (refinement stalled above 1e-12), else 0. */
 int snumat_timings(const snumat_t *h, double *t16);
-/* the C -= AB launches of the last factorize split by tile: out[0..2] the
- * 64 x 64 ones (kernel time in ms with snumat_set_profiling, launches,
- * GFLOP), out[3..5] the 128 x 128 ones.  Not part of the reference
- * interface (profiling aid). */
+/* the C -= AB launches of the last factorize (leaf fronts and dense
+ * levels) split by tile: out[0..2] the 64 x 64 ones (kernel time in ms with
+ * snumat_set_profiling, launches, GFLOP), out[3..5] the 128 x 128 ones.  Not
+ * part of the reference interface (profiling aid). */
 int snumat_gemm_split(const snumat_t *h, double *out6);
 /* algorithmic counts from the symbolic structure: c[0] F_dense (flops),
  * c[1] F_leaf (flops), c[2] leaf factor bytes, c[3] solve bytes per rhs. */

@@ -424,8 +424,8 @@ class SnuMat:
         return {k: float(t[i]) for i, k in enumerate(keys)}
 
     def gemm_split(self) -> dict:
-        """C -= AB launches of the last factorize by tile size (kernel ms
-        when profiling, launches, GFLOP)."""
+        """C -= AB launches of the last factorize (leaf fronts and dense
+        levels) by tile size (kernel ms when profiling, launches, GFLOP)."""
         t = (C.c_double * 6)()
         _check(lib().snumat_gemm_split(self._h, t))
         return {"t64_ms": t[0], "t64_launches": t[1], "t64_gflop": t[2],

@@ -304,7 +304,7 @@ int factorize_values(snumat *h, int stage) {
   fs.fronts = &vals<T>(h).fronts;
   fs.eps = h->opts.eps;
   static const DensePlan<T> empty;
-  double gms[3] = {0, 0, 0};
+  double gms[5] = {0, 0, 0, 0, 0};
   int rc = run_factor<T>(fs, stage >= 2 ? VV.fact : empty, h->stream, &h->ev, h->profile && stage >= 2,
                          &h->pev, gms, h->use_graph && stage >= 2 ? &h->graph : nullptr);
   if (rc) return rc;
@@ -332,11 +332,19 @@ int factorize_values(snumat *h, int stage) {
         h->gsplit[q + 2] += L.flops * 1e-9;
       }
     }
+    // the per-tile split also counts the leaf fronts' C -= AB launches (the
+    // same kernel instantiations)
+    for (const Launch &L : VV.fronts.dense.launches)
+      if (L.kind == 2 || L.kind == 5) {
+        const int q = L.kind == 5 ? 3 : 0;
+        h->gsplit[q + 1] += 1;
+        h->gsplit[q + 2] += L.flops * 1e-9;
+      }
     if (h->profile) {
       h->t[7] = gms[0];
       h->t[10] = gms[1];
-      h->gsplit[0] = gms[0] - gms[2];
-      h->gsplit[3] = gms[2];
+      h->gsplit[0] = gms[0] - gms[2] + gms[3];
+      h->gsplit[3] = gms[2] + gms[4];
     }
   }
   h->factored = stage >= 2;

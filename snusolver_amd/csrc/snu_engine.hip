@@ -1496,10 +1496,13 @@ void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
 }
 }  // namespace
 
-// ev (optional): 6 events marking prepare / leaf rows / leaf sep / assembly /
-// dense; pev (optional, with profile): a begin/end pair per GEMM launch,
-// their times summed into gemm_ms[0] (C -= AB), gemm_ms[1] (C = +-AB) and
-// gemm_ms[2] (the C -= AB launches on 128 x 128 tiles, part of gemm_ms[0]).
+// ev (optional): 6 events marking prepare / leaf fronts / leaf Schur /
+// assembly / dense; pev (optional, with profile): a begin/end pair per GEMM
+// launch (the leaf fronts' launches, then the dense ones), their times summed
+// into gemm_ms[0] (dense C -= AB), gemm_ms[1] (dense C = +-AB), gemm_ms[2]
+// (the dense C -= AB launches on 128 x 128 tiles, part of gemm_ms[0]) and
+// gemm_ms[3] / gemm_ms[4] (the fronts' C -= AB launches on 64 x 64 / 128 x
+// 128 tiles).
 // The launches of one factorization on st (no host synchronisation).
 template <typename T>
 int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
@@ -1512,15 +1515,33 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
   // leaf fronts, height by height: whole fronts in LDS; larger fronts
   // assembled, factored by the dense launches of their height, written back
   const FrontPlan<T> &fp = *(const FrontPlan<T> *)fs.fronts;
+  int ngemm = 0;
+  for (const Launch &L : fp.dense.launches) ngemm += is_gemm(L.kind);
+  for (const Launch &L : plan.launches) ngemm += is_gemm(L.kind);
+  const bool prof = profile && pev;
+  if (prof && (int)pev->size() < 2 * ngemm) {
+    for (hipEvent_t e : *pev) (void)hipEventDestroy(e);
+    pev->assign(2 * ngemm, nullptr);
+    for (auto &e : *pev) HIPCHK(hipEventCreate(&e));
+  }
+  int g = 0;
+  auto run_timed = [&](const Launch &L, const DensePlan<T> &dp) -> int {
+    const bool gm = prof && is_gemm(L.kind);
+    if (gm) HIPCHK(hipEventRecord((*pev)[2 * g], st));
+    run_launch<T>(L, dp.d_gp, dp.d_gs, dp.d_dp, nullptr, nullptr, nullptr, st);
+    if (gm) HIPCHK(hipEventRecord((*pev)[2 * g++ + 1], st));
+    return SNU_OK;
+  };
   size_t li = 0;
   const size_t nl = fp.dense.launches.size();
   for (int hh = 0; hh < (int)fp.levels.size(); hh++) {
     const FrontLevel &lv = fp.levels[hh];
     launch_front_small<T>(fp.d_small + lv.small0, lv.nsmall, lv.max_f, fs.eps, st);
     launch_front_assemble<T>(fp.d_large, fp.d_work + lv.work0, lv.nwork, st);
-    for (; li < nl && front_key_height(fp.dense.launches[li].key) == hh; li++)
-      run_launch<T>(fp.dense.launches[li], fp.dense.d_gp, fp.dense.d_gs, fp.dense.d_dp, nullptr, nullptr,
-                    nullptr, st);
+    for (; li < nl && front_key_height(fp.dense.launches[li].key) == hh; li++) {
+      int rc = run_timed(fp.dense.launches[li], fp.dense);
+      if (rc) return rc;
+    }
     launch_front_writeback<T>(fp.d_large, fp.d_work + lv.work0, lv.nwork, st);
   }
   HIPCHK(hipGetLastError());
@@ -1532,22 +1553,9 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
                            wd.leaves, st);
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[4], st));
-  int ngemm = 0;
-  for (const Launch &L : plan.launches) ngemm += is_gemm(L.kind);
-  if (profile && pev && (int)pev->size() < 2 * ngemm) {
-    for (hipEvent_t e : *pev) (void)hipEventDestroy(e);
-    pev->assign(2 * ngemm, nullptr);
-    for (auto &e : *pev) HIPCHK(hipEventCreate(&e));
-  }
-  int g = 0;
   for (const Launch &L : plan.launches) {
-    const bool gm = is_gemm(L.kind);
-    if (profile && pev && gm) HIPCHK(hipEventRecord((*pev)[2 * g], st));
-    run_launch<T>(L, plan.d_gp, plan.d_gs, plan.d_dp, nullptr, nullptr, nullptr, st);
-    if (profile && pev && gm) {
-      HIPCHK(hipEventRecord((*pev)[2 * g + 1], st));
-      g++;
-    }
+    int rc = run_timed(L, plan);
+    if (rc) return rc;
   }
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[5], st));
@@ -1590,8 +1598,16 @@ int run_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st,
   HIPCHK(hipStreamSynchronize(st));
   if (profile && pev && gemm_ms) {
     HIPCHK(hipStreamSynchronize(st));
-    gemm_ms[0] = gemm_ms[1] = gemm_ms[2] = 0;
+    for (int q = 0; q < 5; q++) gemm_ms[q] = 0;
     int g = 0;
+    const FrontPlan<T> &fp = *(const FrontPlan<T> *)fs.fronts;
+    for (const Launch &L : fp.dense.launches) {
+      if (!is_gemm(L.kind)) continue;
+      const double ms = ev_ms((*pev)[2 * g], (*pev)[2 * g + 1]);
+      if (L.kind == 2) gemm_ms[3] += ms;
+      if (L.kind == 5) gemm_ms[4] += ms;
+      g++;
+    }
     for (const Launch &L : plan.launches) {
       if (!is_gemm(L.kind)) continue;
       const double ms = ev_ms((*pev)[2 * g], (*pev)[2 * g + 1]);
