@@ -262,6 +262,9 @@ int snumat_leaf_front_sizes(const snumat_t *h, int rank, int *sizes4);
 int snumat_leaf_fronts(const snumat_t *h, int rank, int *front5, int *idx_ptr,
                        int *idx, int *rel, int *slot_ptr, int *slot, int *pos,
                        int *child_ptr, int *child, int *srow, int *scol);
+/* the pivots K of each front (ascending within a front; k_ptr has
+ * fronts + 1 entries, k the leaf size). */
+int snumat_leaf_front_pivots(const snumat_t *h, int rank, int *k_ptr, int *k);
 /* leaf LU values after factorize (copied back from the device). */
 int snumat_leaf_values(const snumat_t *h, int rank, double *lu_data);
 /* dense diagonal block of separator e (1 <= e < np), s_e x s_e col-major. */

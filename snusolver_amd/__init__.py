@@ -386,6 +386,9 @@ class SnuMat:
         _check(lib().snumat_leaf_fronts(self._h, rank, *[_ip(a[k]) for k in keys]))
         fr = a.pop("front").reshape(nf, 5)
         a.update(k0=fr[:, 0], nk=fr[:, 1], f=fr[:, 2], parent=fr[:, 3], height=fr[:, 4])
+        a["k_ptr"] = np.zeros(nf + 1, np.int32)
+        a["k"] = np.zeros(int(fr[:, 1].sum()), np.int32)
+        _check(lib().snumat_leaf_front_pivots(self._h, rank, _ip(a["k_ptr"]), _ip(a["k"])))
         return a
 
     def leaf_values(self, rank: int) -> np.ndarray:

@@ -701,7 +701,7 @@ extern "C" int snumat_info(const snumat_t *h, int64_t *info) {
   info[12] = h->factored;
   info[13] = (int64_t)h->dev_bytes;
   int64_t nfr = 0;
-  for (const LeafPlan &L : P.leaves) nfr += (int64_t)L.fk0.size();
+  for (const LeafPlan &L : P.leaves) nfr += (int64_t)L.ff.size();
   info[14] = nfr;
   info[15] = h->wd.ntarget;
   return SNU_OK;
@@ -747,7 +747,7 @@ extern "C" int snumat_leaf_struct(const snumat_t *h, int rank, int *rp, int *ci,
 extern "C" int snumat_leaf_front_sizes(const snumat_t *h, int rank, int *s4) {
   if (!h || !s4 || rank < 0 || rank >= h->P.np) return set_error(SNU_ERR_BAD_INPUT, "bad rank");
   const LeafPlan &L = h->P.leaves[rank];
-  s4[0] = (int)L.fk0.size();
+  s4[0] = (int)L.ff.size();
   s4[1] = (int)L.fidx.size();
   s4[2] = (int)L.fslot.size();
   s4[3] = (int)L.fchild.size();
@@ -758,9 +758,9 @@ extern "C" int snumat_leaf_fronts(const snumat_t *h, int rank, int *front5, int 
                                   int *child, int *srow, int *scol) {
   if (!h || rank < 0 || rank >= h->P.np) return set_error(SNU_ERR_BAD_INPUT, "bad rank");
   const LeafPlan &L = h->P.leaves[rank];
-  const size_t nf = L.fk0.size();
+  const size_t nf = L.ff.size();
   for (size_t f = 0; f < nf; f++) {
-    front5[5 * f + 0] = L.fk0[f];
+    front5[5 * f + 0] = L.fk[L.fkp[f]];  // first pivot
     front5[5 * f + 1] = L.fnk[f];
     front5[5 * f + 2] = L.ff[f];
     front5[5 * f + 3] = L.fpar[f];
@@ -776,6 +776,13 @@ extern "C" int snumat_leaf_fronts(const snumat_t *h, int rank, int *front5, int 
   std::copy(L.fchild.begin(), L.fchild.end(), child);
   std::copy(L.fsrow.begin(), L.fsrow.end(), srow);
   std::copy(L.fscol.begin(), L.fscol.end(), scol);
+  return SNU_OK;
+}
+extern "C" int snumat_leaf_front_pivots(const snumat_t *h, int rank, int *k_ptr, int *k) {
+  if (!h || !k_ptr || !k || rank < 0 || rank >= h->P.np) return set_error(SNU_ERR_BAD_INPUT, "bad rank");
+  const LeafPlan &L = h->P.leaves[rank];
+  std::copy(L.fkp.begin(), L.fkp.end(), k_ptr);
+  std::copy(L.fk.begin(), L.fk.end(), k);
   return SNU_OK;
 }
 extern "C" int snumat_leaf_values(const snumat_t *h, int rank, double *out) {

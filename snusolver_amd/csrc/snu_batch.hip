@@ -430,7 +430,7 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
       merge_fronts(parts, fp);
       int r;
       if ((r = upload_dense_plan(fp.dense)) || (r = bupload(b, &fp.d_small, fp.small)) ||
-          (r = bupload(b, &fp.d_large, fp.large)) || (r = bupload(b, &fp.d_work, fp.work)) ||
+          (r = bupload(b, &fp.d_large, fp.large)) || (r = bupload(b, &fp.d_work, fp.work)) || (r = bupload(b, &fp.d_wrange, fp.wrange)) ||
           (r = bupload(b, &fp.d_ls, fp.ls)) || (r = bupload(b, &fp.d_tiles, fp.tiles)))
         return r;
       return SNU_OK;

@@ -75,13 +75,15 @@ struct FrontLevel {
 template <typename T>
 struct FrontPlan {
   std::vector<FrontDev> small, large;
-  std::vector<int2> work;
+  std::vector<int4> work;   // larger fronts: {front, first column, ranges offset, 0}
+  std::vector<int> wrange;  // per work item: its owned-slot range, then per child its column range
   std::vector<FrontLevel> levels;
   DensePlan<T> dense;
   std::vector<LeafSchurDev> ls;
   std::vector<int4> tiles;
   FrontDev *d_small = nullptr, *d_large = nullptr;
-  int2 *d_work = nullptr;
+  int4 *d_work = nullptr;
+  int *d_wrange = nullptr;
   LeafSchurDev *d_ls = nullptr;
   int4 *d_tiles = nullptr;
 };

@@ -949,7 +949,7 @@ int build_front_plan(snumat *h) {
     ints.insert(ints.end(), L.fpos.begin(), L.fpos.end());
     o_rel[r] = ints.size();
     ints.insert(ints.end(), L.frel.begin(), L.frel.end());
-    const int nf = (int)L.fk0.size();
+    const int nf = (int)L.ff.size();
     foff[r].resize(nf);
     for (int f = 0; f < nf; f++) {
       foff[r][f] = ftotal;
@@ -977,7 +977,7 @@ int build_front_plan(snumat *h) {
   std::vector<std::vector<int>> ch0(np);
   for (int r = 0; r < np; r++) {
     const LeafPlan &L = P.leaves[r];
-    const int nf = (int)L.fk0.size();
+    const int nf = (int)L.ff.size();
     ch0[r].resize(nf);
     for (int f = 0; f < nf; f++) {
       ch0[r][f] = (int)ch.size();
@@ -992,9 +992,10 @@ int build_front_plan(snumat *h) {
   if ((rc = dupload(h, &d_ch, ch))) return rc;
   // fronts by height
   std::vector<std::vector<FrontDev>> sm(levels), lg(levels);
+  std::vector<std::vector<std::pair<int, int>>> lgid(levels);  // (leaf, front) of each larger front
   for (int r = 0; r < np; r++) {
     const LeafPlan &L = P.leaves[r];
-    for (int f = 0; f < (int)L.fk0.size(); f++) {
+    for (int f = 0; f < (int)L.ff.size(); f++) {
       FrontDev D;
       D.F = fa + foff[r][f];
       D.vals = VV.arena + P.leaf_off[r];
@@ -1006,6 +1007,7 @@ int build_front_plan(snumat *h) {
       D.nslot = L.fslot_ptr[f + 1] - L.fslot_ptr[f];
       D.nchild = L.fchild_ptr[f + 1] - L.fchild_ptr[f];
       (D.f <= fmax ? sm : lg)[L.fheight[f]].push_back(D);
+      if (D.f > fmax) lgid[L.fheight[f]].push_back({r, f});
     }
   }
   FP.levels.assign(levels, FrontLevel());
@@ -1021,7 +1023,21 @@ int build_front_plan(snumat *h) {
     int kmax = 0;
     for (size_t q = 0; q < lg[hh].size(); q++) {
       const FrontDev &D = lg[hh][q];
-      for (int c0 = 0; c0 < D.f; c0 += 64) FP.work.push_back(make_int2(lbase + (int)q, c0));
+      const LeafPlan &L = P.leaves[lgid[hh][q].first];
+      const int fr = lgid[hh][q].second;
+      const int *pos = L.fpos.data() + L.fslot_ptr[fr];
+      for (int c0 = 0; c0 < D.f; c0 += 64) {
+        const int c1 = std::min(D.f, c0 + 64);
+        FP.work.push_back(make_int4(lbase + (int)q, c0, (int)FP.wrange.size(), 0));
+        FP.wrange.push_back((int)(std::lower_bound(pos, pos + D.nslot, c0 * D.f) - pos));
+        FP.wrange.push_back((int)(std::lower_bound(pos, pos + D.nslot, c1 * D.f) - pos));
+        for (int k = L.fchild_ptr[fr]; k < L.fchild_ptr[fr + 1]; k++) {
+          const int c = L.fchild[k];
+          const int *rel = L.frel.data() + L.fidx_ptr[c], nb = L.fidx_ptr[c + 1] - L.fidx_ptr[c];
+          FP.wrange.push_back((int)(std::lower_bound(rel, rel + nb, c0) - rel));
+          FP.wrange.push_back((int)(std::lower_bound(rel, rel + nb, c1) - rel));
+        }
+      }
       kmax = std::max(kmax, D.nk);
     }
     lv.nwork = (int)FP.work.size() - lv.work0;
@@ -1076,7 +1092,7 @@ int build_front_plan(snumat *h) {
     const LeafPlan &L = P.leaves[r];
     const int root0 = (int)roots.size();
     int k = 0;
-    for (int f = 0; f < (int)L.fk0.size(); f++) {
+    for (int f = 0; f < (int)L.ff.size(); f++) {
       if (L.fpar[f] >= 0) continue;
       const int ff = L.ff[f], nk = L.fnk[f];
       roots.push_back(FrontRoot{fa + foff[r][f] + nk + (long long)nk * ff, d_ints + o_root[r][k].first,
@@ -1095,7 +1111,8 @@ int build_front_plan(snumat *h) {
   if ((rc = dupload(h, &d_roots, roots))) return rc;
   for (LeafSchurDev &l : FP.ls) l.root = d_roots + (uintptr_t)l.root;
   if ((rc = dupload(h, &FP.d_small, FP.small)) || (rc = dupload(h, &FP.d_large, FP.large)) ||
-      (rc = dupload(h, &FP.d_work, FP.work)) || (rc = dupload(h, &FP.d_ls, FP.ls)) ||
+      (rc = dupload(h, &FP.d_work, FP.work)) || (rc = dupload(h, &FP.d_wrange, FP.wrange)) ||
+      (rc = dupload(h, &FP.d_ls, FP.ls)) ||
       (rc = dupload(h, &FP.d_tiles, FP.tiles)))
     return rc;
   return SNU_OK;
@@ -1106,6 +1123,12 @@ void merge_fronts(const std::vector<const FrontPlan<T> *> &in, FrontPlan<T> &out
   out.small.clear();
   out.large.clear();
   out.work.clear();
+  out.wrange.clear();
+  std::vector<int> rbase;
+  for (const FrontPlan<T> *p : in) {
+    rbase.push_back((int)out.wrange.size());
+    out.wrange.insert(out.wrange.end(), p->wrange.begin(), p->wrange.end());
+  }
   out.ls.clear();
   out.tiles.clear();
   size_t levels = 0;
@@ -1115,7 +1138,8 @@ void merge_fronts(const std::vector<const FrontPlan<T> *> &in, FrontPlan<T> &out
     FrontLevel &lv = out.levels[hh];
     lv.small0 = (int)out.small.size();
     lv.work0 = (int)out.work.size();
-    for (const FrontPlan<T> *p : in) {
+    for (size_t pi = 0; pi < in.size(); pi++) {
+      const FrontPlan<T> *p = in[pi];
       if (hh >= p->levels.size()) continue;
       const FrontLevel &pl = p->levels[hh];
       out.small.insert(out.small.end(), p->small.begin() + pl.small0, p->small.begin() + pl.small0 + pl.nsmall);
@@ -1130,7 +1154,7 @@ void merge_fronts(const std::vector<const FrontPlan<T> *> &in, FrontPlan<T> &out
       const int base = (int)out.large.size();
       out.large.insert(out.large.end(), p->large.begin() + first, p->large.begin() + last + 1);
       for (int q = pl.work0; q < pl.work0 + pl.nwork; q++)
-        out.work.push_back(make_int2(base + p->work[q].x - first, p->work[q].y));
+        out.work.push_back(make_int4(base + p->work[q].x - first, p->work[q].y, rbase[pi] + p->work[q].z, 0));
     }
     lv.nsmall = (int)out.small.size() - lv.small0;
     lv.nwork = (int)out.work.size() - lv.work0;
@@ -1435,7 +1459,7 @@ size_t estimate_device_bytes(const snumat *h) {
     // position / extend-add tables, root row / column maps, descriptors
     fronts += (double)L.fentries;
     for (int f : L.ff) fronts += f > fmax ? 2.0 * 64 * 64 : 0.0;
-    idx += 2.0 * L.fslot.size() + L.frel.size() + (double)(L.nbr + L.nbc) + 16.0 * L.fk0.size();
+    idx += 2.0 * L.fslot.size() + L.frel.size() + (double)(L.nbr + L.nbc) + 16.0 * L.ff.size();
   }
   S_total += fronts;
   for (int p = 1; p < P.np; p++) inv += 2.0 * P.block_size[p] * P.block_size[p];
@@ -1537,12 +1561,12 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
   for (int hh = 0; hh < (int)fp.levels.size(); hh++) {
     const FrontLevel &lv = fp.levels[hh];
     launch_front_small<T>(fp.d_small + lv.small0, lv.nsmall, lv.max_f, fs.eps, st);
-    launch_front_assemble<T>(fp.d_large, fp.d_work + lv.work0, lv.nwork, st);
+    launch_front_assemble<T>(fp.d_large, fp.d_work + lv.work0, fp.d_wrange, lv.nwork, st);
     for (; li < nl && front_key_height(fp.dense.launches[li].key) == hh; li++) {
       int rc = run_timed(fp.dense.launches[li], fp.dense);
       if (rc) return rc;
     }
-    launch_front_writeback<T>(fp.d_large, fp.d_work + lv.work0, lv.nwork, st);
+    launch_front_writeback<T>(fp.d_large, fp.d_work + lv.work0, fp.d_wrange, lv.nwork, st);
   }
   HIPCHK(hipGetLastError());
   if (ev) HIPCHK(hipEventRecord((*ev)[2], st));

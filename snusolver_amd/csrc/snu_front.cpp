@@ -3,27 +3,35 @@
 // The leaf's no-pivot LU (the reference's core_numfact_v2 and core_GEMM,
 // src/kernel_sparse.cu:740-781,953-971) is computed here as a sequence of
 // dense partial factorizations of frontal matrices instead of row by row:
-// a front holds a contiguous range K of leaf pivots and every later core
-// index B that those pivots touch (their rows and columns beyond K, in the
-// symmetrized pattern of the core); factoring the |K| pivots of a front
-// gives the final L and U entries of those pivots and leaves the update
-// -L(B,K) U(K,B) on B x B, which is added into the parent front (extend-add)
-// and, at the leaf's root fronts, forms the leaf's boundary Schur block.
+// a front holds a set K of leaf pivots and every later core index B that
+// those pivots touch (their rows and columns outside K, in the symmetrized
+// pattern of the core); factoring the |K| pivots of a front in ascending
+// order gives the final L and U entries of those pivots and leaves the
+// update -L(B,K) U(K,B) on B x B, which is added into the parent front
+// (extend-add) and, at the leaf's root fronts, forms the leaf's boundary
+// Schur block.
 //
 // Fronts follow the elimination tree of the symmetrized core pattern
-// (symbolic Cholesky of A + A^T restricted to the leaf): every subtree of at
-// most kSmallSubtree pivots whose indices are contiguous becomes one front,
-// and above those, chains (pivot k whose only child is k - 1) share one.
-// The union of the structures of a front's pivots beyond K equals the
-// structure of its last pivot (column containment of the symbolic Cholesky
-// factor), and a child's B minus its parent pivot lies inside the parent
-// front, so the extend-add never meets an index the parent lacks.
+// (symbolic Cholesky of A + A^T restricted to the leaf).  Every maximal
+// subtree whose front (its pivots and the structure of its root) has order
+// at most kSubtreeFront becomes one front, however its pivots are numbered;
+// above those, a pivot whose only child is a pivot of the same kind joins
+// that child's front (chains); then fronts with at most kRelaxPivots pivots
+// join their parent fronts.  A front's pivots are eliminated in
+// ascending order; pivots outside its subtree are neither descendants nor
+// ancestors of its pivots, so they update none of its entries, and the
+// factors are those of the reference's elimination order.  The union of the
+// structures of a front's pivots outside K equals the structure of its top
+// pivot (column containment of the symbolic Cholesky factor), and a child's
+// B minus the parent's pivots lies inside the parent front, so the
+// extend-add never meets an index the parent lacks.
 //
-// Every LU entry (i, j) of the reference's pattern belongs to the front whose
-// K holds min(i, j): that front reads A's value from the entry's slot and
-// writes the factor value back.  Positions of a front outside the LU pattern
-// stay exactly zero (each of their updates multiplies a structural zero), so
-// the factors are those of the reference's elimination, to rounding.
+// Every LU entry (i, j) of the reference's pattern belongs to the front
+// whose K holds min(i, j): that front reads A's value from the entry's slot
+// and writes the factor value back.  Positions of a front outside the LU
+// pattern stay exactly zero (each of their updates multiplies a structural
+// zero), so the factors are those of the reference's elimination, to
+// rounding.
 #include <algorithm>
 #include <numeric>
 #include <string>
@@ -34,12 +42,14 @@
 namespace snu {
 
 namespace {
-constexpr int kSmallSubtree = 32;
+constexpr int kSubtreeFront = 128;  // the order of the fronts factored whole in LDS (FP64)
+constexpr int kRelaxPivots = 16;    // fronts with at most this many pivots merge into their parent
 }
 
 int build_fronts(LeafPlan &L) {
   const int ls = L.leaf_size, cn = L.core_n;
-  L.fk0.clear();
+  L.fkp.assign(1, 0);
+  L.fk.clear();
   L.fnk.clear();
   L.ff.clear();
   L.fpar.clear();
@@ -84,46 +94,77 @@ int build_fronts(LeafPlan &L) {
       kids[s[0]].push_back(k);
     }
   }
-  // subtree sizes and first indices
-  std::vector<int> sub(ls, 1), first(ls);
-  std::iota(first.begin(), first.end(), 0);
+  // subtree pivot counts; a subtree collapses into one front when its order
+  // (pivots + the root's structure) is within kSubtreeFront.  The order grows
+  // towards the root (a child's structure lies in its parent's pivot and
+  // structure), so the collapsed fronts are the maximal such subtrees.
+  std::vector<int> sub(ls, 1);
   for (int k = 0; k < ls; k++)
-    if (parent[k] >= 0) {
-      sub[parent[k]] += sub[k];
-      first[parent[k]] = std::min(first[parent[k]], first[k]);
-    }
-  auto croot = [&](int r) {
-    return sub[r] <= kSmallSubtree && (parent[r] < 0 || sub[parent[r]] > kSmallSubtree) &&
-           first[r] + sub[r] - 1 == r;
-  };
-  // fronts as contiguous pivot ranges, ascending (postorder of the front tree)
-  std::vector<int> front_of(ls, -1);
-  for (int k = 0; k < ls;) {
-    int r = k;
-    while (!croot(r) && sub[r] <= kSmallSubtree && parent[r] >= 0) r = parent[r];
-    int end;
-    if (croot(r) && first[r] == k) {
-      end = r;
-    } else {
-      end = k;
-      while (parent[end] == end + 1 && kids[end + 1].size() == 1 && sub[end + 1] > kSmallSubtree) end++;
-    }
-    for (int x = k; x <= end; x++) front_of[x] = (int)L.fk0.size();
-    L.fk0.push_back(k);
-    L.fnk.push_back(end - k + 1);
-    k = end + 1;
+    if (parent[k] >= 0) sub[parent[k]] += sub[k];
+  auto coll = [&](int r) { return sub[r] + (long long)st[r].size() <= kSubtreeFront; };
+  // representative pivot of every pivot's front: the root of its collapsed
+  // subtree, or the bottom of its chain
+  std::vector<int> rep(ls);
+  for (int k = ls - 1; k >= 0; k--)  // parents first
+    if (coll(k)) rep[k] = parent[k] >= 0 && coll(parent[k]) ? rep[parent[k]] : k;
+  for (int k = 0; k < ls; k++) {  // children first
+    if (coll(k)) continue;
+    rep[k] = kids[k].size() == 1 && !coll(kids[k][0]) ? rep[kids[k][0]] : k;
   }
-  const int nf = (int)L.fk0.size();
+  // fronts ordered by their top (largest) pivot: a parent front's top is an
+  // etree ancestor of its children's tops, so children come first
+  std::vector<int> topof(ls, -1);  // per representative: the largest pivot of its front
+  for (int k = 0; k < ls; k++) topof[rep[k]] = std::max(topof[rep[k]], k);
+  std::vector<int> front_of(ls, -1), tops;
+  for (int k = 0; k < ls; k++)
+    if (topof[rep[k]] == k) front_of[k] = (int)tops.size(), tops.push_back(k);
+  for (int k = 0; k < ls; k++) front_of[k] = front_of[topof[rep[k]]];
+  // relaxed amalgamation: a front with at most kRelaxPivots pivots joins its
+  // parent front (its pivots become rows and columns of the parent's front,
+  // whose structure B already holds the child's), children first; this
+  // shortens the front tree, which is processed height by height
+  {
+    const int n0 = (int)tops.size();
+    std::vector<int> cnt(n0, 0), into(n0, -1);
+    for (int k = 0; k < ls; k++) cnt[front_of[k]]++;
+    for (int f = 0; f < n0; f++) {
+      const std::vector<int> &B = st[tops[f]];
+      if (B.empty() || B[0] >= ls || cnt[f] > kRelaxPivots) continue;
+      const int p = front_of[B[0]];  // not merged yet: p > f
+      into[f] = p;
+      cnt[p] += cnt[f];
+    }
+    std::vector<int> alive_id(n0, -1), ntops;
+    for (int f = 0; f < n0; f++)
+      if (into[f] < 0) alive_id[f] = (int)ntops.size(), ntops.push_back(tops[f]);
+    for (int k = 0; k < ls; k++) {
+      int f = front_of[k];
+      while (into[f] >= 0) f = into[f];
+      front_of[k] = alive_id[f];
+    }
+    tops.swap(ntops);
+  }
+  const int nf = (int)tops.size();
+  L.fnk.assign(nf, 0);
+  for (int k = 0; k < ls; k++) L.fnk[front_of[k]]++;
+  for (int f = 0; f < nf; f++) L.fkp.push_back(L.fkp.back() + L.fnk[f]);
+  L.fk.resize(ls);
+  {
+    std::vector<int> at(L.fkp.begin(), L.fkp.end() - 1);
+    for (int k = 0; k < ls; k++) L.fk[at[front_of[k]]++] = k;  // ascending within a front
+  }
+  auto inK = [&](int f, int x) {
+    return std::binary_search(L.fk.begin() + L.fkp[f], L.fk.begin() + L.fkp[f + 1], x);
+  };
   L.ff.resize(nf);
   L.fpar.assign(nf, -1);
   L.fheight.assign(nf, 0);
   for (int f = 0; f < nf; f++) {
-    const int k0 = L.fk0[f], k1 = k0 + L.fnk[f] - 1;
-    const std::vector<int> &B = st[k1];
+    const std::vector<int> &B = st[tops[f]];
     // containment: every pivot's structure lies in K or in B
-    for (int k = k0; k < k1; k++)
-      for (int j : st[k])
-        if (j > k1 && !std::binary_search(B.begin(), B.end(), j))
+    for (int q = L.fkp[f]; q < L.fkp[f + 1]; q++)
+      for (int j : st[L.fk[q]])
+        if (!inK(f, j) && !std::binary_search(B.begin(), B.end(), j))
           return set_error(SNU_ERR_STRUCTURE, "leaf front structure is not closed (internal error)");
     L.fidx.insert(L.fidx.end(), B.begin(), B.end());
     L.fidx_ptr.push_back((int)L.fidx.size());
@@ -143,11 +184,12 @@ int build_fronts(LeafPlan &L) {
   }
   // local index of core index x inside front f (-1: not in it)
   auto loc = [&](int f, int x) -> int {
-    const int k0 = L.fk0[f], nk = L.fnk[f];
-    if (x >= k0 && x < k0 + nk) return x - k0;
+    auto kb = L.fk.begin() + L.fkp[f], ke = L.fk.begin() + L.fkp[f + 1];
+    auto kt = std::lower_bound(kb, ke, x);
+    if (kt != ke && *kt == x) return (int)(kt - kb);
     auto b = L.fidx.begin() + L.fidx_ptr[f], e = L.fidx.begin() + L.fidx_ptr[f + 1];
     auto it = std::lower_bound(b, e, x);
-    return it != e && *it == x ? nk + (int)(it - b) : -1;
+    return it != e && *it == x ? L.fnk[f] + (int)(it - b) : -1;
   };
   // extend-add maps: each B entry's local index in the parent front
   L.frel.assign(L.fidx.size(), -1);

@@ -21,19 +21,6 @@
 
 namespace snu {
 
-namespace {
-// first index in the ascending array a[0, n) whose value is >= v
-__device__ __forceinline__ int lower_idx(const int *a, int n, int v) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (a[mid] < v) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo;
-}
-}  // namespace
-
 extern __shared__ __align__(16) char front_lds[];
 
 // One workgroup per front, the whole front in LDS (f^2 values).
@@ -102,26 +89,30 @@ void launch_front_small(const FrontDev *fr, int nfront, int max_f, double eps, h
 // Columns [c0, c0 + 64) of one front in global memory: zeroed, the owned
 // slots placed, the children's update columns that land there added (child
 // after child).  Within one workgroup the phases are separated by barriers;
-// different work items touch different columns.
+// different work items touch different columns.  The work item's ranges
+// (its owned slots, each child's columns in the chunk) come from the host.
 template <typename T>
 __global__ __launch_bounds__(256) void k_front_assemble(const FrontDev *__restrict__ fr,
-                                                        const int2 *__restrict__ work) {
-  const int2 w = work[blockIdx.x];
+                                                        const int4 *__restrict__ work,
+                                                        const int *__restrict__ wrange) {
+  const int4 w = work[blockIdx.x];
   const FrontDev D = fr[w.x];
+  const int *rg = wrange + w.z;
   const int f = D.f, c0 = w.y, c1 = min(f, c0 + 64), tid = threadIdx.x, nt = blockDim.x;
   T *F = (T *)D.F;
   const long long base = (long long)c0 * f, cnt = (long long)(c1 - c0) * f;
   for (long long i = tid; i < cnt; i += nt) F[base + i] = T(0);
   __syncthreads();
   const T *vin = (const T *)D.vals;
-  const int q0 = lower_idx(D.pos, D.nslot, c0 * f), q1 = lower_idx(D.pos, D.nslot, c1 * f);
+  const int q0 = rg[0], q1 = rg[1];
   for (int q = q0 + tid; q < q1; q += nt) F[D.pos[q]] = vin[D.slot[q]];
   __syncthreads();
   for (int c = 0; c < D.nchild; c++) {
+    const int b0 = rg[2 + 2 * c], b1 = rg[3 + 2 * c];
+    if (b0 == b1) continue;  // uniform: no column of this child lands here
     const FrontChild C = D.child[c];
     const T *U = (const T *)C.U;
     const int nb = C.nb;
-    const int b0 = lower_idx(C.rel, nb, c0), b1 = lower_idx(C.rel, nb, c1);
     const int tot = nb * (b1 - b0);
     for (int i = tid; i < tot; i += nt) {
       const int a = i % nb, b = b0 + i / nb;
@@ -133,25 +124,25 @@ __global__ __launch_bounds__(256) void k_front_assemble(const FrontDev *__restri
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_front_writeback(const FrontDev *__restrict__ fr,
-                                                         const int2 *__restrict__ work) {
-  const int2 w = work[blockIdx.x];
+                                                         const int4 *__restrict__ work,
+                                                         const int *__restrict__ wrange) {
+  const int4 w = work[blockIdx.x];
   const FrontDev D = fr[w.x];
-  const int f = D.f, c0 = w.y, c1 = min(f, c0 + 64);
   const T *F = (const T *)D.F;
   T *vout = (T *)D.vals;
-  const int q0 = lower_idx(D.pos, D.nslot, c0 * f), q1 = lower_idx(D.pos, D.nslot, c1 * f);
+  const int q0 = wrange[w.z], q1 = wrange[w.z + 1];
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) vout[D.slot[q]] = F[D.pos[q]];
 }
 
 template <typename T>
-void launch_front_assemble(const FrontDev *fr, const int2 *work, int nwork, hipStream_t st) {
+void launch_front_assemble(const FrontDev *fr, const int4 *work, const int *wrange, int nwork, hipStream_t st) {
   if (nwork <= 0) return;
-  hipLaunchKernelGGL(k_front_assemble<T>, dim3(nwork), dim3(256), 0, st, fr, work);
+  hipLaunchKernelGGL(k_front_assemble<T>, dim3(nwork), dim3(256), 0, st, fr, work, wrange);
 }
 template <typename T>
-void launch_front_writeback(const FrontDev *fr, const int2 *work, int nwork, hipStream_t st) {
+void launch_front_writeback(const FrontDev *fr, const int4 *work, const int *wrange, int nwork, hipStream_t st) {
   if (nwork <= 0) return;
-  hipLaunchKernelGGL(k_front_writeback<T>, dim3(nwork), dim3(256), 0, st, fr, work);
+  hipLaunchKernelGGL(k_front_writeback<T>, dim3(nwork), dim3(256), 0, st, fr, work, wrange);
 }
 
 // One 64 x 64 tile of a leaf's S per workgroup: accumulated column by column
@@ -194,8 +185,8 @@ void launch_front_schur(const LeafSchurDev *ls, const int4 *tiles, int ntile, hi
 #define SNU_FRONT_INST(T)                                                                      \
   template int front_small_max<T>();                                                           \
   template void launch_front_small<T>(const FrontDev *, int, int, double, hipStream_t);        \
-  template void launch_front_assemble<T>(const FrontDev *, const int2 *, int, hipStream_t);    \
-  template void launch_front_writeback<T>(const FrontDev *, const int2 *, int, hipStream_t);   \
+  template void launch_front_assemble<T>(const FrontDev *, const int4 *, const int *, int, hipStream_t);  \
+  template void launch_front_writeback<T>(const FrontDev *, const int4 *, const int *, int, hipStream_t); \
   template void launch_front_schur<T>(const LeafSchurDev *, const int4 *, int, hipStream_t);
 SNU_FRONT_INST(double)
 SNU_FRONT_INST(float)

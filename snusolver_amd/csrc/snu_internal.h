@@ -31,9 +31,10 @@ struct LeafPlan {
   std::vector<int> lev_ptr, lev_rows;    // factor / forward level sets
   std::vector<int> ulev_ptr, ulev_rows;  // backward level sets
   double flops = 0;         // structural leaf flops (LU + Schur)
-  // multifrontal form of the leaf factorization (build_fronts), fronts in
-  // ascending pivot order (children before parents):
-  std::vector<int> fk0, fnk, ff;         // first pivot, pivots, front order
+  // multifrontal form of the leaf factorization (build_fronts), fronts
+  // ordered by their largest pivot (children before parents):
+  std::vector<int> fkp, fk;              // pivots K of each front (ascending)
+  std::vector<int> fnk, ff;              // pivots, front order
   std::vector<int> fpar, fheight;        // parent front (-1: root), height
   std::vector<int> fidx_ptr, fidx;       // B of each front: core indices beyond K
   std::vector<int> frel;                 // per B entry: local index in the parent

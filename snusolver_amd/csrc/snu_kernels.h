@@ -234,9 +234,9 @@ int front_small_max();
 // updates) and write-back of the owned slots after the dense partial LU, one
 // workgroup per (front, 64-column chunk) work item.
 template <typename T>
-void launch_front_assemble(const FrontDev *fr, const int2 *work, int nwork, hipStream_t st);
+void launch_front_assemble(const FrontDev *fr, const int4 *work, const int *wrange, int nwork, hipStream_t st);
 template <typename T>
-void launch_front_writeback(const FrontDev *fr, const int2 *work, int nwork, hipStream_t st);
+void launch_front_writeback(const FrontDev *fr, const int4 *work, const int *wrange, int nwork, hipStream_t st);
 // Boundary Schur blocks from the root fronts, one workgroup per 64 x 64 S
 // tile (leaf, first row, first column).
 template <typename T>

@@ -97,9 +97,12 @@ def test_fronts_structure_and_values_vs_oracle(S, oracle_mod):
             ls, cn, _, lnz = M.leaf_sizes(r)
             Fs = M.leaf_fronts(r)
             nf = len(Fs["k0"])
-            # pivots partitioned into contiguous ascending ranges
-            assert Fs["k0"][0] == 0 and np.all(Fs["k0"][1:] == Fs["k0"][:-1] + Fs["nk"][:-1])
-            assert Fs["k0"][-1] + Fs["nk"][-1] == ls
+            # pivots partitioned among the fronts, ascending within each,
+            # fronts ordered by their largest pivot
+            kp, K = Fs["k_ptr"], Fs["k"]
+            assert np.array_equal(np.sort(K), np.arange(ls)) and np.all(np.diff(kp) == Fs["nk"])
+            assert all(np.all(np.diff(K[kp[f]:kp[f + 1]]) > 0) for f in range(nf))
+            assert np.all(np.diff(K[kp[1:] - 1]) > 0) and np.array_equal(Fs["k0"], K[kp[:-1]])
             # every LU slot owned once; children precede their parents
             assert np.array_equal(np.sort(Fs["slot"]), np.arange(lnz))
             par = Fs["parent"]

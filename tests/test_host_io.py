@@ -114,17 +114,17 @@ def test_drivers_usage(name):
 
 def test_wide_leaf_fronts(S):
     """A single leaf of 40,000 rows (2D 5-point grid, np = 1) is factored as
-    fronts: the pivots are partitioned into contiguous ranges, the largest
-    front is of the order of the widest separator (not of the leaf), and
-    more leaves give smaller fronts."""
+    fronts: the pivots are partitioned among the fronts, the largest front is
+    of the order of the widest separator (not of the leaf), and more leaves
+    give smaller fronts."""
     A = S.gen_grid(200, 200, 1, 5, 0, 4.1, 0)
     biggest = []
     for np_ in (1, 4):
         with S.SnuMat(A, np_, grid=(200, 200, 1), device=-1) as M:
             f = M.leaf_fronts(0)
             ls = M.leaf_sizes(0)[0]
-            assert f["k0"][0] == 0 and f["k0"][-1] + f["nk"][-1] == ls
-            assert np.all(f["k0"][1:] == f["k0"][:-1] + f["nk"][:-1])
+            assert np.array_equal(np.sort(f["k"]), np.arange(ls))
+            assert np.all(np.diff(f["k_ptr"]) == f["nk"])
             biggest.append(int(f["f"].max()))
             assert biggest[-1] <= 3 * 200
     assert biggest[0] >= biggest[1]
