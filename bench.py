@@ -4,7 +4,7 @@
 Workload (default): 64^3 7-point convection-diffusion operator, n = 262,144,
 nnz = 1,810,432, diagonal 6.1, off-diagonals -1 + 0.25 (u - 0.5) (seed 0),
 b = A x_true with x_true ~ U[-1,1) (seed 1), FP64, 128 nested-dissection
-leaves of ~1.6-2k rows (geometric ordering).  One step = one numeric
+leaves of ~1.5-2k rows (the library's graph ND, seed 0).  One step = one numeric
 factorization from A's values resident in HBM plus one solve with b resident
 in HBM (the reference's "Factsolve", src/solver.cu:68-76).  Construct
 (ordering, symbolic analysis, upload) is outside the timed region.
@@ -87,7 +87,10 @@ def main():
     _, b1 = S.gen_rhs(A, 1, 1)
     _, b16 = S.gen_rhs(A, 16, 1)
     t0 = time.perf_counter()
-    M = S.SnuMat(A, np_, grid=grid, device=0)
+    order, sizes, order_name = S.config_order(args.config, A, grid, np_)
+    order_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    M = S.SnuMat(A, np_, order=order, sizes=sizes, device=0)
     construct_s = time.perf_counter() - t0
     counts = M.counts()
 
@@ -186,7 +189,7 @@ def main():
         "config": {
             "workload": "64^3 7-point convection-diffusion, FP64, 128 ND leaves, 1 rhs, factorize+solve"
             if args.config == "c2" else "96^3 27-point anisotropic, FP64, 512 ND leaves, 1 rhs",
-            "n": n, "nnz": A.nnz, "leaves": np_, "nrhs": 1, "ordering": "geometric ND",
+            "n": n, "nnz": A.nnz, "leaves": np_, "nrhs": 1, "ordering": order_name,
         },
         "factor_ms": round(tim["factor_ms"], 3),
         "solve_ms": round(tim["solve_ms"], 3),
@@ -211,11 +214,12 @@ def main():
         "solves_per_s_nrhs16": round(solves16, 2),
         "backward_error": berr,
         "backward_error_nrhs16": berr16,
+        "order_s": round(order_s, 3),
         "construct_s": round(construct_s, 3),
         "roofline": roofline,
     }
     if not args.no_cpu:
-        line["cpu_baseline"], xo = cpu_baseline(A, grid, np_, b1[:, 0], args.cpu_threads)
+        line["cpu_baseline"], xo = cpu_baseline(A, order, sizes, np_, b1[:, 0], args.cpu_threads)
         line["rel_diff_vs_oracle"] = float(np.max(np.abs(x - xo)) / np.max(np.abs(xo)))
     legs = ((args.no_precision, "precision", precision_leg), (args.no_batch, "batch", batch_leg),
             (args.no_serve, "serving", lambda S_, np_: serve_leg(S_, np_, args.serve_rate, args.serve_requests)))
@@ -548,7 +552,7 @@ def host_cpu():
     return cores, info
 
 
-def cpu_baseline(A, grid, np_, b, threads):
+def cpu_baseline(A, order, sizes, np_, b, threads):
     """The oracle's MKL host path (reference offlvl = -1 semantics), one full
     factor+solve of the same workload, on every host core this job may use
     (leaves one per thread, dense levels through MKL's threads)."""
@@ -556,10 +560,8 @@ def cpu_baseline(A, grid, np_, b, threads):
     cores = threads or usable
     os.environ["MKL_NUM_THREADS"] = str(cores)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import snusolver_amd as S
     from oracle import Oracle, dense_backend
 
-    order, sizes = S.order_grid(*grid, np_)
     O = Oracle(A, np_, order, sizes)
     t0 = time.perf_counter()
     x = O.factor_solve(b, nthreads=cores)

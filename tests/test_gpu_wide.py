@@ -219,12 +219,17 @@ def test_serve_under_memory_pressure_vs_oracle(S, oracle_mod):
         _vs_oracle(S, oracle_mod, A, np_, o, z, x, b, f"request {q} (matrix {key}, {b.shape[1]} cols)")
 
 
-def test_c2_full_size_vs_oracle(S, oracle_mod):
-    """64^3 7-point, 128 leaves, geometric ND (the benchmark's workload):
-    every leaf's LU against the oracle's, x against the oracle; the
-    factorization runs the 128 x 128 Schur launches."""
+@pytest.mark.parametrize("ordering", ["benchmark", "geometric"])
+def test_c2_full_size_vs_oracle(S, oracle_mod, ordering):
+    """64^3 7-point, 128 leaves, with the benchmark's ordering (the graph
+    ND) and with the geometric ND: every leaf's LU against the oracle's, x
+    against the oracle; the factorization runs the 128 x 128 Schur
+    launches."""
     A, grid, np_ = S.config_matrix("c2")
-    order, sizes = S.order_grid(*grid, np_)
+    if ordering == "benchmark":
+        order, sizes, _ = S.config_order("c2", A, grid, np_)
+    else:
+        order, sizes = S.order_grid(*grid, np_)
     _, b = S.gen_rhs(A, 1, 1)
     b = b[:, 0]
     O = oracle_mod.Oracle(A, np_, order, sizes)
