@@ -16,8 +16,9 @@
 // subtree whose front (its pivots and the structure of its root) has order
 // at most kSubtreeFront becomes one front, however its pivots are numbered;
 // above those, a pivot whose only child is a pivot of the same kind joins
-// that child's front (chains); then fronts with at most kRelaxPivots pivots
-// join their parent fronts.  A front's pivots are eliminated in
+// that child's front (chains); then fronts of order above kSubtreeFront with
+// at most kRelaxPivots pivots join their parent fronts.  A front's pivots
+// are eliminated in
 // ascending order; pivots outside its subtree are neither descendants nor
 // ancestors of its pivots, so they update none of its entries, and the
 // factors are those of the reference's elimination order.  The union of the
@@ -42,8 +43,9 @@
 namespace snu {
 
 namespace {
-constexpr int kSubtreeFront = 128;  // the order of the fronts factored whole in LDS (FP64)
-constexpr int kRelaxPivots = 16;    // fronts with at most this many pivots merge into their parent
+constexpr int kSubtreeFront = 64;   // subtrees collapse into one front up to this order
+constexpr int kRelaxPivots = 32;    // larger thin fronts with at most this many pivots merge into their parent
+constexpr int kThin = 8;            // thin: pivots x kThin <= order
 }
 
 int build_fronts(LeafPlan &L) {
@@ -119,17 +121,22 @@ int build_fronts(LeafPlan &L) {
   for (int k = 0; k < ls; k++)
     if (topof[rep[k]] == k) front_of[k] = (int)tops.size(), tops.push_back(k);
   for (int k = 0; k < ls; k++) front_of[k] = front_of[topof[rep[k]]];
-  // relaxed amalgamation: a front with at most kRelaxPivots pivots joins its
-  // parent front (its pivots become rows and columns of the parent's front,
-  // whose structure B already holds the child's), children first; this
-  // shortens the front tree, which is processed height by height
+  // relaxed amalgamation: a front of order above kSubtreeFront with at most
+  // kRelaxPivots pivots joins its parent front (its pivots become rows and
+  // columns of the parent's front, whose structure B already holds the
+  // child's), children first; this shortens the front tree, which is
+  // processed height by height (each height of the larger fronts costs a
+  // sequence of launches)
   {
     const int n0 = (int)tops.size();
     std::vector<int> cnt(n0, 0), into(n0, -1);
     for (int k = 0; k < ls; k++) cnt[front_of[k]]++;
     for (int f = 0; f < n0; f++) {
       const std::vector<int> &B = st[tops[f]];
-      if (B.empty() || B[0] >= ls || cnt[f] > kRelaxPivots) continue;
+      const long long order = cnt[f] + (long long)B.size();
+      if (B.empty() || B[0] >= ls || cnt[f] > kRelaxPivots || order <= kSubtreeFront ||
+          (long long)cnt[f] * kThin > order)
+        continue;
       const int p = front_of[B[0]];  // not merged yet: p > f
       into[f] = p;
       cnt[p] += cnt[f];
