@@ -70,6 +70,7 @@ struct GemvPlan {
 // (key: height, step within it); the tiles of the boundary Schur blocks.
 struct FrontLevel {
   int small0 = 0, nsmall = 0, max_f = 0;
+  int nsmall_lo = 0, max_f_lo = 0;  // the first nsmall_lo small fronts have order <= kFrontLo
   int work0 = 0, nwork = 0;
 };
 template <typename T>

@@ -294,6 +294,9 @@ std::vector<std::pair<int, int>> nz_runs(int len, F any) {
 // Outer panel width of the blocked LU of the separators' diagonal blocks.
 constexpr int kPanel = 512;
 
+// Small fronts up to this order run in their own launch per height.
+constexpr int kFrontLo = 64;
+
 // Launch keys: tree height (1 = parents of the leaves) major, then the step
 // inside the level.  Matrices of different depths align at equal height.
 inline long long fkey(int height, long long step) { return (long long)height * 100000000LL + step; }
@@ -1016,7 +1019,13 @@ int build_front_plan(snumat *h) {
     FrontLevel &lv = FP.levels[hh];
     lv.small0 = (int)FP.small.size();
     lv.nsmall = (int)sm[hh].size();
-    for (const FrontDev &D : sm[hh]) lv.max_f = std::max(lv.max_f, D.f);
+    // fronts of order <= kFrontLo first: their launch needs a quarter of the
+    // LDS or less, so more of them share a CU
+    std::stable_partition(sm[hh].begin(), sm[hh].end(), [](const FrontDev &D) { return D.f <= kFrontLo; });
+    for (const FrontDev &D : sm[hh]) {
+      if (D.f <= kFrontLo) lv.nsmall_lo++, lv.max_f_lo = std::max(lv.max_f_lo, D.f);
+      lv.max_f = std::max(lv.max_f, D.f);
+    }
     FP.small.insert(FP.small.end(), sm[hh].begin(), sm[hh].end());
     lv.work0 = (int)FP.work.size();
     const int lbase = (int)FP.large.size();
@@ -1142,8 +1151,9 @@ void merge_fronts(const std::vector<const FrontPlan<T> *> &in, FrontPlan<T> &out
       const FrontPlan<T> *p = in[pi];
       if (hh >= p->levels.size()) continue;
       const FrontLevel &pl = p->levels[hh];
-      out.small.insert(out.small.end(), p->small.begin() + pl.small0, p->small.begin() + pl.small0 + pl.nsmall);
+      out.small.insert(out.small.end(), p->small.begin() + pl.small0, p->small.begin() + pl.small0 + pl.nsmall_lo);
       lv.max_f = std::max(lv.max_f, pl.max_f);
+      lv.max_f_lo = std::max(lv.max_f_lo, pl.max_f_lo);
       // this matrix's larger fronts of the height, in the order its work items name them
       int first = -1, last = -1;
       for (int q = pl.work0; q < pl.work0 + pl.nwork; q++) {
@@ -1156,6 +1166,13 @@ void merge_fronts(const std::vector<const FrontPlan<T> *> &in, FrontPlan<T> &out
       for (int q = pl.work0; q < pl.work0 + pl.nwork; q++)
         out.work.push_back(make_int4(base + p->work[q].x - first, p->work[q].y, rbase[pi] + p->work[q].z, 0));
     }
+    lv.nsmall_lo = (int)out.small.size() - lv.small0;
+    for (const FrontPlan<T> *p : in)  // then every matrix's larger small fronts
+      if (hh < p->levels.size()) {
+        const FrontLevel &pl = p->levels[hh];
+        out.small.insert(out.small.end(), p->small.begin() + pl.small0 + pl.nsmall_lo,
+                         p->small.begin() + pl.small0 + pl.nsmall);
+      }
     lv.nsmall = (int)out.small.size() - lv.small0;
     lv.nwork = (int)out.work.size() - lv.work0;
   }
@@ -1560,7 +1577,8 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
   const size_t nl = fp.dense.launches.size();
   for (int hh = 0; hh < (int)fp.levels.size(); hh++) {
     const FrontLevel &lv = fp.levels[hh];
-    launch_front_small<T>(fp.d_small + lv.small0, lv.nsmall, lv.max_f, fs.eps, st);
+    launch_front_small<T>(fp.d_small + lv.small0, lv.nsmall_lo, lv.max_f_lo, fs.eps, st);
+    launch_front_small<T>(fp.d_small + lv.small0 + lv.nsmall_lo, lv.nsmall - lv.nsmall_lo, lv.max_f, fs.eps, st);
     launch_front_assemble<T>(fp.d_large, fp.d_work + lv.work0, fp.d_wrange, lv.nwork, st);
     for (; li < nl && front_key_height(fp.dense.launches[li].key) == hh; li++) {
       int rc = run_timed(fp.dense.launches[li], fp.dense);

@@ -107,16 +107,19 @@ __global__ __launch_bounds__(256) void k_front_assemble(const FrontDev *__restri
   const int q0 = rg[0], q1 = rg[1];
   for (int q = q0 + tid; q < q1; q += nt) F[D.pos[q]] = vin[D.slot[q]];
   __syncthreads();
+  // children: wave w takes the child's columns b0 + w, b0 + w + 4, ...,
+  // its lanes the rows (reads of U run down its columns)
+  const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   for (int c = 0; c < D.nchild; c++) {
     const int b0 = rg[2 + 2 * c], b1 = rg[3 + 2 * c];
     if (b0 == b1) continue;  // uniform: no column of this child lands here
     const FrontChild C = D.child[c];
     const T *U = (const T *)C.U;
     const int nb = C.nb;
-    const int tot = nb * (b1 - b0);
-    for (int i = tid; i < tot; i += nt) {
-      const int a = i % nb, b = b0 + i / nb;
-      F[C.rel[a] + (long long)C.rel[b] * f] += U[a + (long long)b * C.ld];
+    for (int b = b0 + wv; b < b1; b += nw) {
+      T *Fc = F + (long long)C.rel[b] * f;
+      const T *Ub = U + (long long)b * C.ld;
+      for (int a = lane; a < nb; a += 64) Fc[C.rel[a]] += Ub[a];
     }
     __syncthreads();
   }
