@@ -323,6 +323,19 @@ class SnuMat:
     def set_profiling(self, on: bool):
         _check(lib().snumat_set_profiling(self._h, 1 if on else 0))
 
+    def detach(self):
+        """Release the handle's device memory, keeping its symbolic plan
+        (host-only afterwards, as constructed with device=-1)."""
+        _check(lib().snumat_detach(self._h))
+        return self
+
+    def attach(self, device: int = 0, values=None):
+        """Move a host-only or detached handle onto a device with A's
+        values (default: those it was constructed with)."""
+        v = np.ascontiguousarray(self.A.data if values is None else values, np.float64)
+        _check(lib().snumat_attach(self._h, device, _dp(v)))
+        return self
+
     def use_graph(self, on: bool = True):
         """Factorize by replaying a HIP graph of the launch sequence (captured
         on the next factorize)."""

@@ -377,6 +377,28 @@ def C_ref(v):
     return ctypes.byref(v)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_detach_attach_refactorizes_bitwise(S, precision):
+    """A handle detached (device memory released, plan kept) and attached
+    again factorizes and solves to the same bits; a host-only handle
+    attached later gives the same bits too."""
+    A = S.gen_grid(14, 12, 10, 7, 0, 6.1, 3)
+    order, sizes = S.order_nd(A, 8, 0)
+    _, b = S.gen_rhs(A, 2, 1)
+    with S.SnuMat(A, 8, order=order, sizes=sizes, device=0, precision=precision) as M:
+        x0 = M.factorize().solve(b)
+        assert M.info()["on_device"]
+        M.detach()
+        assert not M.info()["on_device"]
+        with pytest.raises(S.SnuError):
+            M.factorize()
+        x1 = M.attach(0).factorize().solve(b)
+    with S.SnuMat(A, 8, order=order, sizes=sizes, device=-1, precision=precision) as H:
+        x2 = H.attach(0).factorize().solve(b)
+    assert np.array_equal(x0, x1) and np.array_equal(x0, x2)
+    assert S.backward_error(A, x0, b) <= BERR_TOL
+
+
 def test_graph_replays_equal_eager(S):
     """Factorizations replayed from a captured HIP graph give the eager
     results on every replay, with eager work and synchronisations between
