@@ -199,10 +199,12 @@ namespace {
 
 // A grouped C -= sum A B launch, split by tile size: problems whose size
 // bucket the autotuner gave to the 128 x 128 kernel take it when the launch
-// holds enough of those tiles to fill the device twice over (fewer would
-// leave CUs idle that 64 x 64 tiles keep busy); the rest take 64 x 64 tiles
-// (launch kind 5 and kind 2).
-constexpr int kMin128Tiles = 512;
+// holds at least kMin128Tiles of those tiles (16 per CU: on smaller launches
+// the 64 x 64 tiles, four times as many and more workgroups per CU, balance
+// better: 64^3 factor 37.2 -> 35.1 ms, 96^3 FP64 440 -> 433 ms against a
+// threshold of 512, measured); the rest take 64 x 64 tiles (launch kind 5
+// and kind 2).
+constexpr int kMin128Tiles = 4096;
 template <typename T>
 struct GemmBatch {
   struct Pend {
