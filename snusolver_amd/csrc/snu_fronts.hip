@@ -72,9 +72,13 @@ __global__ __launch_bounds__(256) void k_front_small(const FrontDev *__restrict_
     for (int r = nk + lane; r < f; r += 64) G[r + (long long)c * f] = F[r + c * f];
 }
 
+// Largest front factored whole in LDS (order): above it a front goes
+// through HBM and the dense kernels.  Measured 64 / 80 / 96 / 128 (FP64) on
+// the 64^3, 96^3 and batch workloads: smaller LDS fronts keep more
+// workgroups per CU, and the larger ones are faster on MFMA.
 template <typename T>
 int front_small_max() {
-  return sizeof(T) == 8 ? 128 : 160;
+  return sizeof(T) == 8 ? 64 : 96;
 }
 
 template <typename T>

@@ -185,7 +185,7 @@ void launch_assemble_schur(int ntile, const int *tile_blk, const int *tile_row0,
 // tiles of unequal K (problems of one launch differ) spread evenly over the
 // XCDs instead of one XCD receiving a whole run of long tiles.  The last
 // partial round of 8 chunks keeps the identity order.
-constexpr int kXcdChunk = 32;
+constexpr int kXcdChunk = 16;
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   const int full = nb / (8 * kXcdChunk) * (8 * kXcdChunk);
   if (bid >= full) return bid;
