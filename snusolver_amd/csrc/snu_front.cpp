@@ -43,9 +43,9 @@
 namespace snu {
 
 namespace {
-constexpr int kSubtreeFront = 64;   // subtrees collapse into one front up to this order
+constexpr int kSubtreeFront = 48;   // subtrees collapse into one front up to this order
 constexpr int kRelaxPivots = 32;    // larger thin fronts with at most this many pivots merge into their parent
-constexpr int kThin = 8;            // thin: pivots x kThin <= order
+constexpr int kThin = 4;            // thin: pivots x kThin <= order
 }
 
 int build_fronts(LeafPlan &L) {
