@@ -67,7 +67,7 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {max(args.gpus, world)}); "
+        print(f"bench.py: this library runs on one GPU only (asked for {max(args.gpus, world)}); "
               "the separator tree is not sharded across GPUs, so there is no multi-GPU figure.",
               file=sys.stderr)
         sys.exit(2)
