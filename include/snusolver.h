@@ -83,12 +83,22 @@ typedef struct {
                       factorized on first need, kept until the next
                       factorize; about the memory of an FP64 handle) */
   double ir_tol;   /* SNU_FP32: target backward error per column (1e-15) */
+  int fp32_gemm;   /* SNU_FP32: arithmetic of the factorization's 128 x 128
+                      GEMM tiles.  SNU_GEMM_SPLIT (default): each FP32 operand
+                      as three bf16 pieces (h + m + l == x exactly), the six
+                      products of weight >= 2^-16 on the bf16 MFMA, FP32
+                      accumulation: FP32-accurate (terms left out <= about
+                      2^-23 |a||b| per product).  SNU_GEMM_F32: the FP32 MFMA.
+                      The reference's FP32 path is cuBLAS SGEMM. */
 } snu_opts_t;
+
+#define SNU_GEMM_SPLIT 0
+#define SNU_GEMM_F32 1
 
 typedef struct snumat snumat_t;
 
 /* Fills opts with the defaults: np = 4, ND ordering, eps = 1e-15, FP64,
- * ir_max_steps = 10, ir_tol = 1e-15. */
+ * ir_max_steps = 10, ir_tol = 1e-15, fp32_gemm = SNU_GEMM_SPLIT. */
 void snu_default_opts(snu_opts_t *opts);
 
 /* Selects the device and checks that it is usable (<- initialize()). */
@@ -368,7 +378,9 @@ int snumat_use_graph(snumat_t *h, int on);
  * library to the parity tests. */
 /* assign = 0: C -= A B (64 x 64 tiles), 2: C -= A B (128 x 128 tiles),
  * 1: C = A B, 3: C = -A B, 4: C = A B (128 x 128 tiles), 5: C = -A B
- * (128 x 128 tiles); col-major. */
+ * (128 x 128 tiles); col-major.  snu_test_gemm_f32 also takes 6: C -= A B,
+ * 7: C = A B, 8: C = -A B through the FP32 bf16-split kernel (128 x 128
+ * tiles, as SNU_GEMM_SPLIT). */
 int snu_test_gemm(int M, int N, int K, const double *A, int lda,
                   const double *B, int ldb, double *C, int ldc, int assign);
 int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,
@@ -376,7 +388,8 @@ int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,
 /* average time (ms) of the grouped C -= A B kernel on device-resident
  * operands, K split into nseg concatenated segments; tile 64 or 128 (K chunk
  * 16), 6432 / 12832 (K chunk 32) or 642 / 1282 (K chunk 16, loads two chunks
- * ahead); prec SNU_FP64 or SNU_FP32. */
+ * ahead), or 1283 (SNU_FP32 only: 128 x 128 through bf16 splits); prec
+ * SNU_FP64 or SNU_FP32. */
 int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                    double *ms_out);
 /* tiles3: 64 or 128, the tile of the grouped C -= A B kernel per size bucket

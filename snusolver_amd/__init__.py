@@ -33,6 +33,7 @@ LIB_PATH = os.path.join(_HERE, "libsnusolver.so")
 
 ORDER_ND, ORDER_GRID, ORDER_GIVEN = 0, 1, 2
 FP64, FP32 = 0, 1
+GEMM_SPLIT, GEMM_F32 = 0, 1
 _STATUS = {
     -1: "BAD_INPUT", -2: "STRUCTURE", -3: "ZERO_DIAG", -4: "DUPLICATE",
     -5: "NO_DEVICE", -6: "HIP", -7: "OOM", -8: "NOT_FACTORED", -9: "NUMERIC",
@@ -56,7 +57,8 @@ class _Opts(C.Structure):
     _fields_ = [("np", C.c_int), ("ordering", C.c_int), ("grid", C.c_int * 3),
                 ("order", C.POINTER(C.c_int)), ("sizes", C.POINTER(C.c_int)),
                 ("eps", C.c_double), ("precision", C.c_int), ("device", C.c_int),
-                ("seed", C.c_uint64), ("ir_max_steps", C.c_int), ("ir_tol", C.c_double)]
+                ("seed", C.c_uint64), ("ir_max_steps", C.c_int), ("ir_tol", C.c_double),
+                ("fp32_gemm", C.c_int)]
 
 
 _lib = None
@@ -242,9 +244,13 @@ class SnuMat:
 
     def __init__(self, A: CSR, np_: int = 4, ordering: str = "nd", grid=None,
                  order=None, sizes=None, device: int = 0, eps: float = 1e-15, seed: int = 0,
-                 precision: str = "fp64", ir_max_steps: int = 10, ir_tol: float = 1e-15):
+                 precision: str = "fp64", ir_max_steps: int = 10, ir_tol: float = 1e-15,
+                 fp32_gemm: str = "split"):
         """precision "fp64": the reference's arithmetic; "fp32": FP32 factors
-        and sweeps with FP64 iterative refinement (ir_max_steps, ir_tol)."""
+        and sweeps with FP64 iterative refinement (ir_max_steps, ir_tol).
+        fp32_gemm ("fp32" only): "split", the 128 x 128 GEMM tiles as
+        three-piece bf16 splits on the bf16 MFMA (FP32-accurate), or "f32",
+        the FP32 MFMA."""
         self.A = A
         o = _Opts()
         lib().snu_default_opts(C.byref(o))
@@ -257,6 +263,9 @@ class SnuMat:
         o.precision = FP32 if precision == "fp32" else FP64
         o.ir_max_steps = ir_max_steps
         o.ir_tol = ir_tol
+        if fp32_gemm not in ("split", "f32"):
+            raise ValueError(fp32_gemm)
+        o.fp32_gemm = GEMM_SPLIT if fp32_gemm == "split" else GEMM_F32
         self.precision = precision
         self._keep = []
         if order is not None:

@@ -130,6 +130,7 @@ extern "C" void snu_default_opts(snu_opts_t *o) {
   o->seed = 0;
   o->ir_max_steps = 10;
   o->ir_tol = 1e-15;
+  o->fp32_gemm = SNU_GEMM_SPLIT;
 }
 
 extern "C" const char *snu_last_error(void) { return g_err.c_str(); }
@@ -182,6 +183,8 @@ extern "C" int snumat_construct(const snu_csr_t *A, const snu_opts_t *opts,
   if (!out) return set_error(SNU_ERR_BAD_INPUT, "null output handle");
   *out = nullptr;
   if (!opts) return set_error(SNU_ERR_BAD_INPUT, "null options");
+  if (opts->fp32_gemm != SNU_GEMM_SPLIT && opts->fp32_gemm != SNU_GEMM_F32)
+    return set_error(SNU_ERR_BAD_INPUT, "fp32_gemm must be SNU_GEMM_SPLIT or SNU_GEMM_F32");
   snumat *h = new snumat();
   h->opts = *opts;
   h->prec = opts->precision;
@@ -303,6 +306,7 @@ int factorize_values(snumat *h, int stage) {
   fs.wd = &h->wd;
   fs.fronts = &vals<T>(h).fronts;
   fs.eps = h->opts.eps;
+  fs.split = h->opts.fp32_gemm == SNU_GEMM_SPLIT;
   static const DensePlan<T> empty;
   double gms[5] = {0, 0, 0, 0, 0};
   int rc = run_factor<T>(fs, stage >= 2 ? VV.fact : empty, h->stream, &h->ev, h->profile && stage >= 2,
@@ -918,7 +922,8 @@ namespace {
 template <typename T>
 int test_gemm(int M, int N, int K, const T *A, int lda, const T *B, int ldb, T *C, int ldc,
               int assign) {
-  if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldb < std::max(1, K) || ldc < std::max(1, M))
+  if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldb < std::max(1, K) || ldc < std::max(1, M) ||
+      assign < 0 || assign > (std::is_same<T, float>::value ? 8 : 5))
     return set_error(SNU_ERR_BAD_INPUT, "bad gemm shape");
   T *dA, *dB, *dC;
   size_t na = (size_t)lda * std::max(K, 1), nb = (size_t)ldb * std::max(N, 1), nc = (size_t)ldc * std::max(N, 1);
@@ -941,7 +946,10 @@ int test_gemm(int M, int N, int K, const T *A, int lda, const T *B, int ldb, T *
   if (!gp.empty()) {
     HIPCHK(hipMemcpy(dgp, gp.data(), sizeof(GemmProbT<T>) * gp.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dgs, gs.data(), sizeof(GemmSegT<T>) * gs.size(), hipMemcpyHostToDevice));
-    if (assign == 2 || assign >= 4)
+    if (assign >= 6) {  // FP32 only (checked above)
+      if constexpr (std::is_same<T, float>::value)
+        launch_gemm128_split(dgp, (int)gp.size(), dgs, tiles, assign - 6, nullptr);
+    } else if (assign == 2 || assign >= 4)
       launch_gemm128<T>(dgp, (int)gp.size(), dgs, tiles, assign == 4 ? 1 : assign == 5 ? 2 : 0, nullptr);
     else launch_gemm<T>(dgp, (int)gp.size(), dgs, tiles, assign == 1 ? 1 : assign == 3 ? 2 : 0, nullptr);
     HIPCHK(hipGetLastError());
@@ -981,7 +989,9 @@ int test_diag_lu(int nb, T *D, int ld, T *Linv, T *Uinv) {
 
 // C = C - A B (assign = 0), C = A B (1), C -= A B with 128 x 128 tiles (2),
 // C = -A B (3), C = A B with 128 x 128 tiles (4), C = -A B with 128 x 128
-// tiles (5), one segment, through the grouped MFMA kernel; col-major.
+// tiles (5), one segment, through the grouped MFMA kernel; col-major.  FP32
+// also: C -= A B (6), C = A B (7), C = -A B (8) through the bf16-split
+// 128 x 128 kernel.
 extern "C" int snu_test_gemm(int M, int N, int K, const double *A, int lda,
                              const double *B, int ldb, double *C, int ldc,
                              int assign) {
@@ -1041,7 +1051,7 @@ extern "C" int snumat_set_profiling(snumat_t *h, int on) {
 // time over reps launches (HIP events).
 namespace {
 template <typename T>
-int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int reps,
+int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, bool split, int reps,
                double *ms_out) {
   T *dA, *dB, *dC;
   HIPCHK(hipMalloc((void **)&dA, sizeof(T) * (size_t)M * K));
@@ -1068,7 +1078,15 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int r
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  auto run = [&]() { launch_gemm_variant<T>(dgp, 1, dgs, tiles, tile, kb, depth, nullptr); };
+  auto run = [&]() {
+    if constexpr (std::is_same<T, float>::value) {
+      if (split) {
+        launch_gemm128_split(dgp, 1, dgs, tiles, 0, nullptr);
+        return;
+      }
+    }
+    launch_gemm_variant<T>(dgp, 1, dgs, tiles, tile, kb, depth, nullptr);
+  };
   run();
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipEventRecord(e0, nullptr));
@@ -1091,15 +1109,18 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, int r
 extern "C" int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                               double *ms_out) {
   // tile: 64 or 128 (K chunk 16), 6432 / 12832 (K chunk 32), 642 / 1282
-  // (K chunk 16, loads two chunks ahead)
+  // (K chunk 16, loads two chunks ahead), 1283 (FP32: 128 x 128 through
+  // bf16 splits)
   int t = tile, kb = 16, depth = 1;
+  const bool split = tile == 1283;
   if (tile == 6432 || tile == 12832) t = tile / 100, kb = 32;
-  if (tile == 642 || tile == 1282) t = tile / 10, depth = 2;
+  if (tile == 642 || tile == 1282 || split) t = tile / 10, depth = 2;
+  if (split) depth = 1;
   if (M < 1 || N < 1 || K < 1 || nseg < 1 || reps < 1 || (t != 64 && t != 128) ||
-      (prec != SNU_FP64 && prec != SNU_FP32) || !ms_out)
+      (prec != SNU_FP64 && prec != SNU_FP32) || (split && prec != SNU_FP32) || !ms_out)
     return set_error(SNU_ERR_BAD_INPUT, "bad gemm bench shape");
-  return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, t, kb, depth, reps, ms_out)
-                          : bench_gemm<double>(M, N, K, nseg, t, kb, depth, reps, ms_out);
+  return prec == SNU_FP32 ? bench_gemm<float>(M, N, K, nseg, t, kb, depth, split, reps, ms_out)
+                          : bench_gemm<double>(M, N, K, nseg, t, kb, depth, false, reps, ms_out);
 }
 
 // Average time (ms) of a plain device copy of `bytes` (multiple of 16) over

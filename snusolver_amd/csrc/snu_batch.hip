@@ -189,6 +189,7 @@ int batch_factorize(snubatch *b) {
   fs.wd = &b->wd;
   fs.fronts = std::is_same<T, double>::value ? (const void *)&b->fr64 : (const void *)&b->fr32;
   fs.eps = b->mats[0]->opts.eps;
+  fs.split = b->mats[0]->opts.fp32_gemm == SNU_GEMM_SPLIT;
   const DensePlan<T> &plan = std::is_same<T, double>::value ? (const DensePlan<T> &)b->f64
                                                             : (const DensePlan<T> &)b->f32;
   int rc = run_factor<T>(fs, plan, b->stream, &b->ev, false, nullptr, nullptr,

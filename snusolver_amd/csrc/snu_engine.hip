@@ -21,6 +21,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "snu_engine.h"
 
@@ -1520,7 +1521,13 @@ namespace {
 template <typename T>
 void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
                 const DiagProbT<T> *dp, const GemvProbT<T> *vp, const GemvSegT<T> *vs,
-                T *part, hipStream_t st) {
+                T *part, hipStream_t st, bool split = false) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (split && (L.kind == 5 || L.kind == 10 || L.kind == 11)) {
+      launch_gemm128_split(gp + L.p0, L.np, gs, L.tiles, L.kind == 5 ? 0 : L.kind == 10 ? 1 : 2, st);
+      return;
+    }
+  }
   switch (L.kind) {
     case 0: launch_diag_lu(dp + L.p0, L.np, st); break;
     case 1: launch_gemm(gp + L.p0, L.np, gs, L.tiles, 1, st); break;
@@ -1571,7 +1578,7 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
   auto run_timed = [&](const Launch &L, const DensePlan<T> &dp) -> int {
     const bool gm = prof && is_gemm(L.kind);
     if (gm) HIPCHK(hipEventRecord((*pev)[2 * g], st));
-    run_launch<T>(L, dp.d_gp, dp.d_gs, dp.d_dp, nullptr, nullptr, nullptr, st);
+    run_launch<T>(L, dp.d_gp, dp.d_gs, dp.d_dp, nullptr, nullptr, nullptr, st, fs.split);
     if (gm) HIPCHK(hipEventRecord((*pev)[2 * g++ + 1], st));
     return SNU_OK;
   };

@@ -182,6 +182,10 @@ void launch_gemm(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
 template <typename T>
 void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                     int total_tiles, int mode, hipStream_t st);
+// FP32 128 x 128 tiles through three-piece bf16 splits on the bf16 MFMA
+// (FP32-accurate; modes as launch_gemm128, same tile counts)
+void launch_gemm128_split(const GemmProbT<float> *probs, int nprob, const GemmSegT<float> *segs,
+                          int total_tiles, int mode, hipStream_t st);
 template <typename T>
 void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                          int total_tiles, int tile, int kb, int depth, hipStream_t st);

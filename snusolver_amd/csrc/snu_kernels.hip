@@ -468,6 +468,229 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
         }
   }
 }
+// ---------------------------------------------------------------------------
+// FP32 grouped GEMM on the bf16 MFMA (v_mfma_f32_16x16x32_bf16)
+// ---------------------------------------------------------------------------
+// Every FP32 operand x is split into three bf16 pieces with x = h + m + l
+// exactly: h = bf16(x), m = bf16(x - h), l = bf16(x - h - m), each rounded
+// to nearest even.  x - h and x - h - m are exact in FP32, |m| <= 2^-8 |x|
+// and |l| <= 2^-16 |x|, and x - h - m has at most 8 significant bits, so l
+// is exact as well.  C accumulates in FP32 the six products of pieces of
+// weight >= 2^-16, hh + hm + mh + hl + mm + lh (each bf16 product is exact
+// in FP32); the three left out (ml, lm, ll) total at most about
+// 2^-23 |a| |b|, two units of FP32's own rounding of one product.  Per
+// 16 x 16 x 32 block that is six bf16 MFMAs (16 cycles each) in place of
+// eight 16x16x4 f32 MFMAs (32 cycles each).
+// Workgroup tile 128 x 128, 4 waves in 2 x 2, each 64 x 64; K chunk 32
+// (one bf16 MFMA deep).  LDS holds one chunk, pieces per row of A and per
+// column of B as [h 0..31 | m 0..31 | l 0..31 | pad 8] bf16 (52 dwords: the
+// 16-byte reads of 8 consecutive rows fall on disjoint banks); the next
+// chunk is in flight in registers during the MFMA block, and two workgroups
+// share a CU, so one splits and stores while the other multiplies.
+// Tiles, problems, modes, aliasing and the XCD order are those of k_gemm
+// with 128 x 128 tiles (the same tile counts).
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float fl2_t __attribute__((ext_vector_type(2)));
+constexpr int kSplitRow = 104;  // bf16 per LDS row: 3 pieces x 32 + 8
+// pieces of two FP32 values as packed bf16 pairs (element 0 in the low half)
+__device__ __forceinline__ void split_pair(float x0, float x1, unsigned &h, unsigned &m, unsigned &l) {
+  const fl2_t x = fl2_t{x0, x1};
+  const bf16x2_t hb = __builtin_convertvector(x, bf16x2_t);
+  const fl2_t r = x - __builtin_convertvector(hb, fl2_t);
+  const bf16x2_t mb = __builtin_convertvector(r, bf16x2_t);
+  const fl2_t r2 = r - __builtin_convertvector(mb, fl2_t);
+  const bf16x2_t lb = __builtin_convertvector(r2, bf16x2_t);
+  h = __builtin_bit_cast(unsigned, hb);
+  m = __builtin_bit_cast(unsigned, mb);
+  l = __builtin_bit_cast(unsigned, lb);
+}
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void k_gemm_split(const GemmProbT<float> *__restrict__ probs,
+                                                       int nprob,
+                                                       const GemmSegT<float> *__restrict__ segs) {
+  constexpr int BMB = 4, BNB = 4, BM = 128, BN = 128, KB = 32;
+  constexpr int RW = kSplitRow / 2;  // dwords per LDS row
+  __shared__ __align__(16) unsigned As[BM * RW];
+  __shared__ __align__(16) unsigned Bs[BN * RW];
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  int lo = 0, hi = nprob - 1;
+  if (nprob == (int)gridDim.x) lo = hi = tile;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].tile0 <= tile) lo = mid;
+    else hi = mid - 1;
+  }
+  const GemmProbT<float> P = probs[lo];
+  constexpr int GM = 8;
+  const int lt = tile - P.tile0;
+  const int tiles_m = (P.M + BM - 1) / BM;
+  const int grp = lt / (GM * P.tiles_n), first = grp * GM;
+  const int gsz = min(GM, tiles_m - first), li = lt - grp * GM * P.tiles_n;
+  const int m0 = (first + li % gsz) * BM, n0 = (li / gsz) * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int l16 = lane & 15, lk = lane >> 4;
+  f4 acc[BMB][BNB];
+#pragma unroll
+  for (int a = 0; a < BMB; a++)
+#pragma unroll
+    for (int b = 0; b < BNB; b++) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  // A chunk: row m0 + am, k = 16 ak + 0..15 (coalesced over rows); B chunk:
+  // columns n0 + bc + 16 i (i < 8), k = 2 bp, 2 bp + 1 (a lane pair's two
+  // loads cover 8 bytes, 16 lanes one 128-byte column segment).  Addresses
+  // are clamped into the operands as in k_gemm; the K tail of B is zeroed
+  // before the split.
+  const int am = tid & 127, ak = tid >> 7, bp = tid & 15, bc = tid >> 4;
+  float xa[16], xb[16];
+  auto load = [&](int sgi, int kk0) {
+    const GemmSegT<float> S = segs[P.seg0 + sgi];
+    const auto pa = gptr(S.A) + min(m0 + am, P.M - 1);
+    if (kk0 + KB <= S.K) {
+      const auto p = pa + (long long)(kk0 + 16 * ak) * S.lda;
+#pragma unroll
+      for (int i = 0; i < 16; i++) xa[i] = p[i * S.lda];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 16; i++) xa[i] = pa[(long long)min(kk0 + 16 * ak + i, S.K - 1) * S.lda];
+    }
+    const auto pb = gptr(S.B);
+    if (kk0 + KB <= S.K && n0 + BN <= P.N) {
+      const auto p = pb + kk0 + 2 * bp + (long long)(n0 + bc) * S.ldb;
+      const long long st = 16 * S.ldb;
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        xb[2 * i] = p[i * st];
+        xb[2 * i + 1] = p[i * st + 1];
+      }
+    } else {
+      const int k0c = min(kk0 + 2 * bp, S.K - 1), k1c = min(kk0 + 2 * bp + 1, S.K - 1);
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const auto p = pb + (long long)min(n0 + bc + 16 * i, P.N - 1) * S.ldb;
+        xb[2 * i] = p[k0c];
+        xb[2 * i + 1] = p[k1c];
+      }
+    }
+  };
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  auto store = [&](int kvalid) {
+    unsigned h[8], m[8], l[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) split_pair(xa[2 * j], xa[2 * j + 1], h[j], m[j], l[j]);
+    unsigned *ra = As + am * RW + 8 * ak;
+    *(u4 *)(ra) = u4{h[0], h[1], h[2], h[3]};
+    *(u4 *)(ra + 4) = u4{h[4], h[5], h[6], h[7]};
+    *(u4 *)(ra + 16) = u4{m[0], m[1], m[2], m[3]};
+    *(u4 *)(ra + 20) = u4{m[4], m[5], m[6], m[7]};
+    *(u4 *)(ra + 32) = u4{l[0], l[1], l[2], l[3]};
+    *(u4 *)(ra + 36) = u4{l[4], l[5], l[6], l[7]};
+    const bool z0 = 2 * bp >= kvalid, z1 = 2 * bp + 1 >= kvalid;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      unsigned hb, mb, lb;
+      split_pair(z0 ? 0.f : xb[2 * i], z1 ? 0.f : xb[2 * i + 1], hb, mb, lb);
+      unsigned *rb = Bs + (bc + 16 * i) * RW + bp;
+      rb[0] = hb;
+      rb[16] = mb;
+      rb[32] = lb;
+    }
+  };
+  auto advance = [&](int &sgi, int &kk0) {
+    kk0 += KB;
+    while (sgi < P.nseg && kk0 >= segs[P.seg0 + sgi].K) {
+      sgi++;
+      kk0 = 0;
+    }
+  };
+  // lane l16 + 16 lk supplies row (column) l16 of its block with k slots
+  // 8 lk .. 8 lk + 7 of every piece, for both operands alike
+  auto compute = [&]() {
+    bf16x8_t av[BMB][3];
+#pragma unroll
+    for (int a = 0; a < BMB; a++)
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+        av[a][p] = __builtin_bit_cast(bf16x8_t, *(const u4 *)(As + (wm + a * 16 + l16) * RW + 16 * p + 4 * lk));
+#pragma unroll
+    for (int b = 0; b < BNB; b++) {
+      bf16x8_t bv[3];
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+        bv[p] = __builtin_bit_cast(bf16x8_t, *(const u4 *)(Bs + (wn + b * 16 + l16) * RW + 16 * p + 4 * lk));
+#pragma unroll
+      for (int a = 0; a < BMB; a++) {
+        // small terms first: l h, m m, h l, m h, h m, h h
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[2], av[a][0], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[1], av[a][1], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0], av[a][2], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[1], av[a][0], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0], av[a][1], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bv[0], av[a][0], acc[a][b], 0, 0, 0);
+      }
+    }
+  };
+  if (P.nseg == 0) return;
+  int sg = 0, k0 = 0;
+  load(sg, k0);
+  store(segs[P.seg0].K - k0);
+  __syncthreads();
+  for (;;) {
+    int nsg = sg, nk0 = k0;
+    advance(nsg, nk0);
+    const bool more = nsg < P.nseg;
+    if (more) load(nsg, nk0);
+    compute();
+    if (!more) break;
+    __syncthreads();
+    store(segs[P.seg0 + nsg].K - nk0);
+    __syncthreads();
+    sg = nsg;
+    k0 = nk0;
+  }
+  // D = B^T A^T: register r of lane l holds C[l % 16][4 (l / 16) + r]
+#pragma unroll
+  for (int a0 = 0; a0 < BMB; a0 += 2) {
+    float cold[2][BNB][4];
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < BNB; b++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int m = m0 + wm + (a0 + a) * 16 + l16, n = n0 + wn + b * 16 + 4 * lk + r;
+            cold[a][b][r] = (m < P.M && n < P.N) ? gptr(P.C)[m + (long long)n * P.ldc] : 0.f;
+          }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < BNB; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int m = m0 + wm + (a0 + a) * 16 + l16, n = n0 + wn + b * 16 + 4 * lk + r;
+          if (m < P.M && n < P.N) {
+            auto c = gptr(P.C) + m + (long long)n * P.ldc;
+            const float v = acc[a0 + a][b][r];
+            if (MODE == 1) *c = v;
+            else if (MODE == 2) *c = -v;
+            else *c = cold[a][b][r] - v;
+          }
+        }
+  }
+}
+void launch_gemm128_split(const GemmProbT<float> *probs, int nprob, const GemmSegT<float> *segs,
+                          int total_tiles, int mode, hipStream_t st) {
+  if (total_tiles <= 0 || nprob <= 0) return;
+  if (mode == 1)
+    hipLaunchKernelGGL(k_gemm_split<1>, dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  else if (mode == 2)
+    hipLaunchKernelGGL(k_gemm_split<2>, dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  else
+    hipLaunchKernelGGL(k_gemm_split<0>, dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+}
+
 // Grouped 2-D copy over GEMM descriptors: C (ldc) = A of the problem's first
 // segment (lda), one workgroup per 64 x 64 tile, rows contiguous per wave.
 template <typename T>

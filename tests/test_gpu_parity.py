@@ -168,6 +168,42 @@ def test_gemm_kernel_f32_vs_numpy(S):
             assert np.all(np.abs(Cg - ref) <= bound), (M, N, K, assign, np.max(np.abs(Cg - ref) / bound))
 
 
+def test_gemm_kernel_split_vs_numpy(S):
+    """FP32 GEMM through three-piece bf16 splits on the bf16 MFMA (the
+    SNU_GEMM_SPLIT form of the FP32 factorization's 128 x 128 tiles)
+    against an FP64 product of the same FP32 operands, elementwise within
+    (K + 4) u32 (|A||B| + |C|): the FP32 dot-product bound plus two units for
+    the piece products left out.  A bf16-accurate product (one piece) misses
+    this bound by about 2^15, and a mismatched k order of the two operands by
+    far more.  Operands span 2^+-20 so the splits see every exponent range."""
+    import ctypes as C
+    L = S.lib()
+    rng = np.random.default_rng(4)
+    u32 = 2.0 ** -24
+    for (M, N, K) in [(1, 1, 1), (128, 128, 32), (65, 63, 17), (130, 7, 200), (5, 200, 64),
+                      (257, 129, 3), (300, 260, 100), (129, 383, 64), (1000, 999, 33), (256, 256, 1024)]:
+        A = rng.standard_normal((M, K)) * 2.0 ** rng.integers(-20, 21, (M, K))
+        B = rng.standard_normal((K, N)) * 2.0 ** rng.integers(-20, 21, (K, N))
+        A = np.asfortranarray(A.astype(np.float32))
+        B = np.asfortranarray(B.astype(np.float32))
+        C0 = np.asfortranarray(rng.standard_normal((M, N)).astype(np.float32))
+        A64, B64, C64 = A.astype(np.float64), B.astype(np.float64), C0.astype(np.float64)
+        absprod = np.abs(A64) @ np.abs(B64)
+        P = A64 @ B64
+        for assign in (6, 7, 8):
+            Cg = C0.copy(order="F")
+            rc = L.snu_test_gemm_f32(M, N, K, A.ctypes.data_as(C.c_void_p), M, B.ctypes.data_as(C.c_void_p), K,
+                                     Cg.ctypes.data_as(C.c_void_p), M, assign)
+            assert rc == 0
+            ref = P if assign == 7 else (-P if assign == 8 else C64 - P)
+            bound = (K + 4) * u32 * (absprod + (np.abs(C64) if assign == 6 else 0)) + 1e-30
+            assert np.all(np.abs(Cg - ref) <= bound), (M, N, K, assign, np.max(np.abs(Cg - ref) / bound))
+    # the FP64 hook has no split form
+    z = np.zeros(1)
+    assert L.snu_test_gemm(1, 1, 1, z.ctypes.data_as(C.c_void_p), 1, z.ctypes.data_as(C.c_void_p), 1,
+                           z.ctypes.data_as(C.c_void_p), 1, 6) != 0
+
+
 def test_diag_lu_kernel_f32(S):
     import ctypes as C
     L = S.lib()

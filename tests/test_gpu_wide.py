@@ -250,6 +250,31 @@ def test_c2_full_size_vs_oracle(S, oracle_mod, ordering):
     assert rel <= REL_TOL and berr <= BERR_TOL
 
 
+@pytest.mark.parametrize("fp32_gemm", ["split", "f32"])
+def test_c2_full_size_fp32_vs_oracle(S, oracle_mod, fp32_gemm):
+    """64^3 7-point, 128 leaves, benchmark ordering, FP32 factors + FP64
+    refinement with both forms of the FP32 128 x 128 GEMM tiles (bf16
+    splits, FP32 MFMA): x against the oracle's FP64 solution."""
+    A, grid, np_ = S.config_matrix("c2")
+    order, sizes, _ = S.config_order("c2", A, grid, np_)
+    _, b = S.gen_rhs(A, 1, 1)
+    b = b[:, 0]
+    O = oracle_mod.Oracle(A, np_, order, sizes)
+    xo = O.factor_solve(b, NT)
+    O.close()
+    with S.SnuMat(A, np_, order=order, sizes=sizes, device=0, precision="fp32", ir_tol=1e-12,
+                  fp32_gemm=fp32_gemm) as M:
+        x = M.factorize().solve(b)
+        split = M.gemm_split()
+        steps = M.timings()["ir_steps"]
+    print(f"  128x128 launches {split['t128_launches']:.0f}, refinement steps {steps}")
+    assert split["t128_launches"] > 0
+    rel = _rel(x, xo)
+    berr = S.backward_error(A, x, b)
+    print(f"  rel diff vs oracle {rel:.2e}  backward error {berr:.2e}")
+    assert rel <= REL_TOL and berr <= BERR_TOL
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_many_leaves_27pt_vs_oracle(S, oracle_mod, precision):
     """27-point anisotropic operator (the precision workload's recipe) on a
