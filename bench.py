@@ -249,18 +249,21 @@ def profiled_traffic(tile):
 
 def precision_leg(S, np, reps=3):
     """96^3 27-point anisotropic operator: plain FP64 vs FP32 factors with
-    FP64 refinement (factor and solve times are the best of reps)."""
+    FP64 refinement (factor and solve times are the best of reps).  The FP32
+    factorization runs its GEMM tiles as three-piece bf16 splits
+    (the default); the same FP32 path on the FP32 MFMA is timed beside it."""
     A, grid, np_ = S.config_matrix("c3")
     _, b = S.gen_rhs(A, 1, 1)
     b = b[:, 0]
     out = {"workload": "96^3 27-point anisotropic 1:10:100, 512 ND leaves, 1 rhs",
            "n": A.n, "nnz": A.nnz}
     xs = {}
-    for prec in ("fp64", "fp32"):
-        print(f"precision leg: {prec}", file=sys.stderr, flush=True)
+    for key, prec, form in (("fp64", "fp64", "split"), ("fp32", "fp32", "split"),
+                            ("fp32_f32_mfma", "fp32", "f32")):
+        print(f"precision leg: {key}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         # refinement stops at the metric's backward-error target (1e-12)
-        M = S.SnuMat(A, np_, grid=grid, device=0, precision=prec, ir_tol=1e-12)
+        M = S.SnuMat(A, np_, grid=grid, device=0, precision=prec, ir_tol=1e-12, fp32_gemm=form)
         construct = time.perf_counter() - t0
         d_b = S.DeviceBuffer(8 * A.n)
         d_x = S.DeviceBuffer(8 * A.n)
@@ -282,23 +285,38 @@ def precision_leg(S, np, reps=3):
         M.factorize()
         M.set_profiling(False)
         pt = M.timings()
+        gs = M.gemm_split()
         x = np.zeros(A.n)
         d_x.download(x)
-        xs[prec] = x
-        out[prec] = {"factor_ms": round(min(fms), 3), "solve_ms": round(min(sms), 3), "phases_ms": phases,
+        xs[key] = x
+        out[key] = {"factor_ms": round(min(fms), 3), "solve_ms": round(min(sms), 3), "phases_ms": phases,
                      "factorize_solve_per_s": round(1e3 / (min(fms) + min(sms)), 4),
                      "backward_error": S.backward_error(A, x, b), "construct_s": round(construct, 2),
                      "device_gb": round(M.info()["device_bytes"] / 1e9, 2),
                      "schur_gemm": {"ms": round(pt["gemm_sub_ms"], 3), "launches": int(pt["gemm_sub_launches"]),
                                     "tflops": round(pt["gemm_sub_gflop"] / max(pt["gemm_sub_ms"], 1e-9), 3)}}
+        for t in (64, 128):
+            tt = {"ms": round(gs[f"t{t}_ms"], 3), "launches": int(gs[f"t{t}_launches"]),
+                  "tflops": round(gs[f"t{t}_gflop"] / max(gs[f"t{t}_ms"], 1e-9), 3)}
+            if prec == "fp32" and form == "split":
+                # six bf16 products per FP32 multiply-add: the MFMA work executed
+                tt["form"] = "FP32 operands as three bf16 pieces, six products on v_mfma_f32_16x16x32_bf16"
+                tt["bf16_mfma_tflops"] = round(6 * tt["tflops"], 3)
+                tt["frac_of_bf16_peak"] = round(6 * tt["tflops"] / 2500.0, 4)
+                tt["fp32_equiv_frac_of_fp32_peak"] = round(tt["tflops"] / 157.3, 4)
+            out[key]["schur_gemm"][f"tile{t}"] = tt
         if prec == "fp32":
-            out[prec]["ir_steps"] = int(steps)
-            out[prec]["ir_backward_error"] = ibe
+            out[key]["ir_steps"] = int(steps)
+            out[key]["ir_backward_error"] = ibe
         M.close()
         d_b.free()
         d_x.free()
     out["rel_diff_fp32ir_vs_fp64"] = float(np.max(np.abs(xs["fp32"] - xs["fp64"])) / np.max(np.abs(xs["fp64"])))
     out["speedup_fp32ir_vs_fp64"] = round(out["fp32"]["factorize_solve_per_s"] / out["fp64"]["factorize_solve_per_s"], 3)
+    out["speedup_fp32ir_f32_mfma_vs_fp64"] = round(
+        out["fp32_f32_mfma"]["factorize_solve_per_s"] / out["fp64"]["factorize_solve_per_s"], 3)
+    out["rel_diff_split_vs_f32_mfma"] = float(np.max(np.abs(xs["fp32"] - xs["fp32_f32_mfma"]))
+                                              / np.max(np.abs(xs["fp64"])))
     return out
 
 

@@ -83,8 +83,8 @@ typedef struct {
                       factorized on first need, kept until the next
                       factorize; about the memory of an FP64 handle) */
   double ir_tol;   /* SNU_FP32: target backward error per column (1e-15) */
-  int fp32_gemm;   /* SNU_FP32: arithmetic of the factorization's 128 x 128
-                      GEMM tiles.  SNU_GEMM_SPLIT (default): each FP32 operand
+  int fp32_gemm;   /* SNU_FP32: arithmetic of the factorization's dense GEMM
+                      tiles.  SNU_GEMM_SPLIT (default): each FP32 operand
                       as three bf16 pieces (h + m + l == x exactly), the six
                       products of weight >= 2^-16 on the bf16 MFMA, FP32
                       accumulation: FP32-accurate (terms left out <= about
@@ -168,8 +168,9 @@ int snumat_destroy(snumat_t *h);
 
 /* ---------------- batches (no reference counterpart) ----------------
  * A batch runs the factorization and the solves of several constructed
- * matrices (same device and precision) as one launch per step across all of
- * them: the leaves of all matrices share the leaf launches, separators of
+ * matrices (same device, precision and, in FP32, fp32_gemm) as one launch
+ * per step across all of them: the leaves of all matrices share the leaf
+ * launches, separators of
  * equal tree height (counted from the leaves) share one grouped launch per
  * dense step.  Each matrix's arithmetic is that of its standalone
  * factorization; after snubatch_factorize every member is factored and may
@@ -380,7 +381,7 @@ int snumat_use_graph(snumat_t *h, int on);
  * 1: C = A B, 3: C = -A B, 4: C = A B (128 x 128 tiles), 5: C = -A B
  * (128 x 128 tiles); col-major.  snu_test_gemm_f32 also takes 6: C -= A B,
  * 7: C = A B, 8: C = -A B through the FP32 bf16-split kernel (128 x 128
- * tiles, as SNU_GEMM_SPLIT). */
+ * tiles, as SNU_GEMM_SPLIT) and 9, 10, 11 the same on 64 x 64 tiles. */
 int snu_test_gemm(int M, int N, int K, const double *A, int lda,
                   const double *B, int ldb, double *C, int ldc, int assign);
 int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,
@@ -388,7 +389,8 @@ int snu_test_gemm_f32(int M, int N, int K, const float *A, int lda,
 /* average time (ms) of the grouped C -= A B kernel on device-resident
  * operands, K split into nseg concatenated segments; tile 64 or 128 (K chunk
  * 16), 6432 / 12832 (K chunk 32) or 642 / 1282 (K chunk 16, loads two chunks
- * ahead), or 1283 (SNU_FP32 only: 128 x 128 through bf16 splits); prec
+ * ahead), or 1283 / 643 (SNU_FP32 only: 128 x 128 / 64 x 64 through bf16
+ * splits); prec
  * SNU_FP64 or SNU_FP32. */
 int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                    double *ms_out);

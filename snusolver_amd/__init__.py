@@ -248,7 +248,7 @@ class SnuMat:
                  fp32_gemm: str = "split"):
         """precision "fp64": the reference's arithmetic; "fp32": FP32 factors
         and sweeps with FP64 iterative refinement (ir_max_steps, ir_tol).
-        fp32_gemm ("fp32" only): "split", the 128 x 128 GEMM tiles as
+        fp32_gemm ("fp32" only): "split", the factorization's GEMM tiles as
         three-piece bf16 splits on the bf16 MFMA (FP32-accurate), or "f32",
         the FP32 MFMA."""
         self.A = A

@@ -923,7 +923,7 @@ template <typename T>
 int test_gemm(int M, int N, int K, const T *A, int lda, const T *B, int ldb, T *C, int ldc,
               int assign) {
   if (M < 0 || N < 0 || K < 0 || lda < std::max(1, M) || ldb < std::max(1, K) || ldc < std::max(1, M) ||
-      assign < 0 || assign > (std::is_same<T, float>::value ? 8 : 5))
+      assign < 0 || assign > (std::is_same<T, float>::value ? 11 : 5))
     return set_error(SNU_ERR_BAD_INPUT, "bad gemm shape");
   T *dA, *dB, *dC;
   size_t na = (size_t)lda * std::max(K, 1), nb = (size_t)ldb * std::max(N, 1), nc = (size_t)ldc * std::max(N, 1);
@@ -938,7 +938,7 @@ int test_gemm(int M, int N, int K, const T *A, int lda, const T *B, int ldb, T *
   int tiles = 0;
   double fl = 0;
   add_gemm<T>(gp, gs, tiles, dC, ldc, M, N, {GemmSegT<T>{dA, dB, lda, ldb, K}}, fl,
-              assign == 2 || assign >= 4 ? 128 : 64);
+              assign == 2 || (assign >= 4 && assign <= 8) ? 128 : 64);
   GemmProbT<T> *dgp;
   GemmSegT<T> *dgs;
   HIPCHK(hipMalloc((void **)&dgp, sizeof(GemmProbT<T>) * std::max<size_t>(1, gp.size())));
@@ -947,8 +947,10 @@ int test_gemm(int M, int N, int K, const T *A, int lda, const T *B, int ldb, T *
     HIPCHK(hipMemcpy(dgp, gp.data(), sizeof(GemmProbT<T>) * gp.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dgs, gs.data(), sizeof(GemmSegT<T>) * gs.size(), hipMemcpyHostToDevice));
     if (assign >= 6) {  // FP32 only (checked above)
-      if constexpr (std::is_same<T, float>::value)
-        launch_gemm128_split(dgp, (int)gp.size(), dgs, tiles, assign - 6, nullptr);
+      if constexpr (std::is_same<T, float>::value) {
+        if (assign <= 8) launch_gemm128_split(dgp, (int)gp.size(), dgs, tiles, assign - 6, nullptr);
+        else launch_gemm_split(dgp, (int)gp.size(), dgs, tiles, assign - 9, nullptr);
+      }
     } else if (assign == 2 || assign >= 4)
       launch_gemm128<T>(dgp, (int)gp.size(), dgs, tiles, assign == 4 ? 1 : assign == 5 ? 2 : 0, nullptr);
     else launch_gemm<T>(dgp, (int)gp.size(), dgs, tiles, assign == 1 ? 1 : assign == 3 ? 2 : 0, nullptr);
@@ -991,7 +993,7 @@ int test_diag_lu(int nb, T *D, int ld, T *Linv, T *Uinv) {
 // C = -A B (3), C = A B with 128 x 128 tiles (4), C = -A B with 128 x 128
 // tiles (5), one segment, through the grouped MFMA kernel; col-major.  FP32
 // also: C -= A B (6), C = A B (7), C = -A B (8) through the bf16-split
-// 128 x 128 kernel.
+// 128 x 128 kernel, and (9, 10, 11) through its 64 x 64 form.
 extern "C" int snu_test_gemm(int M, int N, int K, const double *A, int lda,
                              const double *B, int ldb, double *C, int ldc,
                              int assign) {
@@ -1081,7 +1083,8 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, bool 
   auto run = [&]() {
     if constexpr (std::is_same<T, float>::value) {
       if (split) {
-        launch_gemm128_split(dgp, 1, dgs, tiles, 0, nullptr);
+        if (tile == 128) launch_gemm128_split(dgp, 1, dgs, tiles, 0, nullptr);
+        else launch_gemm_split(dgp, 1, dgs, tiles, 0, nullptr);
         return;
       }
     }
@@ -1109,13 +1112,13 @@ int bench_gemm(int M, int N, int K, int nseg, int tile, int kb, int depth, bool 
 extern "C" int snu_bench_gemm(int M, int N, int K, int nseg, int tile, int prec, int reps,
                               double *ms_out) {
   // tile: 64 or 128 (K chunk 16), 6432 / 12832 (K chunk 32), 642 / 1282
-  // (K chunk 16, loads two chunks ahead), 1283 (FP32: 128 x 128 through
-  // bf16 splits)
+  // (K chunk 16, loads two chunks ahead), 1283 / 643 (FP32: through bf16
+  // splits)
   int t = tile, kb = 16, depth = 1;
-  const bool split = tile == 1283;
+  const bool split = tile == 1283 || tile == 643;
   if (tile == 6432 || tile == 12832) t = tile / 100, kb = 32;
-  if (tile == 642 || tile == 1282 || split) t = tile / 10, depth = 2;
-  if (split) depth = 1;
+  if (tile == 642 || tile == 1282) t = tile / 10, depth = 2;
+  if (split) t = tile / 10;
   if (M < 1 || N < 1 || K < 1 || nseg < 1 || reps < 1 || (t != 64 && t != 128) ||
       (prec != SNU_FP64 && prec != SNU_FP32) || (split && prec != SNU_FP32) || !ms_out)
     return set_error(SNU_ERR_BAD_INPUT, "bad gemm bench shape");

@@ -324,8 +324,9 @@ extern "C" int snubatch_create(snumat_t *const *mats, int count, snubatch_t **ou
   *out = nullptr;
   for (int m = 0; m < count; m++) {
     if (!mats[m] || !mats[m]->on_device) return set_error(SNU_ERR_BAD_INPUT, "batch member not on a device");
-    if (mats[m]->prec != mats[0]->prec || mats[m]->device != mats[0]->device)
-      return set_error(SNU_ERR_BAD_INPUT, "batch members must share device and precision");
+    if (mats[m]->prec != mats[0]->prec || mats[m]->device != mats[0]->device ||
+        (mats[m]->prec == SNU_FP32 && mats[m]->opts.fp32_gemm != mats[0]->opts.fp32_gemm))
+      return set_error(SNU_ERR_BAD_INPUT, "batch members must share device, precision and FP32 GEMM form");
     for (int q = 0; q < m; q++)
       if (mats[q] == mats[m]) return set_error(SNU_ERR_BAD_INPUT, "matrix listed twice in a batch");
   }

@@ -278,7 +278,7 @@ struct FactorSet {
   const WorkDev *wd;
   const void *fronts;  // FrontPlan of the set's value type
   double eps;
-  bool split = false;  // FP32: 128 x 128 GEMM tiles through bf16 splits
+  bool split = false;  // FP32: GEMM tiles through bf16 splits (SNU_GEMM_SPLIT)
 };
 // graph (optional, not with profile): capture the launches on the first
 // call, replay them afterwards; the phase events then all mark the end.

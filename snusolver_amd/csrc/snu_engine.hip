@@ -206,6 +206,27 @@ namespace {
 // threshold of 512, measured); the rest take 64 x 64 tiles (launch kind 5
 // and kind 2).
 constexpr int kMin128Tiles = 4096;
+// FP32 factorizations whose 128 x 128 tiles run as bf16 splits
+// (SNU_GEMM_SPLIT: about twice the rate of the FP32 64 x 64 tile) take the
+// large tile for every problem with min(M, N) >= 128, from kMin128Split
+// tiles per launch.
+constexpr int kMin128Split = 512;
+struct TileRule {
+  int prec = SNU_FP64;
+  bool split = false;
+  int pick(int M, int N) const {
+    if (split) return std::min(M, N) >= 128 ? 128 : 64;
+    return tile_for(prec, M, N);
+  }
+  long long min128() const { return split ? kMin128Split : kMin128Tiles; }
+};
+template <typename T>
+TileRule tile_rule(const snumat *h) {
+  TileRule r;
+  r.prec = std::is_same<T, float>::value ? SNU_FP32 : SNU_FP64;
+  r.split = r.prec == SNU_FP32 && h->opts.fp32_gemm == SNU_GEMM_SPLIT;
+  return r;
+}
 template <typename T>
 struct GemmBatch {
   struct Pend {
@@ -215,21 +236,21 @@ struct GemmBatch {
     std::vector<GemmSegT<T>> segs;
   };
   std::vector<Pend> pend;
+  TileRule rule;
   void add(T *C, long long ldc, int M, int N, const std::vector<GemmSegT<T>> &segs) {
     if (M > 0 && N > 0) pend.push_back(Pend{C, ldc, M, N, segs});
   }
   void flush(std::vector<GemmProbT<T>> &gp, std::vector<GemmSegT<T>> &gs, std::vector<Launch> &out,
              long long key) {
-    const int prec = std::is_same<T, float>::value ? SNU_FP32 : SNU_FP64;
     long long big = 0;
     for (const Pend &q : pend)
-      if (tile_for(prec, q.M, q.N) == 128) big += (long long)((q.M + 127) / 128) * ((q.N + 127) / 128);
-    const bool use128 = big >= kMin128Tiles;
+      if (rule.pick(q.M, q.N) == 128) big += (long long)((q.M + 127) / 128) * ((q.N + 127) / 128);
+    const bool use128 = big >= rule.min128();
     std::vector<GemmProbT<T>> p[2];
     int tiles[2] = {0, 0};
     double flops[2] = {0, 0};
     for (const Pend &q : pend) {
-      const int t = use128 && tile_for(prec, q.M, q.N) == 128 ? 1 : 0;
+      const int t = use128 && rule.pick(q.M, q.N) == 128 ? 1 : 0;
       add_gemm(p[t], gs, tiles[t], q.C, q.ldc, q.M, q.N, q.segs, flops[t], t ? 128 : 64);
     }
     pend.clear();
@@ -323,7 +344,7 @@ void build_dense_plan(snumat *h) {
   using GemmSeg = GemmSegT<T>;
   using DiagProb = DiagProbT<T>;
   const Plan &P = h->P;
-  const int prec = std::is_same<T, float>::value ? SNU_FP32 : SNU_FP64;
+  const TileRule rule = tile_rule<T>(h);
   Values<T> &VV = vals<T>(h);
   T *dense = VV.arena + P.leaf_total;
   auto R = [&](int p) { return dense + P.offR[p]; };
@@ -387,6 +408,7 @@ void build_dense_plan(snumat *h) {
       // trailing block D[K1:, K1:] as one product with K = K1 - K0 (a K = 64
       // update reads and writes the whole trailing block per step)
       GemmBatch<T> tu;
+      tu.rule = rule;
       const int K0 = k - k % kPanel;
       for (int i = lo; i <= hi; i++) {
         const int s = P.block_size[i];
@@ -419,10 +441,10 @@ void build_dense_plan(snumat *h) {
         const int s = P.block_size[i];
         for (int r0 = 0; r0 + hh < s; r0 += 2 * hh) {
           const int hb = std::min(hh, s - r0 - hh);
-          if (tile_for(prec, hh, hb) == 128) big += 2LL * ((hh + 127) / 128) * ((hb + 127) / 128);
+          if (rule.pick(hh, hb) == 128) big += 2LL * ((hh + 127) / 128) * ((hb + 127) / 128);
         }
       }
-      const int W = big >= kMin128Tiles ? 128 : 64;
+      const int W = big >= rule.min128() ? 128 : 64;
       t1.kind = W == 128 ? 10 : 1;
       t2.kind = W == 128 ? 11 : 6;
       for (int i = lo; i <= hi; i++) {
@@ -486,10 +508,10 @@ void build_dense_plan(snumat *h) {
       long long big = 0;
       for (int i = lo; i <= hi; i++) {
         const int s = P.block_size[i], nl = P.nL[i], nu = P.nU[i];
-        if (nl && tile_for(prec, nl, s) == 128) big += (long long)((nl + 127) / 128) * ((s + 127) / 128);
-        if (nu && tile_for(prec, s, nu) == 128) big += (long long)((s + 127) / 128) * ((nu + 127) / 128);
+        if (nl && rule.pick(nl, s) == 128) big += (long long)((nl + 127) / 128) * ((s + 127) / 128);
+        if (nu && rule.pick(s, nu) == 128) big += (long long)((s + 127) / 128) * ((nu + 127) / 128);
       }
-      const int W = big >= kMin128Tiles ? 128 : 64;
+      const int W = big >= rule.min128() ? 128 : 64;
       Launch pm{W == 128 ? 10 : 1, (int)h_gp.size(), 0, 0, 0, fkey(height, 60000000LL)};
       std::vector<GemmProbT<T>> cp;
       Launch cl{3, 0, 0, 0, 0, fkey(height, 60000001LL)};
@@ -534,6 +556,7 @@ void build_dense_plan(snumat *h) {
     // contributors that store both the L-panel rows and the U-panel columns
     // of that tile (the other products are exactly zero).
     GemmBatch<T> sc;
+    sc.rule = rule;
     for (int p = 1; p < lo; p++) {
       int lp = 31 - __builtin_clz(p);
       int first = p << (l - lp), last = ((p + 1) << (l - lp)) - 1;
@@ -1060,6 +1083,7 @@ int build_front_plan(snumat *h) {
       Launch d{0, (int)DP.dp.size(), 0, 0, 0, front_key(hh, step, 0)};
       Launch tr{1, (int)DP.gp.size(), 0, 0, 0, front_key(hh, step, 1)};
       GemmBatch<T> tu;
+      tu.rule = tile_rule<T>(h);
       for (const FrontDev &D : lg[hh]) {
         if (D.nk <= k) continue;
         const int f = D.f, nb = std::min(64, D.nk - k), m = f - k - nb;
@@ -1518,6 +1542,9 @@ void free_device(snumat *h) {
 // executors
 // ---------------------------------------------------------------------------
 namespace {
+// SNU_GEMM_SPLIT factorizations run their 64 x 64 GEMM tiles as bf16 splits
+// too
+constexpr bool kSplit64 = true;
 template <typename T>
 void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
                 const DiagProbT<T> *dp, const GemvProbT<T> *vp, const GemvSegT<T> *vs,
@@ -1525,6 +1552,10 @@ void run_launch(const Launch &L, const GemmProbT<T> *gp, const GemmSegT<T> *gs,
   if constexpr (std::is_same<T, float>::value) {
     if (split && (L.kind == 5 || L.kind == 10 || L.kind == 11)) {
       launch_gemm128_split(gp + L.p0, L.np, gs, L.tiles, L.kind == 5 ? 0 : L.kind == 10 ? 1 : 2, st);
+      return;
+    }
+    if (split && kSplit64 && (L.kind == 1 || L.kind == 2 || L.kind == 6)) {
+      launch_gemm_split(gp + L.p0, L.np, gs, L.tiles, L.kind == 2 ? 0 : L.kind == 1 ? 1 : 2, st);
       return;
     }
   }

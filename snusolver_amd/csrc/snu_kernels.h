@@ -186,6 +186,9 @@ void launch_gemm128(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *seg
 // (FP32-accurate; modes as launch_gemm128, same tile counts)
 void launch_gemm128_split(const GemmProbT<float> *probs, int nprob, const GemmSegT<float> *segs,
                           int total_tiles, int mode, hipStream_t st);
+// the same on 64 x 64 tiles (modes and tile counts of launch_gemm)
+void launch_gemm_split(const GemmProbT<float> *probs, int nprob, const GemmSegT<float> *segs,
+                       int total_tiles, int mode, hipStream_t st);
 template <typename T>
 void launch_gemm_variant(const GemmProbT<T> *probs, int nprob, const GemmSegT<T> *segs,
                          int total_tiles, int tile, int kb, int depth, hipStream_t st);

@@ -469,7 +469,8 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
   }
 }
 // ---------------------------------------------------------------------------
-// FP32 grouped GEMM on the bf16 MFMA (v_mfma_f32_16x16x32_bf16)
+// FP32 grouped GEMM on the bf16 MFMA (v_mfma_f32_16x16x32_bf16; the
+// SNU_GEMM_SPLIT form of the FP32 factorization's GEMM launches)
 // ---------------------------------------------------------------------------
 // Every FP32 operand x is split into three bf16 pieces with x = h + m + l
 // exactly: h = bf16(x), m = bf16(x - h), l = bf16(x - h - m), each rounded
@@ -481,14 +482,15 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProbT<T> *__restrict__ p
 // 2^-23 |a| |b|, two units of FP32's own rounding of one product.  Per
 // 16 x 16 x 32 block that is six bf16 MFMAs (16 cycles each) in place of
 // eight 16x16x4 f32 MFMAs (32 cycles each).
-// Workgroup tile 128 x 128, 4 waves in 2 x 2, each 64 x 64; K chunk 32
-// (one bf16 MFMA deep).  LDS holds one chunk, pieces per row of A and per
-// column of B as [h 0..31 | m 0..31 | l 0..31 | pad 8] bf16 (52 dwords: the
-// 16-byte reads of 8 consecutive rows fall on disjoint banks); the next
-// chunk is in flight in registers during the MFMA block, and two workgroups
-// share a CU, so one splits and stores while the other multiplies.
+// Workgroup tile (32 BMB) x (32 BNB) (128 x 128 or 64 x 64), 4 waves in
+// 2 x 2; K chunk 32 (one bf16 MFMA deep).  LDS holds one chunk, pieces per
+// row of A and per column of B as [h 0..31 | m 0..31 | l 0..31 | pad 8] bf16
+// (52 dwords: the 16-byte reads of 8 consecutive rows fall on disjoint
+// banks); the next chunk is in flight in registers during the MFMA block,
+// and several workgroups share a CU (2 of the large tile, 4 of the small),
+// so one splits and stores while another multiplies.
 // Tiles, problems, modes, aliasing and the XCD order are those of k_gemm
-// with 128 x 128 tiles (the same tile counts).
+// with the same tile size (the same tile counts).
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 typedef float fl2_t __attribute__((ext_vector_type(2)));
@@ -505,12 +507,12 @@ __device__ __forceinline__ void split_pair(float x0, float x1, unsigned &h, unsi
   m = __builtin_bit_cast(unsigned, mb);
   l = __builtin_bit_cast(unsigned, lb);
 }
-template <int MODE>
-__global__ __launch_bounds__(256, 2) void k_gemm_split(const GemmProbT<float> *__restrict__ probs,
-                                                       int nprob,
-                                                       const GemmSegT<float> *__restrict__ segs) {
-  constexpr int BMB = 4, BNB = 4, BM = 128, BN = 128, KB = 32;
-  constexpr int RW = kSplitRow / 2;  // dwords per LDS row
+template <int MODE, int BMB, int BNB>
+__global__ __launch_bounds__(256, BMB * BNB >= 16 ? 2 : 4) void k_gemm_split(
+    const GemmProbT<float> *__restrict__ probs, int nprob, const GemmSegT<float> *__restrict__ segs) {
+  constexpr int BM = 32 * BMB, BN = 32 * BNB, KB = 32;
+  constexpr int NA = BM / 8, NBC = BN / 16;  // A: NA consecutive k per thread; B: NBC columns
+  constexpr int RW = kSplitRow / 2;          // dwords per LDS row
   __shared__ __align__(16) unsigned As[BM * RW];
   __shared__ __align__(16) unsigned Bs[BN * RW];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -529,44 +531,45 @@ __global__ __launch_bounds__(256, 2) void k_gemm_split(const GemmProbT<float> *_
   const int gsz = min(GM, tiles_m - first), li = lt - grp * GM * P.tiles_n;
   const int m0 = (first + li % gsz) * BM, n0 = (li / gsz) * BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int wm = (wave & 1) * 16 * BMB, wn = (wave >> 1) * 16 * BNB;
   const int l16 = lane & 15, lk = lane >> 4;
   f4 acc[BMB][BNB];
 #pragma unroll
   for (int a = 0; a < BMB; a++)
 #pragma unroll
     for (int b = 0; b < BNB; b++) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
-  // A chunk: row m0 + am, k = 16 ak + 0..15 (coalesced over rows); B chunk:
-  // columns n0 + bc + 16 i (i < 8), k = 2 bp, 2 bp + 1 (a lane pair's two
-  // loads cover 8 bytes, 16 lanes one 128-byte column segment).  Addresses
-  // are clamped into the operands as in k_gemm; the K tail of B is zeroed
-  // before the split.
-  const int am = tid & 127, ak = tid >> 7, bp = tid & 15, bc = tid >> 4;
-  float xa[16], xb[16];
+  // A chunk: row m0 + am, k = NA ak + 0..NA-1 (coalesced over rows); B
+  // chunk: columns n0 + bc + 16 i (i < NBC), k = 2 bp, 2 bp + 1 (a lane
+  // pair's two loads cover 8 bytes, 16 lanes one 128-byte column segment).
+  // Addresses are clamped into the operands as in k_gemm; the K tail of B is
+  // zeroed before the split.
+  static_assert(NA % 8 == 0 && 256 % BM == 0, "thread-to-element map");
+  const int am = tid % BM, ak = tid / BM, bp = tid & 15, bc = tid >> 4;
+  float xa[NA], xb[2 * NBC];
   auto load = [&](int sgi, int kk0) {
     const GemmSegT<float> S = segs[P.seg0 + sgi];
     const auto pa = gptr(S.A) + min(m0 + am, P.M - 1);
     if (kk0 + KB <= S.K) {
-      const auto p = pa + (long long)(kk0 + 16 * ak) * S.lda;
+      const auto p = pa + (long long)(kk0 + NA * ak) * S.lda;
 #pragma unroll
-      for (int i = 0; i < 16; i++) xa[i] = p[i * S.lda];
+      for (int i = 0; i < NA; i++) xa[i] = p[i * S.lda];
     } else {
 #pragma unroll
-      for (int i = 0; i < 16; i++) xa[i] = pa[(long long)min(kk0 + 16 * ak + i, S.K - 1) * S.lda];
+      for (int i = 0; i < NA; i++) xa[i] = pa[(long long)min(kk0 + NA * ak + i, S.K - 1) * S.lda];
     }
     const auto pb = gptr(S.B);
     if (kk0 + KB <= S.K && n0 + BN <= P.N) {
       const auto p = pb + kk0 + 2 * bp + (long long)(n0 + bc) * S.ldb;
       const long long st = 16 * S.ldb;
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
+      for (int i = 0; i < NBC; i++) {
         xb[2 * i] = p[i * st];
         xb[2 * i + 1] = p[i * st + 1];
       }
     } else {
       const int k0c = min(kk0 + 2 * bp, S.K - 1), k1c = min(kk0 + 2 * bp + 1, S.K - 1);
 #pragma unroll
-      for (int i = 0; i < 8; i++) {
+      for (int i = 0; i < NBC; i++) {
         const auto p = pb + (long long)min(n0 + bc + 16 * i, P.N - 1) * S.ldb;
         xb[2 * i] = p[k0c];
         xb[2 * i + 1] = p[k1c];
@@ -575,19 +578,19 @@ __global__ __launch_bounds__(256, 2) void k_gemm_split(const GemmProbT<float> *_
   };
   typedef unsigned u4 __attribute__((ext_vector_type(4)));
   auto store = [&](int kvalid) {
-    unsigned h[8], m[8], l[8];
+    unsigned h[NA / 2], m[NA / 2], l[NA / 2];
 #pragma unroll
-    for (int j = 0; j < 8; j++) split_pair(xa[2 * j], xa[2 * j + 1], h[j], m[j], l[j]);
-    unsigned *ra = As + am * RW + 8 * ak;
-    *(u4 *)(ra) = u4{h[0], h[1], h[2], h[3]};
-    *(u4 *)(ra + 4) = u4{h[4], h[5], h[6], h[7]};
-    *(u4 *)(ra + 16) = u4{m[0], m[1], m[2], m[3]};
-    *(u4 *)(ra + 20) = u4{m[4], m[5], m[6], m[7]};
-    *(u4 *)(ra + 32) = u4{l[0], l[1], l[2], l[3]};
-    *(u4 *)(ra + 36) = u4{l[4], l[5], l[6], l[7]};
+    for (int j = 0; j < NA / 2; j++) split_pair(xa[2 * j], xa[2 * j + 1], h[j], m[j], l[j]);
+    unsigned *ra = As + am * RW + (NA / 2) * ak;
+#pragma unroll
+    for (int v = 0; v < NA / 8; v++) {
+      *(u4 *)(ra + 4 * v) = u4{h[4 * v], h[4 * v + 1], h[4 * v + 2], h[4 * v + 3]};
+      *(u4 *)(ra + 16 + 4 * v) = u4{m[4 * v], m[4 * v + 1], m[4 * v + 2], m[4 * v + 3]};
+      *(u4 *)(ra + 32 + 4 * v) = u4{l[4 * v], l[4 * v + 1], l[4 * v + 2], l[4 * v + 3]};
+    }
     const bool z0 = 2 * bp >= kvalid, z1 = 2 * bp + 1 >= kvalid;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < NBC; i++) {
       unsigned hb, mb, lb;
       split_pair(z0 ? 0.f : xb[2 * i], z1 ? 0.f : xb[2 * i + 1], hb, mb, lb);
       unsigned *rb = Bs + (bc + 16 * i) * RW + bp;
@@ -649,12 +652,13 @@ __global__ __launch_bounds__(256, 2) void k_gemm_split(const GemmProbT<float> *_
     k0 = nk0;
   }
   // D = B^T A^T: register r of lane l holds C[l % 16][4 (l / 16) + r]
+  constexpr int AG = BMB > 2 ? 2 : BMB;
 #pragma unroll
-  for (int a0 = 0; a0 < BMB; a0 += 2) {
-    float cold[2][BNB][4];
+  for (int a0 = 0; a0 < BMB; a0 += AG) {
+    float cold[AG][BNB][4];
     if constexpr (MODE == 0) {
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+      for (int a = 0; a < AG; a++)
 #pragma unroll
         for (int b = 0; b < BNB; b++)
 #pragma unroll
@@ -664,7 +668,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_split(const GemmProbT<float> *_
           }
     }
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < AG; a++)
 #pragma unroll
       for (int b = 0; b < BNB; b++)
 #pragma unroll
@@ -684,11 +688,21 @@ void launch_gemm128_split(const GemmProbT<float> *probs, int nprob, const GemmSe
                           int total_tiles, int mode, hipStream_t st) {
   if (total_tiles <= 0 || nprob <= 0) return;
   if (mode == 1)
-    hipLaunchKernelGGL(k_gemm_split<1>, dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm_split<1, 4, 4>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
   else if (mode == 2)
-    hipLaunchKernelGGL(k_gemm_split<2>, dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm_split<2, 4, 4>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
   else
-    hipLaunchKernelGGL(k_gemm_split<0>, dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+    hipLaunchKernelGGL((k_gemm_split<0, 4, 4>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+}
+void launch_gemm_split(const GemmProbT<float> *probs, int nprob, const GemmSegT<float> *segs,
+                       int total_tiles, int mode, hipStream_t st) {
+  if (total_tiles <= 0 || nprob <= 0) return;
+  if (mode == 1)
+    hipLaunchKernelGGL((k_gemm_split<1, 2, 2>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  else if (mode == 2)
+    hipLaunchKernelGGL((k_gemm_split<2, 2, 2>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
+  else
+    hipLaunchKernelGGL((k_gemm_split<0, 2, 2>), dim3(total_tiles), dim3(256), 0, st, probs, nprob, segs);
 }
 
 // Grouped 2-D copy over GEMM descriptors: C (ldc) = A of the problem's first
@@ -819,18 +833,22 @@ __global__ __launch_bounds__(256) void k_gemv(const GemvProbT<T> *__restrict__ p
       auto A = gptr(S.A) + mc;
       auto X = gptr(S.x) + (long long)c0 * S.ldx;
       int k = kb + w;
-      // eight columns of A in flight per wave (the loads are independent of
-      // the accumulation), added in two groups of four
-      for (; k + 28 < ke; k += 32) {
-        T a[8];
+      // NF columns of A in flight per wave (the loads are independent of the
+      // accumulation), added in groups of four: 8 in FP64, 16 in FP32 (the
+      // same bytes in flight)
+      constexpr int NF = sizeof(T) == 4 ? 16 : 8;
+      for (; k + 4 * (NF - 1) < ke; k += 4 * NF) {
+        T a[NF];
 #pragma unroll
-        for (int j = 0; j < 8; j++) a[j] = A[(long long)(k + 4 * j) * S.lda];
+        for (int j = 0; j < NF; j++) a[j] = A[(long long)(k + 4 * j) * S.lda];
 #pragma unroll
         for (int c = 0; c < CG; c++)
           if (c < nc) {
-            auto xc = X + (long long)c * S.ldx;
-            acc[c] += a[0] * xc[k] + a[1] * xc[k + 4] + a[2] * xc[k + 8] + a[3] * xc[k + 12];
-            acc[c] += a[4] * xc[k + 16] + a[5] * xc[k + 20] + a[6] * xc[k + 24] + a[7] * xc[k + 28];
+            auto xc = X + (long long)c * S.ldx + k;
+#pragma unroll
+            for (int j = 0; j < NF; j += 4)
+              acc[c] += a[j] * xc[4 * j] + a[j + 1] * xc[4 * j + 4] + a[j + 2] * xc[4 * j + 8] +
+                        a[j + 3] * xc[4 * j + 12];
           }
       }
       for (; k + 12 < ke; k += 16) {
@@ -907,18 +925,22 @@ __global__ __launch_bounds__(256) void k_gemv_part(const GemvProbT<T> *__restric
       auto A = gptr(S.A) + mc;
       auto X = gptr(S.x) + (long long)c0 * S.ldx;
       int k = kb + w;
-      // eight columns of A in flight per wave (the loads are independent of
-      // the accumulation), added in two groups of four
-      for (; k + 28 < ke; k += 32) {
-        T a[8];
+      // NF columns of A in flight per wave (the loads are independent of the
+      // accumulation), added in groups of four: 8 in FP64, 16 in FP32 (the
+      // same bytes in flight)
+      constexpr int NF = sizeof(T) == 4 ? 16 : 8;
+      for (; k + 4 * (NF - 1) < ke; k += 4 * NF) {
+        T a[NF];
 #pragma unroll
-        for (int j = 0; j < 8; j++) a[j] = A[(long long)(k + 4 * j) * S.lda];
+        for (int j = 0; j < NF; j++) a[j] = A[(long long)(k + 4 * j) * S.lda];
 #pragma unroll
         for (int c = 0; c < CG; c++)
           if (c < nc) {
-            auto xc = X + (long long)c * S.ldx;
-            acc[c] += a[0] * xc[k] + a[1] * xc[k + 4] + a[2] * xc[k + 8] + a[3] * xc[k + 12];
-            acc[c] += a[4] * xc[k + 16] + a[5] * xc[k + 20] + a[6] * xc[k + 24] + a[7] * xc[k + 28];
+            auto xc = X + (long long)c * S.ldx + k;
+#pragma unroll
+            for (int j = 0; j < NF; j += 4)
+              acc[c] += a[j] * xc[4 * j] + a[j + 1] * xc[4 * j + 4] + a[j + 2] * xc[4 * j + 8] +
+                        a[j + 3] * xc[4 * j + 12];
           }
       }
       for (; k + 12 < ke; k += 16) {

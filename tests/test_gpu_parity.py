@@ -170,7 +170,8 @@ def test_gemm_kernel_f32_vs_numpy(S):
 
 def test_gemm_kernel_split_vs_numpy(S):
     """FP32 GEMM through three-piece bf16 splits on the bf16 MFMA (the
-    SNU_GEMM_SPLIT form of the FP32 factorization's 128 x 128 tiles)
+    SNU_GEMM_SPLIT form of the FP32 factorization's GEMM tiles, 128 x 128
+    and 64 x 64)
     against an FP64 product of the same FP32 operands, elementwise within
     (K + 4) u32 (|A||B| + |C|): the FP32 dot-product bound plus two units for
     the piece products left out.  A bf16-accurate product (one piece) misses
@@ -190,13 +191,13 @@ def test_gemm_kernel_split_vs_numpy(S):
         A64, B64, C64 = A.astype(np.float64), B.astype(np.float64), C0.astype(np.float64)
         absprod = np.abs(A64) @ np.abs(B64)
         P = A64 @ B64
-        for assign in (6, 7, 8):
+        for assign in (6, 7, 8, 9, 10, 11):  # 128 x 128, then 64 x 64 tiles
             Cg = C0.copy(order="F")
             rc = L.snu_test_gemm_f32(M, N, K, A.ctypes.data_as(C.c_void_p), M, B.ctypes.data_as(C.c_void_p), K,
                                      Cg.ctypes.data_as(C.c_void_p), M, assign)
             assert rc == 0
-            ref = P if assign == 7 else (-P if assign == 8 else C64 - P)
-            bound = (K + 4) * u32 * (absprod + (np.abs(C64) if assign == 6 else 0)) + 1e-30
+            ref = P if assign in (7, 10) else (-P if assign in (8, 11) else C64 - P)
+            bound = (K + 4) * u32 * (absprod + (np.abs(C64) if assign in (6, 9) else 0)) + 1e-30
             assert np.all(np.abs(Cg - ref) <= bound), (M, N, K, assign, np.max(np.abs(Cg - ref) / bound))
     # the FP64 hook has no split form
     z = np.zeros(1)

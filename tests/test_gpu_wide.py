@@ -253,8 +253,8 @@ def test_c2_full_size_vs_oracle(S, oracle_mod, ordering):
 @pytest.mark.parametrize("fp32_gemm", ["split", "f32"])
 def test_c2_full_size_fp32_vs_oracle(S, oracle_mod, fp32_gemm):
     """64^3 7-point, 128 leaves, benchmark ordering, FP32 factors + FP64
-    refinement with both forms of the FP32 128 x 128 GEMM tiles (bf16
-    splits, FP32 MFMA): x against the oracle's FP64 solution."""
+    refinement with both forms of the FP32 GEMM tiles (bf16 splits, FP32
+    MFMA): x against the oracle's FP64 solution."""
     A, grid, np_ = S.config_matrix("c2")
     order, sizes, _ = S.config_order("c2", A, grid, np_)
     _, b = S.gen_rhs(A, 1, 1)
