@@ -7,11 +7,12 @@
 //
 // usage: snu_driver <matrix.mtx> [--np N] [--out x.mtx] [--seed S]
 //                   [--device D] [--grid nx ny nz] [--nrhs K] [--b b.mtx]
-//                   [--fp32] [--file-order]
+//                   [--fp32] [--fp32-mfma] [--file-order]
 // --b reads one right-hand side from a Matrix Market vector file instead of
 // drawing it; --file-order reads the matrix as the reference's read_matrix
 // does (file order within rows, no symmetric expansion); --fp32 factors in
-// FP32 and refines in FP64.
+// FP32 (GEMM tiles as bf16 splits) and refines in FP64; --fp32-mfma does the
+// same with the GEMM tiles on the FP32 MFMA.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -24,7 +25,7 @@
 int main(int argc, char **argv) {
   if (argc < 2) {
     std::fprintf(stderr, "Usage: %s [matrix-market-filename] [--np N] [--out file] [--seed S] [--device D] "
-                 "[--grid nx ny nz] [--nrhs K] [--b file] [--fp32] [--file-order]\n", argv[0]);
+                 "[--grid nx ny nz] [--nrhs K] [--b file] [--fp32] [--fp32-mfma] [--file-order]\n", argv[0]);
     return 1;
   }
   snu_opts_t o;
@@ -40,6 +41,7 @@ int main(int argc, char **argv) {
     else if (!std::strcmp(argv[i], "--nrhs") && i + 1 < argc) nrhs = std::atoi(argv[++i]);
     else if (!std::strcmp(argv[i], "--b") && i + 1 < argc) bfile = argv[++i];
     else if (!std::strcmp(argv[i], "--fp32")) o.precision = SNU_FP32;
+    else if (!std::strcmp(argv[i], "--fp32-mfma")) o.precision = SNU_FP32, o.fp32_gemm = SNU_GEMM_F32;
     else if (!std::strcmp(argv[i], "--file-order")) flags = SNU_MTX_FILE_ORDER | SNU_MTX_NO_EXPAND;
     else if (!std::strcmp(argv[i], "--grid") && i + 3 < argc) {
       o.ordering = SNU_ORDER_GRID;

@@ -389,7 +389,7 @@ def test_reference_driver_through_compat_shims(S, oracle_mod):
             assert S.backward_error(A, x, gold["b"]) <= BERR_TOL
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp32-mfma"])
 def test_driver_writes_x_vs_oracle(S, oracle_mod, precision):
     """snu_driver: .mtx in (b = A x_true, seed 1), x out; the written x
     against the oracle; also with b read from a file, in file order."""
@@ -400,7 +400,7 @@ def test_driver_writes_x_vs_oracle(S, oracle_mod, precision):
     _, b = S.gen_rhs(A, 1, 1)
     b = b[:, 0]
     xo = oracle_mod.Oracle(A, 4, order, sizes).factor_solve(b)
-    extra = ["--fp32"] if precision == "fp32" else []
+    extra = [] if precision == "fp64" else ["--" + precision]
     with tempfile.TemporaryDirectory() as d:
         xpath, bpath = os.path.join(d, "x.mtx"), os.path.join(d, "b.mtx")
         _write_vec(bpath, b)
