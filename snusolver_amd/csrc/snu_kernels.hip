@@ -1510,14 +1510,18 @@ void launch_leaf_solve(const LeafDev *leaves, int nleaves, const VecDev *vd, T *
   const int cmax = std::max(1, std::min(std::min(max_rhs, 16), (kLds - xs_off) / std::max(col_bytes, 1)));
   const size_t lds = (size_t)xs_off + (size_t)cmax * col_bytes;
   // many leaves (batches): the LDS sized for the largest would leave most
-  // of them waiting for a CU; the global-memory kernels keep them resident
+  // of them waiting for a CU; the global-memory kernels keep them resident.
+  // That pays for single-column FP64 sweeps only (512-matrix batch: 4.50 ms
+  // against 4.67 with the LDS kernel); FP32 sweeps (13.5 against 14.2 ms)
+  // and several columns (96^3 over 512 leaves, 16 columns: 34.7 against 55.2
+  // ms) are faster queued on the LDS kernel
   const void *kern = bwd ? (const void *)k_leaf_solve<T, true> : (const void *)k_leaf_solve<T, false>;
   (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   int dev = 0, cus = 0, per_cu = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kLeafSolveThreads, lds);
-  if ((long long)nleaves > (long long)cus * std::max(per_cu, 1)) {
+  if (sizeof(T) == 8 && max_rhs == 1 && (long long)nleaves > (long long)cus * std::max(per_cu, 1)) {
     if (bwd) launch_leaf_bwd(leaves, nleaves, vd, bp, st);
     else launch_leaf_fwd(leaves, nleaves, vd, bp, st);
     return;

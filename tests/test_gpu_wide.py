@@ -282,12 +282,17 @@ def test_many_leaves_27pt_vs_oracle(S, oracle_mod, precision):
     oracle's (FP64), x against the oracle in FP64 and FP32 + refinement."""
     A = S.gen_grid(48, 48, 48, 27, 1, 0.0, 0)
     order, sizes = S.order_grid(48, 48, 48, 512)
-    _, b = S.gen_rhs(A, 1, 1)
-    b = b[:, 0]
+    _, B = S.gen_rhs(A, 3, 1)
+    b = B[:, 0]
     O = oracle_mod.Oracle(A, 512, order, sizes)
     xo = O.factor_solve(b, NT)
+    XO = O.factor_solve_columns(B, NT)
     with S.SnuMat(A, 512, order=order, sizes=sizes, device=0, precision=precision) as M:
         x = M.factorize().solve(b)
+        # several columns over more leaves than fit on the CUs at once: the
+        # LDS leaf solves queue (one column in FP64 takes the global-memory
+        # leaf solves)
+        X = M.solve(B)
         info = M.info()
         bad = []
         if precision == "fp64":
@@ -299,6 +304,9 @@ def test_many_leaves_27pt_vs_oracle(S, oracle_mod, precision):
     berr = S.backward_error(A, x, b)
     print(f"  rel diff vs oracle {rel:.2e}  backward error {berr:.2e}")
     assert rel <= REL_TOL and berr <= BERR_TOL
+    rel3, berr3 = _rel(X, XO), S.backward_error(A, X, B)
+    print(f"  3 columns: rel diff vs oracle {rel3:.2e}  backward error {berr3:.2e}")
+    assert rel3 <= REL_TOL and berr3 <= BERR_TOL
 
 
 def test_stage1_blocks_vs_oracle(S, oracle_mod):
