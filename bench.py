@@ -16,7 +16,8 @@ of the dominant kernel (the grouped FP64 MFMA GEMM, measured with HIP events
 on its stream) and the CPU baseline (the oracle's MKL host path, timed on
 this host's cores on the same workload).
 
-Two further legs (skippable) report the other forms of the path:
+Three further legs (skippable) report the other forms of the path; every
+FP32 factorization refines to the metric's 1e-12 backward error:
   "precision": 96^3 27-point anisotropic operator (n = 884,736, 512 leaves),
      FP32 factors + FP64 refinement to <= 1e-12 backward error against the
      plain FP64 path (factor and solve times, refinement steps, errors);
@@ -43,6 +44,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "factor time & solve/s at 1e-12 backward error, FP64 vs FP32+IR; p99 latency"
+# FP32 refinement target of every leg (precision, batch, serving): the
+# metric's backward error
+IR_TOL = 1e-12
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_BYTES = 8.0e12
 
@@ -263,7 +267,7 @@ def precision_leg(S, np, reps=3):
         print(f"precision leg: {key}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         # refinement stops at the metric's backward-error target (1e-12)
-        M = S.SnuMat(A, np_, grid=grid, device=0, precision=prec, ir_tol=1e-12, fp32_gemm=form)
+        M = S.SnuMat(A, np_, grid=grid, device=0, precision=prec, ir_tol=IR_TOL, fp32_gemm=form)
         construct = time.perf_counter() - t0
         d_b = S.DeviceBuffer(8 * A.n)
         d_x = S.DeviceBuffer(8 * A.n)
@@ -345,7 +349,7 @@ def serve_leg(S, np, rate, nreq, budget=200e9):
     print("serving leg: admitting 16 matrices", file=sys.stderr, flush=True)
     srv = S.Server(budget_bytes=budget, slots=2, max_cols=256)
     t0 = time.perf_counter()
-    ids = [srv.add(A, np_, grid=g, precision="fp32") for A, g, np_ in mix]
+    ids = [srv.add(A, np_, grid=g, precision="fp32", ir_tol=IR_TOL) for A, g, np_ in mix]
     out["admit_s"] = round(time.perf_counter() - t0, 2)
     mats = {i: A for i, (A, _, _) in zip(ids, mix)}
     arr = np.random.default_rng(5)
@@ -369,7 +373,7 @@ def serve_leg(S, np, rate, nreq, budget=200e9):
     tickets, adds, lock = [], [], threading.Lock()
 
     def admit(A, g, np_):
-        mid = srv.add(A, np_, grid=g, precision="fp32")
+        mid = srv.add(A, np_, grid=g, precision="fp32", ir_tol=IR_TOL)
         with lock:
             mats[mid] = A
 
@@ -434,7 +438,7 @@ def serve_pressure(S, mix, nreq):
     budget = int(0.5 * sum(need))
     print("serving leg: under a binding budget", file=sys.stderr, flush=True)
     srv = S.Server(budget_bytes=budget, slots=2, max_cols=256)
-    ids = [srv.add(A, np_, grid=g, precision="fp32") for A, g, np_ in mix]
+    ids = [srv.add(A, np_, grid=g, precision="fp32", ir_tol=IR_TOL) for A, g, np_ in mix]
     arr = np.random.default_rng(6)
 
     def draw():
@@ -490,7 +494,8 @@ def batch_leg(S, np, count=512):
            "total_n": int(sum(A.n for A, _, _ in mem))}
     for prec in ("fp64", "fp32"):
         print(f"batch leg: {prec}", file=sys.stderr, flush=True)
-        mats = [S.SnuMat(A, np_, grid=(side, side, 1), device=0, precision=prec) for A, side, np_ in mem]
+        mats = [S.SnuMat(A, np_, grid=(side, side, 1), device=0, precision=prec, ir_tol=IR_TOL)
+                for A, side, np_ in mem]
         with S.Batch(mats) as B:
             B.factorize()
             B.solve(bs)
