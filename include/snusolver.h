@@ -280,9 +280,10 @@ int snumat_leaf_front_pivots(const snumat_t *h, int rank, int *k_ptr, int *k);
 int snumat_leaf_values(const snumat_t *h, int rank, double *lu_data);
 /* dense diagonal block of separator e (1 <= e < np), s_e x s_e col-major. */
 int snumat_sep_diag(const snumat_t *h, int e, double *out);
-/* debug: run the factorization up to a stage (1 = leaves and the boundary
- * Schur assembly only, 2 = everything) and read a dense separator block
- * (i, j) of one root path, s_i x s_j col-major. */
+/* debug: run the factorization up to a stage (0 = A's values scattered into
+ * place only, 1 = also the leaves and the boundary Schur assembly,
+ * 2 = everything) and read a dense separator block (i, j) of one root path,
+ * s_i x s_j col-major. */
 int snumat_factorize_stage(snumat_t *h, int stage);
 int snumat_sep_block(const snumat_t *h, int i, int j, double *out);
 /* phase timings of the last factorize / solve in ms (GPU events):

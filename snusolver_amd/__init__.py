@@ -426,8 +426,9 @@ class SnuMat:
         return out.reshape(s, s, order="F")
 
     def factorize_stage(self, stage: int):
-        """Debug: stage 1 runs the leaves and the boundary Schur assembly only
-        (the separator blocks then hold A_SS plus the leaf contributions)."""
+        """Debug: stage 0 scatters A's values into place only (the separator
+        blocks then hold A_SS); stage 1 also runs the leaves and the boundary
+        Schur assembly (A_SS plus the leaf contributions)."""
         _check(lib().snumat_factorize_stage(self._h, stage))
         return self
 

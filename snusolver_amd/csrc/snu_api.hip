@@ -307,6 +307,7 @@ int factorize_values(snumat *h, int stage) {
   fs.fronts = &vals<T>(h).fronts;
   fs.eps = h->opts.eps;
   fs.split = h->opts.fp32_gemm == SNU_GEMM_SPLIT;
+  fs.scatter_only = stage == 0;
   static const DensePlan<T> empty;
   double gms[5] = {0, 0, 0, 0, 0};
   int rc = run_factor<T>(fs, stage >= 2 ? VV.fact : empty, h->stream, &h->ev, h->profile && stage >= 2,
@@ -357,6 +358,7 @@ int factorize_values(snumat *h, int stage) {
 
 int factorize_stage(snumat_t *h, int stage) {
   if (!h) return set_error(SNU_ERR_BAD_INPUT, "null handle");
+  if (stage < 0 || stage > 2) return set_error(SNU_ERR_BAD_INPUT, "stage must be 0, 1 or 2");
   if (!h->on_device) return set_error(SNU_ERR_NO_DEVICE, "handle was built host-only");
   HIPCHK(hipSetDevice(h->device));
   int rc = ensure_stream(h);

@@ -279,6 +279,7 @@ struct FactorSet {
   const void *fronts;  // FrontPlan of the set's value type
   double eps;
   bool split = false;  // FP32: GEMM tiles through bf16 splits (SNU_GEMM_SPLIT)
+  bool scatter_only = false;  // A's values into place, nothing factored
 };
 // graph (optional, not with profile): capture the launches on the first
 // call, replay them afterwards; the phase events then all mark the end.

@@ -1593,6 +1593,12 @@ int enqueue_factor(const FactorSet &fs, const DensePlan<T> &plan, hipStream_t st
   launch_prepare<T>(fs.prep, fs.nmat, fs.max_len, fs.max_nnz, st);
   if (fs.ir) launch_rowabs_max(fs.ir, fs.nmat, fs.max_n, fs.anorm_out, st);
   if (ev) HIPCHK(hipEventRecord((*ev)[1], st));
+  if (fs.scatter_only) {
+    if (ev)
+      for (int q = 2; q <= 5; q++) HIPCHK(hipEventRecord((*ev)[q], st));
+    HIPCHK(hipGetLastError());
+    return SNU_OK;
+  }
   // leaf fronts, height by height: whole fronts in LDS; larger fronts
   // assembled, factored by the dense launches of their height, written back
   const FrontPlan<T> &fp = *(const FrontPlan<T> *)fs.fronts;

@@ -309,6 +309,46 @@ def test_many_leaves_27pt_vs_oracle(S, oracle_mod, precision):
     assert rel3 <= REL_TOL and berr3 <= BERR_TOL
 
 
+def test_stage0_scatter_bitexact(S, oracle_mod):
+    """A's separator entries scattered into the dense blocks (stage 0: no
+    leaf, no assembly) land bit for bit where the ordering puts them: block
+    (p, q) holds A[i, j] at (order[i] - start_p, order[j] - start_q) for every
+    entry with order[i] in separator p and order[j] in separator q, and zero
+    elsewhere (the tile-compressed storage included); block ranges from the
+    oracle's layout."""
+    cases = [((20, 20, 20), 7, 0, 6.1, 16), ((33, 17, 1), 9, 0, 8.1, 8), ((10, 9, 8), 27, 1, 0.0, 8)]
+    for grid, st, rec, dg, np_ in cases:
+        A = S.gen_grid(*grid, st, rec, dg, 0)
+        order, sizes = S.order_grid(*grid, np_)
+        O = oracle_mod.Oracle(A, np_, order, sizes)
+        bs, sz = O.blocks()
+        O.close()
+        order = np.asarray(order)
+        rn = order[np.repeat(np.arange(A.n), np.diff(A.rowptr))]
+        cn = order[A.colidx]
+        nblocks = nonzero = 0
+        with S.SnuMat(A, np_, order=order, sizes=sizes, device=0) as M:
+            M.factorize_stage(0)
+            for i in range(1, np_):
+                anc = []
+                a = i // 2
+                while a >= 1:
+                    anc.append(a)
+                    a //= 2
+                for p, q in [(i, i)] + [(i, a) for a in anc] + [(a, i) for a in anc]:
+                    r0, r1 = bs[p - 1], bs[p - 1] + sz[p - 1]
+                    c0, c1 = bs[q - 1], bs[q - 1] + sz[q - 1]
+                    sel = (rn >= r0) & (rn < r1) & (cn >= c0) & (cn < c1)
+                    e = np.zeros((sz[p - 1], sz[q - 1]))
+                    e[rn[sel] - r0, cn[sel] - c0] = A.data[sel]
+                    g = M.sep_block(p, q)
+                    assert g.shape == e.shape and np.array_equal(g, e), (grid, p, q)
+                    nblocks += 1
+                    nonzero += int(sel.sum())
+        print(f"  {grid} {st}-point np={np_}: {nblocks} blocks bit-equal, {nonzero} entries of A", flush=True)
+        assert nonzero > 0
+
+
 def test_stage1_blocks_vs_oracle(S, oracle_mod):
     """After the leaves and the boundary Schur assembly (stage 1) every dense
     separator block equals the oracle's assembled block (A_SS entries, then
