@@ -276,7 +276,8 @@ int snumat_leaf_fronts(const snumat_t *h, int rank, int *front5, int *idx_ptr,
 /* the pivots K of each front (ascending within a front; k_ptr has
  * fronts + 1 entries, k the leaf size). */
 int snumat_leaf_front_pivots(const snumat_t *h, int rank, int *k_ptr, int *k);
-/* leaf LU values after factorize (copied back from the device). */
+/* leaf LU values after factorize (copied back from the device); after
+ * snumat_factorize_stage(h, 0), A's leaf values in the LU layout. */
 int snumat_leaf_values(const snumat_t *h, int rank, double *lu_data);
 /* dense diagonal block of separator e (1 <= e < np), s_e x s_e col-major. */
 int snumat_sep_diag(const snumat_t *h, int e, double *out);

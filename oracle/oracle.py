@@ -97,6 +97,10 @@ class Oracle:
     def leaf_factor(self, nthreads: int = 1):
         lib().oracle_leaf_factor(self._h, nthreads)
 
+    def leaf_init(self):
+        """A's leaf values scattered into the LU layout, nothing factored."""
+        lib().oracle_leaf_init(self._h)
+
     def times(self):
         t = (C.c_double * 4)()
         lib().oracle_last_times(t)

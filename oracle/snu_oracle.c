@@ -726,6 +726,10 @@ static void run_leaves(struct oracle *o, const double *pb, int mode,
   pthread_mutex_destroy(&P.mu);
 }
 
+void oracle_leaf_init(oracle_t *o) {
+  for (int rk = 0; rk < o->np; rk++) leaf_init_values(o, &o->leaves[rk]);
+}
+
 int oracle_leaf_factor(oracle_t *o, int nthreads) {
   run_leaves(o, NULL, 0, nthreads);
   return 0;

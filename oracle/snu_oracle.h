@@ -41,6 +41,9 @@ void oracle_sep_block(oracle_t *o, int i, int j, double *out);
 
 /* Only the per-leaf numeric factorization (core_run up to core_numfact_v2). */
 int oracle_leaf_factor(oracle_t *o, int nthreads);
+/* Only the value scatter that starts core_run: A's leaf values into the LU
+ * layout (zeros at fill positions), nothing factored. */
+void oracle_leaf_init(oracle_t *o);
 
 /* Which dense backend ran: 1 = MKL (libmkl_rt), 0 = the project's own loops. */
 int oracle_dense_backend(void);

@@ -288,6 +288,7 @@ extern "C" int snumat_update_values(snumat_t *h, const double *values) {
   HIPCHK(hipSetDevice(h->device));
   HIPCHK(hipMemcpy(h->d_aval, values, sizeof(double) * (size_t)h->P.nnz, hipMemcpyHostToDevice));
   h->factored = false;
+  h->staged = false;
   return SNU_OK;
 }
 
@@ -353,6 +354,7 @@ int factorize_values(snumat *h, int stage) {
     }
   }
   h->factored = stage >= 2;
+  h->staged = true;
   return SNU_OK;
 }
 
@@ -793,7 +795,7 @@ extern "C" int snumat_leaf_front_pivots(const snumat_t *h, int rank, int *k_ptr,
 }
 extern "C" int snumat_leaf_values(const snumat_t *h, int rank, double *out) {
   if (!h || rank < 0 || rank >= h->P.np) return set_error(SNU_ERR_BAD_INPUT, "bad rank");
-  if (!h->factored) return set_error(SNU_ERR_NOT_FACTORED, "not factored");
+  if (!h->factored && !h->staged) return set_error(SNU_ERR_NOT_FACTORED, "not factored");
   const LeafPlan &L = h->P.leaves[rank];
   return read_values(h, h->P.leaf_off[rank], (size_t)L.lu_nnz, out);
 }

@@ -149,6 +149,7 @@ struct snumat {
   int prec = SNU_FP64;
   bool on_device = false;
   bool factored = false;
+  bool staged = false;  // a factorize_stage ran on the current values
   int device = -1;
   hipStream_t stream = nullptr;
   std::vector<void *> allocs;

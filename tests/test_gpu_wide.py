@@ -315,13 +315,16 @@ def test_stage0_scatter_bitexact(S, oracle_mod):
     (p, q) holds A[i, j] at (order[i] - start_p, order[j] - start_q) for every
     entry with order[i] in separator p and order[j] in separator q, and zero
     elsewhere (the tile-compressed storage included); block ranges from the
-    oracle's layout."""
+    oracle's layout.  The leaves' LU arrays hold A's leaf values exactly where
+    the oracle's value scatter puts them, zeros at the fill positions."""
     cases = [((20, 20, 20), 7, 0, 6.1, 16), ((33, 17, 1), 9, 0, 8.1, 8), ((10, 9, 8), 27, 1, 0.0, 8)]
     for grid, st, rec, dg, np_ in cases:
         A = S.gen_grid(*grid, st, rec, dg, 0)
         order, sizes = S.order_grid(*grid, np_)
         O = oracle_mod.Oracle(A, np_, order, sizes)
         bs, sz = O.blocks()
+        O.leaf_init()
+        leaves_o = [O.leaf_values(r) for r in range(np_)]
         O.close()
         order = np.asarray(order)
         rn = order[np.repeat(np.arange(A.n), np.diff(A.rowptr))]
@@ -345,7 +348,10 @@ def test_stage0_scatter_bitexact(S, oracle_mod):
                     assert g.shape == e.shape and np.array_equal(g, e), (grid, p, q)
                     nblocks += 1
                     nonzero += int(sel.sum())
-        print(f"  {grid} {st}-point np={np_}: {nblocks} blocks bit-equal, {nonzero} entries of A", flush=True)
+            for r in range(np_):
+                assert np.array_equal(M.leaf_values(r), leaves_o[r]), (grid, r)
+        print(f"  {grid} {st}-point np={np_}: {nblocks} blocks and {np_} leaves bit-equal, "
+              f"{nonzero} separator entries of A", flush=True)
         assert nonzero > 0
 
 
