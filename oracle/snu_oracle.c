@@ -12,7 +12,7 @@
  *   core_numfact_v2, core_trsm, core_update_b, core_GEMM (per element:
  *   the same subtraction sequence), the dense level loop and its MKL calls,
  *   the dense and leaf backward solves, the gather / un-permute.
- * What differs (MPI collapse; documented in the design notes):
+ * What differs (one process holds what the reference spreads over MPI ranks):
  *   - one copy per dense block instead of one per rank, so the sibling
  *     contributions (core_reduction / reduction, SnuMat_factorize.cu:29-77,
  *     kernel_sparse.cu:54-90) are added in ascending rank order;
