@@ -58,6 +58,28 @@ def test_unsorted_duplicate_rejected(S):
     assert e.value.code == -4
 
 
+def test_fp32_gemm_option_checked(S):
+    """fp32_gemm takes SNU_GEMM_SPLIT or SNU_GEMM_F32 only (host-only
+    construct: the check needs no GPU)."""
+    import ctypes as C
+    A = S.gen_grid(6, 6, 1, 5, 0, 4.1, 0)
+    for form in ("split", "f32"):
+        with S.SnuMat(A, 4, device=-1, precision="fp32", fp32_gemm=form):
+            pass
+    with pytest.raises(ValueError):
+        S.SnuMat(A, 4, device=-1, precision="fp32", fp32_gemm="bf16")
+    o = S._Opts()
+    S.lib().snu_default_opts(C.byref(o))
+    assert o.fp32_gemm == S.GEMM_SPLIT
+    o.np, o.device, o.precision, o.fp32_gemm = 4, -1, S.FP32, 7
+    csr = S._Csr(A.n, A.n, A.nnz, A.rowptr.ctypes.data_as(C.POINTER(C.c_int)),
+                 A.colidx.ctypes.data_as(C.POINTER(C.c_int)), A.data.ctypes.data_as(C.POINTER(C.c_double)))
+    h = C.c_void_p()
+    rc = S.lib().snumat_construct(C.byref(csr), C.byref(o), C.byref(h))
+    assert rc == -1 and not h.value, rc
+    assert b"fp32_gemm" in S.lib().snu_last_error()
+
+
 def _write(path, text):
     with open(path, "w") as f:
         f.write(text)
