@@ -310,6 +310,8 @@ def precision_leg(S, np, reps=3):
                 tt["fp32_equiv_frac_of_fp32_peak"] = round(tt["tflops"] / 157.3, 4)
             out[key]["schur_gemm"][f"tile{t}"] = tt
         if prec == "fp32":
+            out[key]["gemm_form"] = ("split: FP32 operands as three bf16 pieces, six products, FP32 accumulation"
+                                     if form == "split" else "f32: FP32 MFMA")
             out[key]["ir_steps"] = int(steps)
             out[key]["ir_backward_error"] = ibe
         M.close()
